@@ -1,0 +1,180 @@
+"""CPU oracle for the MorseHGP3D hot path (test infrastructure only).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package, and only as the checker.  The product
+path never imports it.
+
+The compute lives in ``liboracle.so`` (built from ``oracle.c`` and
+``bigint.c`` by ``oracle/Makefile``): exact dyadic-integer restatements of the
+reference's canonical cloud, Morton LBVH, brute-force k-NN, complete-graph
+EMST and exhaustive critical catalogue.  ``tower.py`` adds the small-n
+exhaustive Gamma hierarchy and the direct Morse tower in exact rationals.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+from fractions import Fraction
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        i64, i32 = ctypes.c_int64, ctypes.c_int
+        p = ctypes.c_void_p
+        _lib.or_canonical_order.argtypes = [p, i64, p]
+        _lib.or_lbvh_snapshot.argtypes = [p, i64, p, p, p, p]
+        _lib.or_knn.argtypes = [p, i64, i64, p, p]
+        _lib.or_emst.argtypes = [p, i64, p]
+        _lib.or_catalog_build.argtypes = [p, i64, i32, i32]
+        _lib.or_catalog_build.restype = i64
+        _lib.or_catalog_get.argtypes = [i64, p, p, p, p, p, p, p, p, p, ctypes.c_char_p,
+                                        ctypes.c_char_p, i32]
+        _lib.or_catalog_level_cmp.argtypes = [i64, i64]
+        _lib.or_support_side.argtypes = [p, i32, i64]
+        _lib.or_support_info.argtypes = [p, i32, p, ctypes.c_char_p, ctypes.c_char_p, i32]
+    return _lib
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise OracleError(f"oracle {what} failed with code {rc}")
+
+
+def _pts(points) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+    return a
+
+
+def canonical_order(points) -> np.ndarray:
+    """perm[point_id] = source index (reference point_cloud.cpp:40-95)."""
+    a = _pts(points)
+    perm = np.zeros(len(a), dtype=np.int64)
+    _check(lib().or_canonical_order(a.ctypes.data, len(a), perm.ctypes.data), "canonical_order")
+    return perm
+
+
+def canonicalize(points) -> np.ndarray:
+    a = _pts(points)
+    a = np.where(a == 0.0, 0.0, a)  # -0 -> +0
+    return np.ascontiguousarray(a[canonical_order(a)])
+
+
+@dataclass
+class LbvhSnapshot:
+    leaves: np.ndarray  # (n, 2) code, point id
+    nodes: np.ndarray  # (2n-1, 10) lo[3], hi[3], left, right, begin, end
+    counters: np.ndarray  # point_count, node_count, max_depth, collision groups, max collision
+    codes_by_id: np.ndarray
+
+
+def lbvh_snapshot(canonical_points) -> LbvhSnapshot:
+    a = _pts(canonical_points)
+    n = len(a)
+    leaves = np.zeros((n, 2), dtype=np.uint64)
+    nodes = np.zeros((2 * n - 1, 10), dtype=np.uint64)
+    counters = np.zeros(5, dtype=np.uint64)
+    codes = np.zeros(n, dtype=np.uint64)
+    _check(lib().or_lbvh_snapshot(a.ctypes.data, n, leaves.ctypes.data, nodes.ctypes.data,
+                                  counters.ctypes.data, codes.ctypes.data), "lbvh_snapshot")
+    return LbvhSnapshot(leaves, nodes, counters, codes)
+
+
+def knn(canonical_points, k: int):
+    a = _pts(canonical_points)
+    out = np.zeros((len(a), k), dtype=np.int64)
+    ties = np.zeros(len(a), dtype=np.uint8)
+    _check(lib().or_knn(a.ctypes.data, len(a), k, out.ctypes.data, ties.ctypes.data), "knn")
+    return out, ties.astype(bool)
+
+
+def emst(canonical_points) -> np.ndarray:
+    a = _pts(canonical_points)
+    n = len(a)
+    edges = np.zeros((max(n - 1, 0), 2), dtype=np.int64)
+    _check(lib().or_emst(a.ctypes.data, n, edges.ctypes.data), "emst")
+    return edges
+
+
+@dataclass(frozen=True)
+class CatalogRecord:
+    support: tuple
+    interior: tuple
+    rank: int
+    level: Fraction
+    degenerate: bool
+    shell_count: int
+    shell_witness: int
+
+    @property
+    def key(self):
+        return (self.support, self.interior, self.rank, self.level, self.degenerate)
+
+
+def catalog(canonical_points, k_max: int, max_support: int = 4) -> list:
+    """Exhaustive catalogue: every well-centred support of size 1..4 whose
+    closed rank is <= s_max, plus relevant extra-shell degeneracies."""
+    a = _pts(canonical_points)
+    L = lib()
+    count = L.or_catalog_build(a.ctypes.data, len(a), k_max, max_support)
+    if count < 0:
+        raise OracleError(f"catalogue failed with code {count}")
+    out = []
+    m = ctypes.c_int(); rank = ctypes.c_int(); nint = ctypes.c_int(); deg = ctypes.c_int()
+    shell = ctypes.c_int(); wit = ctypes.c_int64(); lev = ctypes.c_double()
+    sup = np.zeros(4, dtype=np.int64)
+    interior = np.zeros(16, dtype=np.int64)
+    num = ctypes.create_string_buffer(4096)
+    den = ctypes.create_string_buffer(4096)
+    for i in range(count):
+        _check(L.or_catalog_get(i, ctypes.byref(m), sup.ctypes.data, ctypes.byref(rank),
+                                ctypes.byref(nint), interior.ctypes.data, ctypes.byref(deg),
+                                ctypes.byref(shell), ctypes.byref(wit), ctypes.byref(lev),
+                                num, den, 4096), "catalog_get")
+        out.append(CatalogRecord(
+            support=tuple(int(x) for x in sup[: m.value]),
+            interior=tuple(sorted(int(x) for x in interior[: nint.value])),
+            rank=rank.value,
+            level=Fraction(int(num.value), int(den.value)),
+            degenerate=bool(deg.value),
+            shell_count=shell.value,
+            shell_witness=int(wit.value),
+        ))
+    return out
+
+
+def support_info(ids) -> tuple:
+    """(well_centred, level) of a support on the cloud of the last catalog() call."""
+    arr = np.ascontiguousarray(np.asarray(ids, dtype=np.int64))
+    well = ctypes.c_int()
+    num = ctypes.create_string_buffer(4096)
+    den = ctypes.create_string_buffer(4096)
+    rc = lib().or_support_info(arr.ctypes.data, len(arr), ctypes.byref(well), num, den, 4096)
+    if rc == 2:
+        return None
+    return bool(well.value), Fraction(int(num.value), int(den.value))
+
+
+def support_side(ids, x: int) -> int:
+    arr = np.ascontiguousarray(np.asarray(ids, dtype=np.int64))
+    return int(lib().or_support_side(arr.ctypes.data, len(arr), int(x)))
