@@ -1,0 +1,64 @@
+// Per-cloud device state of the MI355X MorseHGP3D backend.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "radix_sort.hpp"
+
+namespace mhgp {
+
+struct CatalogBuffers;   // supports.hpp
+struct EmstBuffers;      // emst.hpp
+struct HostMirror;       // pipeline.cpp
+struct TowerState;       // tower.cpp
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool poisoned = false;
+  std::string last_error;
+
+  int64_t n = 0;
+  bool lbvh_ready = false;
+
+  // input (source order, AoS) and canonical cloud (PointId order, SoA)
+  DevBuf<double> src;
+  DevBuf<double> X, Y, Z;
+  DevBuf<uint32_t> source_index;  // PointId -> source index
+  DevBuf<uint64_t> key0, key1, key2;
+  DevBuf<uint32_t> perm;
+  DevBuf<uint32_t> status;  // [0] duplicate flag, [1] Morton fallback count, [2..] scratch
+
+  // global AABB: lo[3], hi[3] values and PointId witnesses
+  DevBuf<double> bounds;
+  DevBuf<uint32_t> bound_ids;
+
+  // LBVH: leaves in (Morton code, PointId) order
+  DevBuf<uint64_t> codes;
+  DevBuf<uint32_t> leaf_id;   // Morton position -> PointId
+  DevBuf<uint32_t> leaf_pos;  // PointId -> Morton position
+  DevBuf<double> LX, LY, LZ;  // coordinates in Morton order
+  DevBuf<uint64_t> path;      // midpoint path bits inside equal-code runs
+  DevBuf<Node> nodes;         // n-1 internal nodes, root = 0
+  DevBuf<uint32_t> parent_int, parent_leaf;
+  DevBuf<uint32_t> wit;       // 6 witnesses per internal node (lo xyz, hi xyz)
+  DevBuf<uint32_t> counter;
+  RadixSorter sorter;
+
+  std::unique_ptr<CatalogBuffers> catalog;
+  std::unique_ptr<EmstBuffers> emst;
+  std::unique_ptr<HostMirror> host;
+  std::unique_ptr<TowerState> tower;
+
+  Context();
+  ~Context();
+};
+
+// lbvh.hip
+void canonicalize_and_build(Context &c);
+void export_snapshot(Context &c, uint64_t *leaves, uint64_t *nodes, uint64_t *counters);
+
+}  // namespace mhgp

@@ -1,0 +1,48 @@
+// Host-side orchestration of the device stages (C++ layer under the C ABI).
+#pragma once
+
+#include <stdexcept>
+
+#include "../../include/mhgp.h"
+#include "context.hpp"
+
+namespace mhgp {
+
+struct DuplicateError : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+struct NonFiniteError : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+struct StateError : std::logic_error {
+  using std::logic_error::logic_error;
+};
+
+int64_t device_bytes(const Context &c);
+void upload_host(Context &c, const double *xyz, int64_t n);
+void upload_device(Context &c, const double *d_xyz, int64_t n);
+void require_cloud(const Context &c);
+void canonical_points(Context &c, double *xyz_out, uint32_t *source_index_out);
+
+// knn.hip
+void knn_all(Context &c, int k, uint32_t *d_out, uint8_t *d_flags);
+// k nearest neighbours to host memory, near ties settled exactly on the host
+void knn_host(Context &c, int k, uint32_t *ids_out);
+
+// emst.hip
+void emst_host(Context &c, uint32_t *edges_out);
+
+// supports.hip
+void supports(Context &c, int k_max, mhgp_catalog_info *info);
+void supports_fetch(Context &c, uint32_t *ids, uint32_t *meta, uint32_t *interior, double *level,
+                    int64_t *level_class);
+
+// tower.cpp
+void tower(Context &c, int k_max, mhgp_tower_info *info);
+void tower_fetch(Context &c, int32_t *order, int32_t *kind, int64_t *level_class, double *level,
+                 int64_t *child_offset, int64_t *children);
+
+// exact host mirror of the canonical cloud, loaded lazily
+const class ExactCloud &exact_cloud(Context &c);
+
+}  // namespace mhgp

@@ -1,0 +1,200 @@
+"""Python binding of libmhgp.so, the MI355X MorseHGP3D backend (ctypes over the C ABI).
+
+The compute runs in the HIP library; this module only moves arrays across
+the C ABI declared in include/mhgp.h.  There is no CPU fallback: if the
+library is missing or no GPU is visible, every call raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmhgp.so")
+_lib = None
+
+
+class MhgpError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"mhgp error {code}: {message}")
+        self.code = code
+
+
+E_ARGUMENT, E_DUPLICATE, E_NONFINITE, E_DEVICE, E_POISONED, E_STATE = -1, -2, -3, -4, -5, -6
+
+
+class CatalogInfo(ctypes.Structure):
+    _fields_ = [("event_count", ctypes.c_int64), ("diagnostic_count", ctypes.c_int64),
+                ("candidate_count", ctypes.c_int64), ("fallback_count", ctypes.c_int64),
+                ("anchors_widened", ctypes.c_int64), ("s_max", ctypes.c_int32),
+                ("status", ctypes.c_int32)]
+
+
+class TowerInfo(ctypes.Structure):
+    _fields_ = [("node_count", ctypes.c_int64), ("child_count", ctypes.c_int64),
+                ("descent_steps", ctypes.c_int64), ("orders", ctypes.c_int32),
+                ("status", ctypes.c_int32)]
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MhgpError(-9, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        p, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
+        L.mhgp_last_error.restype = ctypes.c_char_p
+        L.mhgp_ctx_create.argtypes = [i32, ctypes.POINTER(p)]
+        L.mhgp_ctx_destroy.argtypes = [p]
+        L.mhgp_ctx_device_bytes.argtypes = [p]
+        L.mhgp_ctx_device_bytes.restype = i64
+        L.mhgp_cloud_upload.argtypes = [p, p, i64]
+        L.mhgp_cloud_upload_device.argtypes = [p, p, i64]
+        L.mhgp_point_count.argtypes = [p]
+        L.mhgp_point_count.restype = i64
+        L.mhgp_canonical_points.argtypes = [p, p, p]
+        L.mhgp_lbvh_snapshot.argtypes = [p, p, p, p]
+        L.mhgp_knn.argtypes = [p, i32, p]
+        L.mhgp_emst.argtypes = [p, p]
+        L.mhgp_supports.argtypes = [p, i32, ctypes.POINTER(CatalogInfo)]
+        L.mhgp_supports_fetch.argtypes = [p, p, p, p, p, p]
+        L.mhgp_tower.argtypes = [p, i32, ctypes.POINTER(TowerInfo)]
+        L.mhgp_tower_fetch.argtypes = [p, p, p, p, p, p, p]
+        _lib = L
+    return _lib
+
+
+EXPORTED = [
+    "mhgp_last_error", "mhgp_ctx_create", "mhgp_ctx_destroy", "mhgp_ctx_device_bytes",
+    "mhgp_cloud_upload", "mhgp_cloud_upload_device", "mhgp_point_count",
+    "mhgp_canonical_points", "mhgp_lbvh_snapshot", "mhgp_knn", "mhgp_emst", "mhgp_supports",
+    "mhgp_supports_fetch", "mhgp_tower", "mhgp_tower_fetch",
+]
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise MhgpError(rc, lib().mhgp_last_error().decode())
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+@dataclass
+class Catalog:
+    ids: np.ndarray        # (E, 4) uint32
+    size: np.ndarray       # (E,) support size
+    rank: np.ndarray       # (E,) closed rank
+    interior: list         # list of tuples
+    degenerate: np.ndarray
+    level: np.ndarray      # FP64 nearest value
+    level_class: np.ndarray
+    info: CatalogInfo
+
+
+class Context:
+    """One cloud on one GPU (owns a HIP stream and the device state)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().mhgp_ctx_create(device, ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            lib().mhgp_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n(self) -> int:
+        return int(lib().mhgp_point_count(self._h))
+
+    def device_bytes(self) -> int:
+        return int(lib().mhgp_ctx_device_bytes(self._h))
+
+    def upload(self, points) -> None:
+        a = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+        _check(lib().mhgp_cloud_upload(self._h, _ptr(a), len(a)))
+
+    def upload_device(self, dev_ptr: int, n: int) -> None:
+        _check(lib().mhgp_cloud_upload_device(self._h, dev_ptr, n))
+
+    def canonical_points(self):
+        n = self.n
+        xyz = np.zeros((n, 3), dtype=np.float64)
+        src = np.zeros(n, dtype=np.uint32)
+        _check(lib().mhgp_canonical_points(self._h, _ptr(xyz), _ptr(src)))
+        return xyz, src
+
+    def lbvh_snapshot(self):
+        n = self.n
+        leaves = np.zeros((n, 2), dtype=np.uint64)
+        nodes = np.zeros((2 * n - 1, 10), dtype=np.uint64)
+        counters = np.zeros(5, dtype=np.uint64)
+        _check(lib().mhgp_lbvh_snapshot(self._h, _ptr(leaves), _ptr(nodes), _ptr(counters)))
+        return leaves, nodes, counters
+
+    def knn(self, k: int) -> np.ndarray:
+        out = np.zeros((self.n, k), dtype=np.uint32)
+        _check(lib().mhgp_knn(self._h, k, _ptr(out)))
+        return out
+
+    def emst(self) -> np.ndarray:
+        out = np.zeros((max(self.n - 1, 0), 2), dtype=np.uint32)
+        _check(lib().mhgp_emst(self._h, _ptr(out)))
+        return out
+
+    def supports(self, k_max: int, fetch: bool = True):
+        info = CatalogInfo()
+        _check(lib().mhgp_supports(self._h, k_max, ctypes.byref(info)))
+        if not fetch:
+            return info
+        e = info.event_count + info.diagnostic_count
+        smax = max(info.s_max, 2)
+        ids = np.zeros((e, 4), dtype=np.uint32)
+        meta = np.zeros(e, dtype=np.uint32)
+        interior = np.zeros((e, smax - 1), dtype=np.uint32)
+        level = np.zeros(e, dtype=np.float64)
+        cls = np.zeros(e, dtype=np.int64)
+        _check(lib().mhgp_supports_fetch(self._h, _ptr(ids), _ptr(meta), _ptr(interior),
+                                         _ptr(level), _ptr(cls)))
+        size = meta & 0xFF
+        rank = (meta >> 8) & 0xFF
+        nint = (meta >> 16) & 0xFF
+        deg = (meta >> 24) & 0xFF
+        inter = [tuple(int(x) for x in interior[i, : nint[i]]) for i in range(e)]
+        return Catalog(ids, size, rank, inter, deg.astype(bool), level, cls, info)
+
+    def tower(self, k_max: int, fetch: bool = True):
+        info = TowerInfo()
+        _check(lib().mhgp_tower(self._h, k_max, ctypes.byref(info)))
+        if not fetch:
+            return info
+        m = info.node_count
+        order = np.zeros(m, dtype=np.int32)
+        kind = np.zeros(m, dtype=np.int32)
+        cls = np.zeros(m, dtype=np.int64)
+        level = np.zeros(m, dtype=np.float64)
+        off = np.zeros(m + 1, dtype=np.int64)
+        kids = np.zeros(max(info.child_count, 1), dtype=np.int64)
+        _check(lib().mhgp_tower_fetch(self._h, _ptr(order), _ptr(kind), _ptr(cls), _ptr(level),
+                                      _ptr(off), _ptr(kids)))
+        return {"order": order, "kind": kind, "level_class": cls, "level": level,
+                "child_offset": off, "children": kids[: info.child_count], "info": info}

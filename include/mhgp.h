@@ -1,0 +1,142 @@
+/* C ABI of the MI355X-native MorseHGP3D backend (libmhgp.so).
+ *
+ * The backend replaces the reference's GPU launcher seam, the
+ * `detail::*_on_gpu` functions declared in morsehgp3d/src/cuda/<stage>_internal.hpp,
+ * which the reference host adapters (morsehgp3d/src/gpu/<adapter>.cpp) call.  Those are
+ * C++ functions over std::span of POD records; here the same work is exposed
+ * as plain C over opaque contexts, pointers and sizes.
+ *
+ * Conventions
+ *  - Every function returns MHGP_OK (0) or a negative MHGP_E_* code; the
+ *    message of the last failure on the calling thread is mhgp_last_error().
+ *  - A context owns one HIP stream and all device state of one cloud.  Calls
+ *    on one context are synchronous and must be serialised by the caller
+ *    (reference: morsehgp3d/include/morsehgp3d/gpu/morton_lbvh_build.hpp:181-184).
+ *  - Any failure inside a device call poisons the context: every later call
+ *    except mhgp_ctx_destroy returns MHGP_E_POISONED (reference fail-closed
+ *    rule, phase14_morton_lbvh_build_internal.hpp:96-110).
+ *  - Point identifiers are canonical PointIds: ranks in the lexicographic
+ *    binary64 total order of the coordinates (point_cloud.cpp:40-95).
+ */
+#ifndef MHGP_H
+#define MHGP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHGP_OK 0
+#define MHGP_E_ARGUMENT -1
+#define MHGP_E_DUPLICATE -2
+#define MHGP_E_NONFINITE -3
+#define MHGP_E_DEVICE -4
+#define MHGP_E_POISONED -5
+#define MHGP_E_STATE -6
+#define MHGP_E_CAPACITY -7
+#define MHGP_E_UNSUPPORTED -8
+#define MHGP_E_INTERNAL -9
+
+typedef struct mhgp_ctx mhgp_ctx;
+
+/* Message of the last failure on this thread ("" when none). */
+const char *mhgp_last_error(void);
+
+/* Creates a context on HIP device `device` with its own stream. */
+int mhgp_ctx_create(int device, mhgp_ctx **out);
+/* Releases every device buffer of the context. */
+void mhgp_ctx_destroy(mhgp_ctx *ctx);
+/* Device bytes currently held by the context. */
+int64_t mhgp_ctx_device_bytes(const mhgp_ctx *ctx);
+
+/* Uploads n points (xyz interleaved, binary64, source order, host memory),
+ * assigns canonical PointIds and builds the exact Morton LBVH.
+ * Replaces propose_phase14_morton_bins_on_gpu and
+ * build_phase14_morton_lbvh_snapshot_on_gpu
+ * (morsehgp3d/src/cuda/phase14_morton_lbvh_build_internal.hpp:137,149).
+ * Returns MHGP_E_DUPLICATE for repeated points, MHGP_E_NONFINITE for NaN/inf. */
+int mhgp_cloud_upload(mhgp_ctx *ctx, const double *xyz, int64_t n);
+/* Same with the input already in device memory. */
+int mhgp_cloud_upload_device(mhgp_ctx *ctx, const double *d_xyz, int64_t n);
+
+int64_t mhgp_point_count(const mhgp_ctx *ctx);
+
+/* Canonical coordinates (n*3, PointId order) and PointId -> source index. */
+int mhgp_canonical_points(mhgp_ctx *ctx, double *xyz_out, uint32_t *source_index_out);
+
+/* Reference snapshot layout (morsehgp3d/include/morsehgp3d/spatial/lbvh.hpp:
+ * 43-103): leaves 2n u64 (Morton code, PointId) in Morton order; nodes 10 u64
+ * each (lower witness ids xyz, upper witness ids xyz, left, right,
+ * leaf_begin, leaf_end) for 2n-1 nodes in strict postorder, leaves included;
+ * counters: point_count, node_count, maximum_depth,
+ * morton_collision_group_count, maximum_morton_collision_size. */
+int mhgp_lbvh_snapshot(mhgp_ctx *ctx, uint64_t *leaves, uint64_t *nodes, uint64_t *counters);
+
+/* k nearest other points of every point, 1 <= k <= 32, n*k PointIds ordered
+ * by exact squared distance then PointId.  Replaces the top-k proposals of
+ * propose_phase14_facet_top_k_candidates_on_gpu
+ * (phase14_facet_top_k_proposal_internal.hpp:183) for data-point queries;
+ * ties are settled exactly, no proposal is left to the caller. */
+int mhgp_knn(mhgp_ctx *ctx, int k, uint32_t *ids_out);
+
+/* Exact Euclidean minimum spanning tree under kappa = (d^2, u, v): n-1 edges
+ * (u < v) in kappa order.  Replaces propose_k1_boruvka_candidates_on_gpu
+ * (phase5_k1_boruvka_internal.hpp:201) together with the host's exact
+ * Borůvka contraction (morsehgp3d/src/cpu/hierarchy/boruvka.cpp:893). */
+int mhgp_emst(mhgp_ctx *ctx, uint32_t *edges_out);
+
+/* Critical-sphere catalogue for orders 1..k_max (s_max = min(min(k_max,n)+1, n)):
+ * every support of 2..4 points whose circumcentre lies strictly inside its
+ * convex hull, whose sphere holds no further point and whose closed rank is
+ * <= s_max, plus relevant extra-shell diagnostics.  Singletons (rank 1,
+ * level 0) are implicit.  Replaces propose_pair_support_phi_on_gpu
+ * (phase9_pair_support_phi_internal.hpp:110) and the direct pair/higher
+ * support streams (pair_support_stream.cpp:2372, higher_support_stream.cpp:2397)
+ * behind build_exact_direct_support_terminal_facade
+ * (direct_support_terminal.cpp:361). */
+typedef struct {
+  int64_t event_count;       /* regular events (sizes 2..4) */
+  int64_t diagnostic_count;  /* extra-shell diagnostics */
+  int64_t candidate_count;   /* supports examined exactly */
+  int64_t fallback_count;    /* predicates settled by the host exact path */
+  int64_t anchors_widened;   /* anchors whose gather radius had to grow */
+  int32_t s_max;
+  int32_t status;            /* 0 exact, 1 conditional */
+} mhgp_catalog_info;
+
+int mhgp_supports(mhgp_ctx *ctx, int k_max, mhgp_catalog_info *info);
+
+/* Event records of the last mhgp_supports call, in canonical order
+ * (level, rank, support).  ids: 4 per event (unused = UINT32_MAX);
+ * meta: size | rank << 8 | interior_count << 16 | degenerate << 24;
+ * interior: s_max-1 ids per event (unused = UINT32_MAX);
+ * level: FP64 value nearest the exact squared radius;
+ * level_class: dense rank of the exact squared radius (equal classes share it). */
+int mhgp_supports_fetch(mhgp_ctx *ctx, uint32_t *ids, uint32_t *meta, uint32_t *interior,
+                        double *level, int64_t *level_class);
+
+/* hgp_reduced merge-forest tower for orders 1..k_max from the last catalogue
+ * (reference build_exact_direct_morse_forest_journal,
+ * morsehgp3d/src/cpu/hierarchy/direct_morse_forest_journal.cpp:1042). */
+typedef struct {
+  int64_t node_count;
+  int64_t child_count;
+  int64_t descent_steps;
+  int32_t orders;
+  int32_t status;  /* 0 certified for the catalogue, 1 conditional */
+} mhgp_tower_info;
+
+int mhgp_tower(mhgp_ctx *ctx, int k_max, mhgp_tower_info *info);
+
+/* Nodes of the last tower: order, kind (0 order-one birth, 1 reduced birth,
+ * 2 multifusion), level class (as in mhgp_supports_fetch), FP64 level, and a
+ * CSR child list (child_offset has node_count + 1 entries). */
+int mhgp_tower_fetch(mhgp_ctx *ctx, int32_t *order, int32_t *kind, int64_t *level_class,
+                     double *level, int64_t *child_offset, int64_t *children);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
