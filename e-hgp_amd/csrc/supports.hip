@@ -1,0 +1,806 @@
+// Critical-sphere catalogue (supports of 2..4 points, closed rank <= s_max).
+//
+// Observable definition (reference reference/morsehgp3d_oracle/catalog.py and
+// morsehgp3d/src/cpu/hierarchy/critical_catalog.cpp): a support U of 1..4
+// affinely independent points whose circumcentre c (in aff U) lies strictly
+// inside conv U; interior I = points strictly inside the sphere, shell = points
+// on it; an event when shell == U and |I| + |U| <= s_max, an extra-shell
+// diagnostic when shell != U and |I| + |U| <= s_max.
+//
+// Enumeration (this backend's own method, replacing the reference's node-tuple
+// branch-and-bound of pair_support_stream.cpp / higher_support_stream.cpp):
+// every support is enumerated once, from its least PointId u (the anchor).
+//  1. Direction-cell certificate.  For a critical sphere through u with centre
+//     u + r d, every open ball of radius rho tangent at u towards d lies in
+//     the open sphere when rho <= r.  The unit sphere of directions is cut
+//     into 384 cube-map cells; for each cell C the radius rho_C is the
+//     (s_max-1)-th smallest radius at which a gathered point lies strictly
+//     inside every tangent ball of the cell.  A sphere with direction in C and
+//     r >= rho_C therefore holds >= s_max-1 interior points and is not
+//     critical.  Since the centre of a well-centred support lies in the
+//     cloud's AABB, r is also below the AABB exit distance of the cell.
+//  2. Region.  All points of a critical sphere with direction in C lie in
+//     the union of closed tangent balls of radius rho'_C over C; a point's
+//     mask records the cells whose union contains it.  Pairs, triangles and
+//     tetrahedra through u are searched among region members only, and their
+//     interior is counted over the whole region.
+//  3. Tetrahedra come from faces (u, v, w) (the three least ids) by the
+//     pencil of spheres through the face's circumcircle: the apex z must be
+//     among the first s_max-3 points swept on its side, so only those are
+//     tested.
+// Every emitted record is certified with the FP64 predicates of
+// predicates.hpp over the anchor's region; anything undecided goes to the
+// host fallback queue with the anchor's region.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "pipeline.hpp"
+#include "predicates.hpp"
+#include "supports.hpp"
+#include "traverse.hpp"
+
+namespace mhgp {
+
+int64_t catalog_device_bytes(const CatalogBuffers &b) {
+  return b.radius.bytes() + b.gcount.bytes() + b.goff.bytes() + b.glist.bytes() + b.need.bytes() +
+         b.status.bytes() + b.events.bytes() + b.fallbacks.bytes() + b.fb_region.bytes() +
+         b.counters.bytes() + b.cells.bytes() + b.scratch.bytes() + b.anchors.bytes();
+}
+
+namespace {
+
+constexpr int kBlock = 256;
+
+enum : uint32_t { ST_PENDING = 0, ST_DONE = 1, ST_WIDEN = 2, ST_BIG = 3 };
+
+struct Params {
+  const double *X, *Y, *Z;       // canonical coordinates
+  const uint32_t *gcount;
+  const uint64_t *goff;
+  const uint32_t *glist;
+  const double *radius;
+  const double *cells;           // 8 doubles per cell
+  double blo[3], bhi[3];         // cloud AABB
+  int s_max;
+  double *need;
+  uint32_t *status;
+  EventRec *events;
+  uint64_t event_cap;
+  FallbackRec *fallbacks;
+  uint64_t fallback_cap;
+  uint32_t *fb_region;
+  uint64_t fb_region_cap;
+  unsigned long long *counters;  // [0] events [1] fallbacks [2] region words [3] candidates
+  const uint32_t *anchors;
+  uint32_t anchor_count;
+  int big;                       // 1: arrays live in global scratch slabs
+  double *scratch;
+  uint64_t slab_doubles;
+  uint32_t gcap, rcap;
+};
+
+struct Work {
+  double *gx, *gy, *gz;
+  uint32_t *gid;
+  double *rho;
+  uint32_t *ridx;
+  uint64_t *rmask;  // 6 words per member
+  uint32_t *kidx;
+};
+
+__device__ inline V3 gpt(const Work &w, uint32_t i) { return {w.gx[i], w.gy[i], w.gz[i]}; }
+
+__device__ void emit_event(const Params &P, uint32_t u, const uint32_t *sup, int m, int nint,
+                           const uint32_t *interior, double level, double lerr) {
+  unsigned long long slot = atomicAdd(&P.counters[0], 1ull);
+  if (slot >= P.event_cap) return;
+  EventRec &e = P.events[slot];
+  e.ids[0] = u;
+  for (int i = 1; i < 4; ++i) e.ids[i] = i < m ? sup[i - 1] : kInvalid;
+  e.meta = (uint32_t)m | ((uint32_t)(nint + m) << 8) | ((uint32_t)nint << 16);
+  for (int i = 0; i < kMaxInterior; ++i) e.interior[i] = i < nint ? interior[i] : kInvalid;
+  e.level = level;
+  e.level_err = (float)fmin(lerr, 1e-3);
+}
+
+// Queues undecided work and, once per anchor, its region for the host.
+__device__ void emit_fallback(const Params &P, const Work &w, uint32_t u, uint32_t kind,
+                              const uint32_t *ids, int m, uint32_t M, uint32_t *region_slot) {
+  if (*region_slot == kInvalid) {
+    uint32_t expected = kInvalid;
+    // first fallback of this anchor publishes its region
+    unsigned long long off = atomicAdd(&P.counters[2], (unsigned long long)M);
+    if (off + M <= P.fb_region_cap) {
+      for (uint32_t s = 0; s < M; ++s) P.fb_region[off + s] = w.gid[w.ridx[s]];
+      atomicCAS(region_slot, expected, (uint32_t)off);
+    }
+  }
+  unsigned long long slot = atomicAdd(&P.counters[1], 1ull);
+  if (slot >= P.fallback_cap) return;
+  FallbackRec &f = P.fallbacks[slot];
+  f.kind = kind;
+  f.ids[0] = u;
+  for (int i = 1; i < 4; ++i) f.ids[i] = i < m ? ids[i - 1] : kInvalid;
+  f.region_off = *region_slot;
+  f.region_len = M;
+  f.pad = 0;
+}
+
+// Certified classification of the region against a candidate sphere.
+// Returns 1 event, 0 not critical, -1 undecided (host).
+template <class S>
+__device__ int classify(const Params &P, const Work &w, uint32_t M, const S &sph,
+                        const uint32_t *skip, int m, uint32_t *interior, int &nint) {
+  nint = 0;
+  const int cap = P.s_max - m;
+  int undecided = 0;
+  for (uint32_t s = 0; s < M; ++s) {
+    uint32_t gi = w.ridx[s];
+    bool on_support = false;
+    for (int k = 0; k < m - 1; ++k) on_support |= (gi == skip[k]);
+    if (on_support) continue;
+    int side = sph.side(gpt(w, gi));
+    if (side < 0) {
+      if (nint < kMaxInterior) interior[nint] = w.gid[gi];
+      nint++;
+      if (nint > cap) return 0;
+    } else if (side == 0) {
+      undecided++;
+    }
+  }
+  return undecided ? -1 : 1;
+}
+
+__device__ void sort_small(uint32_t *a, int n) {
+  for (int i = 1; i < n; ++i) {
+    uint32_t v = a[i];
+    int j = i - 1;
+    while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; j--; }
+    a[j + 1] = v;
+  }
+}
+
+__device__ void process_pair(const Params &P, const Work &w, uint32_t u, uint32_t M, uint32_t vi,
+                             uint32_t *region_slot) {
+  PairSphere sph;
+  sph.init(gpt(w, vi));
+  uint32_t interior[kMaxInterior];
+  int nint;
+  uint32_t skip[1] = {vi};
+  int r = classify(P, w, M, sph, skip, 2, interior, nint);
+  uint32_t vid = w.gid[vi];
+  if (r == 0) return;
+  atomicAdd(&P.counters[3], 1ull);
+  if (r < 0) {
+    emit_fallback(P, w, u, 2, &vid, 2, M, region_slot);
+    return;
+  }
+  sort_small(interior, nint);
+  emit_event(P, u, &vid, 2, nint, interior, sph.level(), 8 * kCert);
+}
+
+__device__ void process_tet(const Params &P, const Work &w, uint32_t u, uint32_t M, const TriSphere &f,
+                            uint32_t vi, uint32_t wi, uint32_t zi, uint32_t *region_slot) {
+  TetSphere t;
+  int wc = t.init(f.a, f.b, gpt(w, zi));
+  uint32_t ids[3] = {w.gid[vi], w.gid[wi], w.gid[zi]};
+  if (wc < 0) return;
+  atomicAdd(&P.counters[3], 1ull);
+  if (wc == 0) { emit_fallback(P, w, u, 4, ids, 4, M, region_slot); return; }
+  uint32_t interior[kMaxInterior];
+  int nint;
+  uint32_t skip[3] = {vi, wi, zi};
+  int r = classify(P, w, M, t, skip, 4, interior, nint);
+  if (r == 0) return;
+  if (r < 0) { emit_fallback(P, w, u, 4, ids, 4, M, region_slot); return; }
+  sort_small(interior, nint);
+  emit_event(P, u, ids, 4, nint, interior, t.level(), t.level_err());
+}
+
+// Face (u, v, w): its triangle event and the apexes swept first by the
+// pencil of spheres through its circumcircle, on each side.
+__device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_t M, uint32_t vi,
+                             uint32_t wi, uint32_t *region_slot) {
+  const uint64_t *mv = w.rmask;  // masks indexed by region slot, looked up by caller
+  (void)mv;
+  TriSphere f;
+  const int wc = f.init(gpt(w, vi), gpt(w, wi));
+  uint32_t ids[2] = {w.gid[vi], w.gid[wi]};
+  if (ids[0] > ids[1]) { uint32_t t = ids[0]; ids[0] = ids[1]; ids[1] = t; uint32_t q = vi; vi = wi; wi = q; }
+  if (!f.nondegenerate()) {
+    // collinear or undecided: a triangle cannot be a support; an undecided
+    // face goes to the host together with its apexes
+    if (cert_sign(f.D, f.DA) == 0) emit_fallback(P, w, u, 3, ids, 3, M, region_slot);
+    return;
+  }
+  const uint32_t wid = ids[1];
+  const int smax = P.s_max;
+  // pass 1: sides and insideness of every region point, triangle interior,
+  // the (s_max-3) smallest sweep keys of outside points on each side, and
+  // the keys of inside points behind each side.
+  int cin[2] = {0, 0};        // inside S3, ahead of side s (H > 0 for s = 0)
+  int total_in = 0;
+  uint32_t tri_int[kMaxInterior];
+  bool undecided = false;
+  const int K = smax - 3;     // apex must be among the first K sweeps
+  double top[2][8];           // smallest upper bounds of tau for outside points ahead
+  int ntop[2] = {0, 0};
+  double behind[2][16];       // lower bounds of tau for inside points behind side s
+  int nbehind[2] = {0, 0};
+  bool behind_over[2] = {false, false};
+  for (uint32_t s = 0; s < M && !undecided; ++s) {
+    uint32_t gi = w.ridx[s];
+    if (gi == vi || gi == wi) continue;
+    V3 p = gpt(w, gi);
+    double Pw, PA, H, HA;
+    f.power(p, Pw, PA);
+    f.height(p, H, HA);
+    int sp = cert_sign(Pw, PA), sh = cert_sign(H, HA);
+    if (sp == 0 || sh == 0) { undecided = true; break; }
+    if (sp < 0) {
+      if (total_in < kMaxInterior) tri_int[total_in] = w.gid[gi];
+      total_in++;
+    }
+    int side = sh > 0 ? 0 : 1;  // side the point is ahead of
+    double tau = Pw / fabs(H);  // sweep key towards its own side
+    double terr = (kCert * PA / fabs(Pw) + kCert * HA / fabs(H) + 1e-15) * fabs(tau);
+    if (sp < 0) {
+      cin[side]++;
+      // behind the opposite side: inside until the sweep key reaches -tau
+      int o = 1 - side;
+      if (nbehind[o] < 16) behind[o][nbehind[o]++] = -tau - terr;
+      else behind_over[o] = true;
+    } else {
+      double hi = tau + terr;
+      if (ntop[side] < K) {
+        int j = ntop[side]++;
+        while (j > 0 && top[side][j - 1] > hi) { top[side][j] = top[side][j - 1]; j--; }
+        top[side][j] = hi;
+      } else if (K > 0 && hi < top[side][K - 1]) {
+        int j = K - 1;
+        while (j > 0 && top[side][j - 1] > hi) { top[side][j] = top[side][j - 1]; j--; }
+        top[side][j] = hi;
+      }
+    }
+  }
+  if (undecided) {
+    emit_fallback(P, w, u, 3, ids, 3, M, region_slot);
+    return;
+  }
+  // triangle event
+  if (wc != 0 && wc > 0 && total_in + 3 <= smax) {
+    atomicAdd(&P.counters[3], 1ull);
+    sort_small(tri_int, total_in);
+    emit_event(P, u, ids, 3, total_in, tri_int, f.level(), f.level_err());
+  } else if (wc == 0 && total_in + 3 <= smax) {
+    emit_fallback(P, w, u, 3, ids, 3, M, region_slot);
+    return;
+  }
+  // apexes: z with id > w, outside S3, ahead of side s, with fewer than
+  // s_max-3 points certainly swept before it
+  for (int side = 0; side < 2; ++side) {
+    const int need = K - cin[side];  // slots left for points swept before z
+    if (need <= 0) continue;
+    const double thr = ntop[side] >= need ? top[side][need - 1] : INFINITY;
+    for (uint32_t s = 0; s < M; ++s) {
+      uint32_t zi = w.ridx[s];
+      if (zi == vi || zi == wi || w.gid[zi] <= wid) continue;
+      V3 p = gpt(w, zi);
+      double Pw, PA, H, HA;
+      f.power(p, Pw, PA);
+      f.height(p, H, HA);
+      if (cert_sign(Pw, PA) <= 0) continue;
+      int sh = cert_sign(H, HA);
+      if ((side == 0 && sh <= 0) || (side == 1 && sh >= 0)) continue;
+      double tau = Pw / fabs(H);
+      double terr = (kCert * PA / fabs(Pw) + kCert * HA / fabs(H) + 1e-15) * tau;
+      double lo = tau - terr;
+      if (lo > thr) continue;  // need points certainly swept before z
+      // inside points behind: counted while their key exceeds z's
+      int lb = cin[side];
+      if (!behind_over[side])
+        for (int b = 0; b < nbehind[side]; ++b) lb += behind[side][b] > tau + terr;
+      if (lb >= K) continue;
+      process_tet(P, w, u, M, f, vi, wi, zi, region_slot);
+    }
+  }
+}
+
+__device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
+  __shared__ uint32_t sM, sK, sflag, sregion;
+  __shared__ double smax_rho;
+  const int t = threadIdx.x;
+  const uint32_t G = P.gcount[u];
+  const uint64_t off = P.goff[u];
+  const double ux = P.X[u], uy = P.Y[u], uz = P.Z[u];
+  const double Rg = P.radius[u];
+  if (t == 0) { sM = 0; sK = 0; sflag = 0; sregion = kInvalid; }
+  for (uint32_t i = t; i < G; i += blockDim.x) {
+    uint32_t id = P.glist[off + i];
+    w.gid[i] = id;
+    w.gx[i] = P.X[id] - ux;
+    w.gy[i] = P.Y[id] - uy;
+    w.gz[i] = P.Z[id] - uz;
+  }
+  __syncthreads();
+  // 1. direction-cell certificate
+  const int need_cover = P.s_max - 1;
+  for (int c = t; c < kCells; c += blockDim.x) {
+    const double *cd = P.cells + 8 * c;
+    const double dx = cd[0], dy = cd[1], dz = cd[2], cosd = cd[3], sind = cd[4], chord = cd[5];
+    double best[12];
+    int nb = 0;
+    for (uint32_t i = 0; i < G; ++i) {
+      double x = w.gx[i], y = w.gy[i], z = w.gz[i];
+      double dist = sqrt(x * x + y * y + z * z);
+      double ct = (x * dx + y * dy + z * dz) / dist;
+      if (ct <= 0.0) continue;
+      double st = sqrt(fmax(0.0, 1.0 - ct * ct));
+      double cs = ct * cosd - st * sind;  // cos(theta + delta)
+      if (cs < 1e-6) continue;
+      double r = dist / (2.0 * cs);
+      if (nb < need_cover) {
+        int j = nb++;
+        while (j > 0 && best[j - 1] > r) { best[j] = best[j - 1]; j--; }
+        best[j] = r;
+      } else if (r < best[need_cover - 1]) {
+        int j = need_cover - 1;
+        while (j > 0 && best[j - 1] > r) { best[j] = best[j - 1]; j--; }
+        best[j] = r;
+      }
+    }
+    double rc = nb >= need_cover ? best[need_cover - 1] * (1.0 + 1e-9) : INFINITY;
+    double dir[3] = {dx, dy, dz};
+    double uu[3] = {ux, uy, uz};
+    double e = INFINITY;
+    for (int a = 0; a < 3; ++a) {
+      double lo = dir[a] - chord, hi = dir[a] + chord;
+      if (lo > 0) e = fmin(e, (P.bhi[a] - uu[a]) / lo);
+      else if (hi < 0) e = fmin(e, (P.blo[a] - uu[a]) / hi);
+    }
+    e = fmax(e, 0.0) * (1.0 + 1e-9) + 1e-300;
+    w.rho[c] = fmin(rc, e);
+  }
+  __syncthreads();
+  if (t == 0) {
+    double m = 0;
+    for (int c = 0; c < kCells; ++c) m = fmax(m, w.rho[c]);
+    smax_rho = m;
+    if (2.0 * m > Rg) {
+      P.need[u] = 2.0 * m;
+      P.status[u] = ST_WIDEN;
+      sflag = 1;
+    }
+  }
+  __syncthreads();
+  if (sflag) return;
+  // 2. region membership masks
+  for (uint32_t i = t; i < G; i += blockDim.x) {
+    double x = w.gx[i], y = w.gy[i], z = w.gz[i];
+    double dist = sqrt(x * x + y * y + z * z);
+    uint64_t mask[6] = {0, 0, 0, 0, 0, 0};
+    bool any = false;
+    for (int c = 0; c < kCells; ++c) {
+      const double *cd = P.cells + 8 * c;
+      double ct = (x * cd[0] + y * cd[1] + z * cd[2]) / dist;
+      double cm;  // cos(max(theta - delta, 0))
+      if (ct >= cd[3]) cm = 1.0;
+      else {
+        double st = sqrt(fmax(0.0, 1.0 - ct * ct));
+        cm = ct * cd[3] + st * cd[4];
+      }
+      if (dist <= 2.0 * w.rho[c] * cm * (1.0 + 1e-9) + 1e-300) {
+        mask[c >> 6] |= 1ull << (c & 63);
+        any = true;
+      }
+    }
+    if (any) {
+      uint32_t slot = atomicAdd(&sM, 1u);
+      if (slot < P.rcap) {
+        w.ridx[slot] = i;
+        for (int k = 0; k < 6; ++k) w.rmask[6 * slot + k] = mask[k];
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t M = sM;
+  if (M > P.rcap) {
+    if (t == 0) P.status[u] = ST_BIG;
+    return;
+  }
+  for (uint32_t s = t; s < M; s += blockDim.x)
+    if (w.gid[w.ridx[s]] > u) w.kidx[atomicAdd(&sK, 1u)] = s;
+  __syncthreads();
+  const uint32_t Kc = sK;
+  // 3. pairs (u, v), v > u
+  for (uint32_t k = t; k < Kc; k += blockDim.x) process_pair(P, w, u, M, w.ridx[w.kidx[k]], &sregion);
+  // 4. faces (u, v, w), u < v < w, whose members share a cell
+  const uint64_t F = (uint64_t)Kc * (Kc - 1) / 2;
+  for (uint64_t fidx = t; fidx < F; fidx += blockDim.x) {
+    // decode the pair index (i < j) of the strictly lower triangle
+    uint64_t j = (uint64_t)((1.0 + sqrt(1.0 + 8.0 * (double)fidx)) * 0.5);
+    while (j * (j - 1) / 2 > fidx) j--;
+    while ((j + 1) * j / 2 <= fidx) j++;
+    uint64_t i = fidx - j * (j - 1) / 2;
+    uint32_t si = w.kidx[i], sj = w.kidx[j];
+    bool share = false;
+    for (int k = 0; k < 6; ++k) share |= (w.rmask[6 * si + k] & w.rmask[6 * sj + k]) != 0;
+    if (!share) continue;
+    // a sphere through u, v, w has radius >= r3; skip faces beyond every shared cell
+    uint32_t vi = w.ridx[si], wi = w.ridx[sj];
+    V3 a = gpt(w, vi), b = gpt(w, wi);
+    double aa = dot(a, a), bb = dot(b, b), ab = dot(a, b);
+    double D = aa * bb - ab * ab;
+    V3 amb = sub(a, b);
+    double r2 = D > 0 ? aa * bb * dot(amb, amb) / (4.0 * D) : INFINITY;
+    double rmax = 0;
+    for (int k = 0; k < 6; ++k) {
+      uint64_t mm = w.rmask[6 * si + k] & w.rmask[6 * sj + k];
+      while (mm) {
+        int bit = __ffsll((long long)mm) - 1;
+        mm &= mm - 1;
+        rmax = fmax(rmax, w.rho[64 * k + bit]);
+      }
+    }
+    // prune only well-conditioned faces (FP64 r2 is then accurate to ~1e-10)
+    if (D > 1e-6 * (aa * bb + fabs(ab) * fabs(ab)) && r2 > rmax * rmax * (1.0 + 1e-6)) continue;
+    process_face(P, w, u, M, vi, wi, &sregion);
+  }
+  __syncthreads();
+  if (t == 0) P.status[u] = ST_DONE;
+}
+
+__global__ void __launch_bounds__(kBlock) k_anchor_lds(Params P) {
+  __shared__ double gx[kGatherCap], gy[kGatherCap], gz[kGatherCap];
+  __shared__ uint32_t gid[kGatherCap];
+  __shared__ double rho[kCells];
+  __shared__ uint32_t ridx[kRegionCap];
+  __shared__ uint64_t rmask[6 * kRegionCap];
+  __shared__ uint32_t kidx[kRegionCap];
+  const uint32_t a = blockIdx.x;
+  if (a >= P.anchor_count) return;
+  const uint32_t u = P.anchors[a];
+  if (P.gcount[u] > P.gcap) {
+    if (threadIdx.x == 0) P.status[u] = ST_BIG;
+    return;
+  }
+  Work w{gx, gy, gz, gid, rho, ridx, rmask, kidx};
+  anchor_work(P, w, u);
+}
+
+// Anchors whose gather or region exceeds LDS: the same work on global slabs.
+__global__ void __launch_bounds__(kBlock) k_anchor_big(Params P) {
+  const uint32_t a = blockIdx.x;
+  if (a >= P.anchor_count) return;
+  const uint32_t u = P.anchors[a];
+  double *slab = P.scratch + (uint64_t)a * P.slab_doubles;
+  const uint32_t G = P.gcap;
+  Work w;
+  w.gx = slab;
+  w.gy = slab + G;
+  w.gz = slab + 2 * G;
+  w.rho = slab + 3 * G;
+  w.rmask = (uint64_t *)(slab + 3 * G + kCells);
+  w.gid = (uint32_t *)(slab + 3 * G + kCells + 6 * G);
+  w.ridx = w.gid + G;
+  w.kidx = w.ridx + G;
+  anchor_work(P, w, u);
+}
+
+// Ball count / emit around every anchor of the list.
+template <bool EMIT>
+__global__ void __launch_bounds__(128) k_gather(int64_t n, const Node *__restrict__ nodes,
+                                                const double *__restrict__ LX,
+                                                const double *__restrict__ LY,
+                                                const double *__restrict__ LZ,
+                                                const uint32_t *__restrict__ leaf_id,
+                                                const uint32_t *__restrict__ leaf_pos,
+                                                const uint32_t *__restrict__ anchors, uint32_t count,
+                                                const double *__restrict__ radius, uint32_t *gcount,
+                                                const uint64_t *__restrict__ goff, uint32_t *glist,
+                                                uint32_t *overflow) {
+  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= count) return;
+  const uint32_t u = anchors[a];
+  const uint32_t p = leaf_pos[u];
+  const double q[3] = {LX[p], LY[p], LZ[p]};
+  const double R = radius[u];
+  const double R2 = R * R * (1.0 + 1e-9);
+  uint32_t cnt = 0;
+  uint64_t base = EMIT ? goff[u] : 0;
+  uint32_t stack[64];
+  int sp = 0;
+  stack[sp++] = 0;
+  if (n == 1) sp = 0;
+  while (sp) {
+    const Node &N = nodes[stack[--sp]];
+    uint32_t ch[2] = {N.left, N.right};
+    for (int c = 0; c < 2; ++c) {
+      uint32_t x = ch[c];
+      if (x & kLeafBit) {
+        uint32_t lp = x & ~kLeafBit;
+        if (lp == p) continue;
+        double dx = LX[lp] - q[0], dy = LY[lp] - q[1], dz = LZ[lp] - q[2];
+        if (dx * dx + dy * dy + dz * dz <= R2) {
+          if (EMIT) glist[base + cnt] = leaf_id[lp];
+          cnt++;
+        }
+      } else if (box_d2(q, nodes[x].lo, nodes[x].hi) <= R2) {
+        if (sp < 64) stack[sp++] = x;
+        else atomicOr(overflow, 1u);
+      }
+    }
+  }
+  if (!EMIT) gcount[u] = cnt;
+}
+
+__global__ void k_initial_radius(int64_t n, const double *__restrict__ X, const double *__restrict__ Y,
+                                 const double *__restrict__ Z, const uint32_t *__restrict__ knn, int k,
+                                 double factor, double *radius) {
+  int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  uint32_t v = knn[u * k + k - 1];
+  double dx = X[v] - X[u], dy = Y[v] - Y[u], dz = Z[v] - Z[u];
+  radius[u] = factor * sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+// exclusive scan of gcount over the anchor list into goff (single workgroup)
+__global__ void k_scan_offsets(const uint32_t *__restrict__ anchors, uint32_t count,
+                               const uint32_t *__restrict__ gcount, uint64_t *goff, uint64_t *total) {
+  __shared__ uint64_t part[1024];
+  const int t = threadIdx.x;
+  const uint32_t per = (count + blockDim.x - 1) / blockDim.x;
+  const uint32_t b = t * per, e = min(b + per, count);
+  uint64_t s = 0;
+  for (uint32_t i = b; i < e; ++i) s += gcount[anchors[i]];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+    uint64_t v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint64_t run = part[t] - s;
+  for (uint32_t i = b; i < e; ++i) {
+    goff[anchors[i]] = run;
+    run += gcount[anchors[i]];
+  }
+  if (t == (int)blockDim.x - 1) *total = part[t];
+}
+
+}  // namespace
+
+// Direction cells: centre, cos/sin of the angular radius, chord bound.
+static std::vector<double> make_cells() {
+  std::vector<double> out(8 * kCells);
+  const int G = 8;
+  int q = 0;
+  for (int f = 0; f < 6; ++f)
+    for (int i = 0; i < G; ++i)
+      for (int j = 0; j < G; ++j) {
+        auto dir = [&](double s, double t, double v[3]) {
+          double a = 2 * s - 1, b = 2 * t - 1, w[3];
+          int ax = f / 2;
+          double sg = (f % 2) ? -1 : 1;
+          w[ax] = sg;
+          w[(ax + 1) % 3] = a;
+          w[(ax + 2) % 3] = b;
+          double nn = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+          for (int k = 0; k < 3; ++k) v[k] = w[k] / nn;
+        };
+        double c[3];
+        dir((i + 0.5) / G, (j + 0.5) / G, c);
+        double delta = 0;
+        for (int ci = 0; ci < 2; ++ci)
+          for (int cj = 0; cj < 2; ++cj) {
+            double v[3];
+            dir((double)(i + ci) / G, (double)(j + cj) / G, v);
+            double dt = c[0] * v[0] + c[1] * v[1] + c[2] * v[2];
+            delta = std::max(delta, std::acos(std::min(1.0, dt)));
+          }
+        delta = delta * 1.01 + 1e-9;
+        double *o = out.data() + 8 * q++;
+        o[0] = c[0]; o[1] = c[1]; o[2] = c[2];
+        o[3] = std::cos(delta);
+        o[4] = std::sin(delta);
+        o[5] = 2.0 * std::sin(delta / 2.0) * 1.001;
+        o[6] = delta;
+        o[7] = 0;
+      }
+  return out;
+}
+
+void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
+                     std::vector<FallbackRec> &fbs, std::vector<uint32_t> &fb_region,
+                     mhgp_catalog_info *info);
+
+void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
+                     std::vector<FallbackRec> &fbs, std::vector<uint32_t> &fb_region,
+                     mhgp_catalog_info *info) {
+  const int64_t n = c.n;
+  if (!c.catalog) c.catalog = std::make_unique<CatalogBuffers>();
+  CatalogBuffers &B = *c.catalog;
+  hipStream_t s = c.stream;
+  const int64_t keff = std::min<int64_t>(k_max, n);
+  const int smax = (int)std::min<int64_t>(keff + 1, n);
+  B.k_max = k_max;
+  B.s_max = smax;
+  info->s_max = smax;
+  if (n < 2 || smax < 2) return;
+  if (smax > 11) throw std::invalid_argument("k_max must lie in 1..10");
+  B.radius.ensure(n);
+  B.gcount.ensure(n);
+  B.goff.ensure(n);
+  B.need.ensure(n);
+  B.status.ensure(n);
+  B.counters.ensure(8);
+  B.anchors.ensure(n);
+  if (B.cells.cap < 8 * kCells) {
+    auto cells = make_cells();
+    B.cells.ensure(8 * kCells);
+    MHGP_HIP(hipMemcpy(B.cells.p, cells.data(), cells.size() * 8, hipMemcpyHostToDevice));
+  }
+  // initial gather radius: twice the distance to the min(32, n-1)-th neighbour
+  const int kq = (int)std::min<int64_t>(32, n - 1);
+  DevBuf<uint32_t> nb;
+  DevBuf<uint8_t> nbf;
+  nb.ensure((size_t)n * kq);
+  nbf.ensure(n);
+  knn_all(c, kq, nb.p, nbf.p);
+  hipLaunchKernelGGL(k_initial_radius, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, c.X.p, c.Y.p,
+                     c.Z.p, nb.p, kq, 2.2, B.radius.p);
+  std::vector<uint32_t> anchors(n);
+  std::iota(anchors.begin(), anchors.end(), 0u);
+  // anchors in Morton order: neighbouring workgroups share cache lines
+  {
+    std::vector<uint32_t> lid(n);
+    MHGP_HIP(hipMemcpy(lid.data(), c.leaf_id.p, n * 4, hipMemcpyDeviceToHost));
+    anchors = lid;
+  }
+  double bounds[6];
+  MHGP_HIP(hipMemcpy(bounds, c.bounds.p, sizeof(bounds), hipMemcpyDeviceToHost));
+  uint64_t event_cap = std::max<uint64_t>(1024, (uint64_t)n * (smax >= 11 ? 640 : 64 * smax));
+  uint64_t fb_cap = std::max<uint64_t>(4096, (uint64_t)n / 4);
+  uint64_t reg_cap = std::max<uint64_t>(1 << 20, (uint64_t)n * 64);
+  B.events.ensure(event_cap);
+  B.fallbacks.ensure(fb_cap);
+  B.fb_region.ensure(reg_cap);
+  MHGP_HIP(hipMemsetAsync(B.counters.p, 0, 8 * 8, s));
+  MHGP_HIP(hipMemsetAsync(B.status.p, 0, n * 4, s));
+  DevBuf<uint32_t> overflow;
+  DevBuf<uint64_t> total;
+  overflow.ensure(1);
+  total.ensure(1);
+  MHGP_HIP(hipMemsetAsync(overflow.p, 0, 4, s));
+  int64_t widened = 0;
+  std::vector<uint32_t> pending = anchors;
+  std::vector<uint32_t> big;
+  for (int iter = 0; !pending.empty(); ++iter) {
+    if (iter > 12) throw std::runtime_error("anchor gather radii did not converge");
+    const uint32_t cnt = (uint32_t)pending.size();
+    MHGP_HIP(hipMemcpyAsync(B.anchors.p, pending.data(), cnt * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_gather<false>, dim3(blocks_for(cnt, 128)), dim3(128), 0, s, n, c.nodes.p,
+                       c.LX.p, c.LY.p, c.LZ.p, c.leaf_id.p, c.leaf_pos.p, B.anchors.p, cnt,
+                       B.radius.p, B.gcount.p, B.goff.p, B.glist.p, overflow.p);
+    hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, B.anchors.p, cnt, B.gcount.p,
+                       B.goff.p, total.p);
+    uint64_t tot = 0;
+    MHGP_HIP(hipMemcpyAsync(&tot, total.p, 8, hipMemcpyDeviceToHost, s));
+    MHGP_HIP(hipStreamSynchronize(s));
+    B.glist.ensure(tot + 1);
+    hipLaunchKernelGGL(k_gather<true>, dim3(blocks_for(cnt, 128)), dim3(128), 0, s, n, c.nodes.p,
+                       c.LX.p, c.LY.p, c.LZ.p, c.leaf_id.p, c.leaf_pos.p, B.anchors.p, cnt,
+                       B.radius.p, B.gcount.p, B.goff.p, B.glist.p, overflow.p);
+    Params P{};
+    P.X = c.X.p; P.Y = c.Y.p; P.Z = c.Z.p;
+    P.gcount = B.gcount.p; P.goff = B.goff.p; P.glist = B.glist.p; P.radius = B.radius.p;
+    P.cells = B.cells.p;
+    for (int a = 0; a < 3; ++a) { P.blo[a] = bounds[a]; P.bhi[a] = bounds[3 + a]; }
+    P.s_max = smax;
+    P.need = B.need.p; P.status = B.status.p;
+    P.events = B.events.p; P.event_cap = event_cap;
+    P.fallbacks = B.fallbacks.p; P.fallback_cap = fb_cap;
+    P.fb_region = B.fb_region.p; P.fb_region_cap = reg_cap;
+    P.counters = (unsigned long long *)B.counters.p;
+    P.anchors = B.anchors.p; P.anchor_count = cnt;
+    P.big = 0; P.gcap = kGatherCap; P.rcap = kRegionCap;
+    hipLaunchKernelGGL(k_anchor_lds, dim3(cnt), dim3(kBlock), 0, s, P);
+    MHGP_LAUNCH_CHECK();
+    std::vector<uint32_t> st(n);
+    std::vector<double> need(n), rad(n);
+    std::vector<uint32_t> gc(n);
+    MHGP_HIP(hipMemcpyAsync(st.data(), B.status.p, n * 4, hipMemcpyDeviceToHost, s));
+    MHGP_HIP(hipMemcpyAsync(need.data(), B.need.p, n * 8, hipMemcpyDeviceToHost, s));
+    MHGP_HIP(hipMemcpyAsync(rad.data(), B.radius.p, n * 8, hipMemcpyDeviceToHost, s));
+    MHGP_HIP(hipMemcpyAsync(gc.data(), B.gcount.p, n * 4, hipMemcpyDeviceToHost, s));
+    MHGP_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> next;
+    for (uint32_t u : pending) {
+      if (st[u] == ST_WIDEN) {
+        rad[u] = std::max(need[u] * 1.02, rad[u] * 1.05);
+        next.push_back(u);
+        widened++;
+      } else if (st[u] == ST_BIG) {
+        big.push_back(u);
+      }
+    }
+    // big anchors: widen inside the big path too
+    if (!big.empty()) {
+      std::vector<uint32_t> redo;
+      for (int biter = 0; !big.empty(); ++biter) {
+        if (biter > 12) throw std::runtime_error("big anchors did not converge");
+        const uint32_t bc = (uint32_t)big.size();
+        uint32_t gmax = 0;
+        for (uint32_t u : big) gmax = std::max(gmax, gc[u]);
+        gmax = std::max<uint32_t>(gmax, 64);
+        const uint64_t slab = 3ull * gmax + kCells + 6ull * gmax + (3ull * gmax + 1) / 2 + 1;
+        const uint32_t chunk = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(bc, (1ull << 30) / (slab * 8)));
+        B.scratch.ensure(slab * chunk);
+        // the CSR of the big anchors is still valid from this iteration's gather
+        std::vector<uint32_t> again;
+        for (uint32_t b0 = 0; b0 < bc; b0 += chunk) {
+          uint32_t bn = std::min(chunk, bc - b0);
+          MHGP_HIP(hipMemcpyAsync(B.anchors.p, big.data() + b0, bn * 4, hipMemcpyHostToDevice, s));
+          Params Q = P;
+          Q.anchors = B.anchors.p; Q.anchor_count = bn; Q.big = 1;
+          Q.scratch = B.scratch.p; Q.slab_doubles = slab; Q.gcap = gmax; Q.rcap = gmax;
+          hipLaunchKernelGGL(k_anchor_big, dim3(bn), dim3(kBlock), 0, s, Q);
+          MHGP_LAUNCH_CHECK();
+        }
+        MHGP_HIP(hipMemcpyAsync(st.data(), B.status.p, n * 4, hipMemcpyDeviceToHost, s));
+        MHGP_HIP(hipMemcpyAsync(need.data(), B.need.p, n * 8, hipMemcpyDeviceToHost, s));
+        MHGP_HIP(hipStreamSynchronize(s));
+        for (uint32_t u : big) {
+          if (st[u] == ST_WIDEN) {
+            rad[u] = std::max(need[u] * 1.02, rad[u] * 1.05);
+            again.push_back(u);
+            widened++;
+          } else if (st[u] != ST_DONE) {
+            throw std::runtime_error("a big anchor did not complete");
+          }
+        }
+        if (again.empty()) break;
+        // regather the widened big anchors
+        const uint32_t ac = (uint32_t)again.size();
+        MHGP_HIP(hipMemcpy(B.radius.p, rad.data(), n * 8, hipMemcpyHostToDevice));
+        MHGP_HIP(hipMemcpyAsync(B.anchors.p, again.data(), ac * 4, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_gather<false>, dim3(blocks_for(ac, 128)), dim3(128), 0, s, n, c.nodes.p,
+                           c.LX.p, c.LY.p, c.LZ.p, c.leaf_id.p, c.leaf_pos.p, B.anchors.p, ac,
+                           B.radius.p, B.gcount.p, B.goff.p, B.glist.p, overflow.p);
+        hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, B.anchors.p, ac, B.gcount.p,
+                           B.goff.p, total.p);
+        MHGP_HIP(hipMemcpyAsync(&tot, total.p, 8, hipMemcpyDeviceToHost, s));
+        MHGP_HIP(hipStreamSynchronize(s));
+        B.glist.ensure(tot + 1);
+        hipLaunchKernelGGL(k_gather<true>, dim3(blocks_for(ac, 128)), dim3(128), 0, s, n, c.nodes.p,
+                           c.LX.p, c.LY.p, c.LZ.p, c.leaf_id.p, c.leaf_pos.p, B.anchors.p, ac,
+                           B.radius.p, B.gcount.p, B.goff.p, B.glist.p, overflow.p);
+        MHGP_HIP(hipMemcpyAsync(gc.data(), B.gcount.p, n * 4, hipMemcpyDeviceToHost, s));
+        MHGP_HIP(hipStreamSynchronize(s));
+        big = again;
+      }
+      big.clear();
+    }
+    MHGP_HIP(hipMemcpy(B.radius.p, rad.data(), n * 8, hipMemcpyHostToDevice));
+    pending = next;
+  }
+  uint32_t ovf = 0;
+  uint64_t ctr[8];
+  MHGP_HIP(hipMemcpy(&ovf, overflow.p, 4, hipMemcpyDeviceToHost));
+  MHGP_HIP(hipMemcpy(ctr, B.counters.p, sizeof(ctr), hipMemcpyDeviceToHost));
+  if (ovf) throw std::runtime_error("LBVH traversal stack overflow in the gather");
+  if (ctr[0] > event_cap || ctr[1] > fb_cap || ctr[2] > reg_cap)
+    throw std::runtime_error("catalogue queue capacity exceeded");
+  events.resize(ctr[0]);
+  fbs.resize(ctr[1]);
+  fb_region.resize(ctr[2]);
+  if (ctr[0]) MHGP_HIP(hipMemcpy(events.data(), B.events.p, ctr[0] * sizeof(EventRec), hipMemcpyDeviceToHost));
+  if (ctr[1]) MHGP_HIP(hipMemcpy(fbs.data(), B.fallbacks.p, ctr[1] * sizeof(FallbackRec), hipMemcpyDeviceToHost));
+  if (ctr[2]) MHGP_HIP(hipMemcpy(fb_region.data(), B.fb_region.p, ctr[2] * 4, hipMemcpyDeviceToHost));
+  info->candidate_count = (int64_t)ctr[3];
+  info->anchors_widened = widened;
+}
+
+}  // namespace mhgp

@@ -7,12 +7,57 @@
 
 namespace mhgp {
 
+constexpr int kCells = 384;      // 6 cube faces x 8 x 8 direction cells
+constexpr int kGatherCap = 1024; // gathered neighbours held in LDS
+constexpr int kRegionCap = 640;  // region members held in LDS
+constexpr int kMaxInterior = 10; // s_max - 2 for s_max <= 11 (K_max <= 10)
+
+// One catalogue record: support ids (unused = kInvalid), meta
+// (size | rank << 8 | interior count << 16 | degenerate << 24), interior ids,
+// FP64 squared radius and a bound on its relative error.
+struct EventRec {
+  uint32_t ids[4];
+  uint32_t meta;
+  uint32_t interior[kMaxInterior];
+  float level_err;
+  double level;
+};
+static_assert(sizeof(EventRec) == 72, "event layout");
+
+// Work the device could not certify: the host settles it exactly.
+// kind 2: pair (u, v); 3: face (u, v, w) incl. its triangle and apexes;
+// 4: tetrahedron (u, v, w, z).  region_off/len locate the anchor's region.
+struct FallbackRec {
+  uint32_t kind;
+  uint32_t ids[4];
+  uint32_t region_off;
+  uint32_t region_len;
+  uint32_t pad;
+};
+
 struct CatalogBuffers {
   bool valid = false;
   int k_max = 0;
   int s_max = 0;
+  DevBuf<double> radius;          // gather radius per anchor
+  DevBuf<uint32_t> gcount;        // gathered count per anchor
+  DevBuf<uint64_t> goff;          // CSR offsets
+  DevBuf<uint32_t> glist;         // gathered ids
+  DevBuf<double> need;            // radius required by the certificate (0 = satisfied)
+  DevBuf<uint32_t> status;        // per anchor: 0 pending, 1 done, 2 widen, 3 big
+  DevBuf<EventRec> events;
+  DevBuf<FallbackRec> fallbacks;
+  DevBuf<uint32_t> fb_region;     // region member ids of anchors with fallbacks
+  DevBuf<uint64_t> counters;      // events, fallbacks, region words, candidates
+  DevBuf<double> cells;           // per cell: dir xyz, cos d, sin d, chord
+  DevBuf<double> scratch;         // global slabs for anchors too big for LDS
+  DevBuf<uint32_t> anchors;       // anchor list of the current launch
+  // host results, canonical order
+  std::vector<EventRec> host_events;
+  std::vector<int64_t> level_class;
+  mhgp_catalog_info info{};
 };
 
-inline int64_t catalog_device_bytes(const CatalogBuffers &) { return 0; }
+int64_t catalog_device_bytes(const CatalogBuffers &b);
 
 }  // namespace mhgp

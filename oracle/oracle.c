@@ -661,6 +661,7 @@ int64_t or_catalog_build(const double *xyz, int64_t n, int k_max, int max_suppor
   cat_rec r;
   int topm = max_support < 4 ? max_support : 4;
   for (int m = 1; m <= topm && m <= n; ++m) {
+    if (m > smax) break; /* closed rank >= |U| > s_max: above rank, never relevant */
     int64_t c[4];
     for (int i = 0; i < m; ++i) c[i] = i;
     int done = 0;
