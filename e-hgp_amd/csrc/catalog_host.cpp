@@ -6,6 +6,7 @@
 
 #include "pipeline.hpp"
 #include "supports.hpp"
+#include "timing.hpp"
 
 namespace mhgp {
 
@@ -78,6 +79,7 @@ void supports(Context &c, int k_max, mhgp_catalog_info *info) {
   std::vector<FallbackRec> fbs;
   std::vector<uint32_t> region;
   supports_device(c, k_max, events, fbs, region, info);
+  StageTimer tm(c.stream);
   CatalogBuffers &B = *c.catalog;
   const int smax = B.s_max;
   // host exact settlement
@@ -106,6 +108,7 @@ void supports(Context &c, int k_max, mhgp_catalog_info *info) {
     }
     info->fallback_count = job.predicates;
   }
+  tm.mark("catalogue host fallback");
   // canonical order: exact level, rank, support ids
   std::vector<int64_t> order(events.size());
   std::iota(order.begin(), order.end(), 0);
@@ -169,6 +172,7 @@ void supports(Context &c, int k_max, mhgp_catalog_info *info) {
     next_class += (grp.empty() ? 0 : gcls.back() + 1);
     i = j;
   }
+  tm.mark("catalogue canonical sort");
   B.host_events = std::move(sorted);
   B.level_class = std::move(cls);
   int64_t ev = 0, dg = 0;

@@ -15,6 +15,7 @@
 
 #include "pipeline.hpp"
 #include "stages.hpp"
+#include "timing.hpp"
 #include "traverse.hpp"
 
 namespace mhgp {
@@ -103,7 +104,7 @@ __global__ void __launch_bounds__(128) k_search(int64_t n, const Node *__restric
         if (d == bd && both) {
           better = edge_key(u, v) < edge_key(u, bv);
         } else {
-          if (!both && fabs(d - bd) <= kSlack * bd) flag = 1;  // order not provable in FP64
+          if (!both && bd < INFINITY && fabs(d - bd) <= kSlack * bd) flag = 1;  // order not provable
           better = d < bd || (d == bd && edge_key(u, v) < edge_key(u, bv));
         }
         if (better) { bd = d; bv = v; bx = ex; }
@@ -361,6 +362,7 @@ void emst_host(Context &c, uint32_t *edges_out) {
     components = cnt[1];
     if (cnt[0] + components != (uint64_t)n) throw std::runtime_error("Borůvka edge count mismatch");
   }
+  StageTimer tm(s);
   // kappa order: FP64 squared length, exact comparison inside rounding ties
   std::vector<uint32_t> e(2 * (n - 1));
   MHGP_HIP(hipMemcpy(e.data(), B.edges.p, e.size() * 4, hipMemcpyDeviceToHost));
@@ -388,6 +390,7 @@ void emst_host(Context &c, uint32_t *edges_out) {
       i = j;
     }
   }
+  tm.mark("emst kappa sort");
   for (int64_t i = 0; i < n - 1; ++i) {
     edges_out[2 * i] = (uint32_t)(ord[i].second >> 32);
     edges_out[2 * i + 1] = (uint32_t)ord[i].second;

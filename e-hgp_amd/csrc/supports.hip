@@ -38,6 +38,7 @@
 #include "pipeline.hpp"
 #include "predicates.hpp"
 #include "supports.hpp"
+#include "timing.hpp"
 #include "traverse.hpp"
 
 namespace mhgp {
@@ -648,6 +649,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
   DevBuf<uint8_t> nbf;
   nb.ensure((size_t)n * kq);
   nbf.ensure(n);
+  StageTimer tm(s);
   knn_all(c, kq, nb.p, nbf.p);
   hipLaunchKernelGGL(k_initial_radius, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, c.X.p, c.Y.p,
                      c.Z.p, nb.p, kq, 2.2, B.radius.p);
@@ -706,7 +708,9 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     P.counters = (unsigned long long *)B.counters.p;
     P.anchors = B.anchors.p; P.anchor_count = cnt;
     P.big = 0; P.gcap = kGatherCap; P.rcap = kRegionCap;
+    tm.mark("catalogue gather");
     hipLaunchKernelGGL(k_anchor_lds, dim3(cnt), dim3(kBlock), 0, s, P);
+    tm.mark("catalogue anchors");
     MHGP_LAUNCH_CHECK();
     std::vector<uint32_t> st(n);
     std::vector<double> need(n), rad(n);
@@ -786,6 +790,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     MHGP_HIP(hipMemcpy(B.radius.p, rad.data(), n * 8, hipMemcpyHostToDevice));
     pending = next;
   }
+  tm.mark("catalogue big anchors");
   uint32_t ovf = 0;
   uint64_t ctr[8];
   MHGP_HIP(hipMemcpy(&ovf, overflow.p, 4, hipMemcpyDeviceToHost));
@@ -799,6 +804,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
   if (ctr[0]) MHGP_HIP(hipMemcpy(events.data(), B.events.p, ctr[0] * sizeof(EventRec), hipMemcpyDeviceToHost));
   if (ctr[1]) MHGP_HIP(hipMemcpy(fbs.data(), B.fallbacks.p, ctr[1] * sizeof(FallbackRec), hipMemcpyDeviceToHost));
   if (ctr[2]) MHGP_HIP(hipMemcpy(fb_region.data(), B.fb_region.p, ctr[2] * 4, hipMemcpyDeviceToHost));
+  tm.mark("catalogue download");
   info->candidate_count = (int64_t)ctr[3];
   info->anchors_widened = widened;
 }

@@ -1,0 +1,368 @@
+// hgp_reduced merge-forest tower from the critical-sphere catalogue.
+//
+// Direct Morse construction (reference morsehgp3d/src/cpu/hierarchy/
+// direct_morse_forest_journal.cpp:1042 and the semantics of the exhaustive
+// Gamma reduction, reference/morsehgp3d_oracle/hierarchy.py build_merge_forest):
+//  * order k minima: critical spheres of closed rank k (order 1: the points);
+//  * order k saddles: critical spheres of closed rank k+1; the strict arms of
+//    a saddle are its closed label minus one shell point;
+//  * an arm is resolved to the minimum of its component by a descent through
+//    facets of strictly decreasing miniball: while the closed miniball of the
+//    facet F holds a point x outside F, the facet F + x - y (y a point of the
+//    miniball's support, x strictly inside) has a smaller miniball and lies in
+//    the same component;
+//  * saddles of one exact level are contracted together: a group of carriers
+//    with no prior reduced root creates a reduced birth (order k >= 2), one
+//    root grows silently, two or more roots create a multifusion.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <unordered_map>
+
+#include "pipeline.hpp"
+#include "predicates.hpp"
+#include "supports.hpp"
+#include "timing.hpp"
+#include "tower_state.hpp"
+
+namespace mhgp {
+
+namespace {
+
+struct LabelHash {
+  size_t operator()(const std::vector<uint32_t> &v) const {
+    uint64_t h = 1469598103934665603ull;
+    for (uint32_t x : v) { h ^= x; h *= 1099511628211ull; }
+    return (size_t)h;
+  }
+};
+
+struct Grid {
+  double lo[3] = {0, 0, 0};
+  double cell = 1;
+  int64_t dims[3] = {1, 1, 1};
+  std::vector<int64_t> start;
+  std::vector<uint32_t> ids;
+  void build(const std::vector<double> &xyz, int64_t n) {
+    double hi[3];
+    for (int a = 0; a < 3; ++a) { lo[a] = INFINITY; hi[a] = -INFINITY; }
+    for (int64_t i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], xyz[3 * i + a]); hi[a] = std::max(hi[a], xyz[3 * i + a]); }
+    double vol = 1;
+    for (int a = 0; a < 3; ++a) vol *= std::max(hi[a] - lo[a], 1e-300);
+    cell = std::max(std::cbrt(vol / (double)n) * 1.5, 1e-300);
+    for (int a = 0; a < 3; ++a) dims[a] = std::min<int64_t>(std::max<int64_t>(1, (int64_t)((hi[a] - lo[a]) / cell) + 1), 1024);
+    for (int a = 0; a < 3; ++a) cell = std::max(cell, (hi[a] - lo[a]) / (double)dims[a] * 1.000001);
+    const int64_t cells = dims[0] * dims[1] * dims[2];
+    start.assign(cells + 1, 0);
+    std::vector<int64_t> key(n);
+    for (int64_t i = 0; i < n; ++i) { key[i] = cell_of(&xyz[3 * i]); start[key[i] + 1]++; }
+    for (int64_t c = 0; c < cells; ++c) start[c + 1] += start[c];
+    ids.resize(n);
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    for (int64_t i = 0; i < n; ++i) ids[fill[key[i]]++] = (uint32_t)i;
+  }
+  int64_t coord(double v, int a) const {
+    int64_t c = (int64_t)std::floor((v - lo[a]) / cell);
+    return std::min<int64_t>(std::max<int64_t>(c, 0), dims[a] - 1);
+  }
+  int64_t cell_of(const double *p) const { return (coord(p[0], 0) * dims[1] + coord(p[1], 1)) * dims[2] + coord(p[2], 2); }
+  template <class F>
+  void ball(const double c[3], double r, F &&f) const {
+    int64_t a0[3], a1[3];
+    for (int a = 0; a < 3; ++a) { a0[a] = coord(c[a] - r, a); a1[a] = coord(c[a] + r, a); }
+    for (int64_t i = a0[0]; i <= a1[0]; ++i)
+      for (int64_t j = a0[1]; j <= a1[1]; ++j)
+        for (int64_t k = a0[2]; k <= a1[2]; ++k) {
+          int64_t cc = (i * dims[1] + j) * dims[2] + k;
+          for (int64_t q = start[cc]; q < start[cc + 1]; ++q) f(ids[q]);
+        }
+  }
+};
+
+struct BallFit {
+  int m = 0;
+  int64_t sup[4];
+  double c[3];
+  double r2 = 0;
+};
+
+class Descender {
+ public:
+  Descender(const std::vector<double> &xyz, int64_t n, const ExactCloud &ex) : xyz_(xyz), n_(n), ex_(ex) {
+    grid_.build(xyz, n);
+  }
+  int64_t steps = 0;
+
+  V3 rel(int64_t x, int64_t o) const {
+    return {xyz_[3 * x] - xyz_[3 * o], xyz_[3 * x + 1] - xyz_[3 * o + 1], xyz_[3 * x + 2] - xyz_[3 * o + 2]};
+  }
+
+  // certified side of x against the circumsphere of the support (exact fallback)
+  int side(const int64_t *sup, int m, int64_t x) const {
+    for (int k = 0; k < m; ++k)
+      if (sup[k] == x) return 0;
+    int s = 0;
+    if (m == 1) return 1;
+    V3 p = rel(x, sup[0]);
+    if (m == 2) { PairSphere ps; ps.init(rel(sup[1], sup[0])); s = ps.side(p); }
+    else if (m == 3) { TriSphere ts; ts.init(rel(sup[1], sup[0]), rel(sup[2], sup[0])); s = ts.side(p); }
+    else { TetSphere tt; tt.init(rel(sup[1], sup[0]), rel(sup[2], sup[0]), rel(sup[3], sup[0])); s = tt.side(p); }
+    if (s != 0) return s;
+    ExactSupport S;
+    if (!S.analyze(ex_, sup, m)) throw std::runtime_error("dependent support in a descent");
+    return S.side(ex_, x);
+  }
+
+  // FP64 circumball of a support (centre in its affine hull); false if degenerate
+  bool circumball(const int64_t *sup, int m, BallFit &b) const {
+    b.m = m;
+    for (int k = 0; k < m; ++k) b.sup[k] = sup[k];
+    const double *p0 = &xyz_[3 * sup[0]];
+    if (m == 1) { for (int a = 0; a < 3; ++a) b.c[a] = p0[a]; b.r2 = 0; return true; }
+    V3 a = rel(sup[1], sup[0]);
+    V3 off;
+    if (m == 2) {
+      off = {0.5 * a.x, 0.5 * a.y, 0.5 * a.z};
+    } else if (m == 3) {
+      V3 bb = rel(sup[2], sup[0]);
+      double aa = dot(a, a), b2 = dot(bb, bb), ab = dot(a, bb);
+      double D = aa * b2 - ab * ab;
+      if (!(D > 1e-14 * (aa * b2 + ab * ab))) return false;
+      double al = b2 * (aa - ab), be = aa * (b2 - ab);
+      off = {(al * a.x + be * bb.x) / (2 * D), (al * a.y + be * bb.y) / (2 * D), (al * a.z + be * bb.z) / (2 * D)};
+    } else {
+      V3 bb = rel(sup[2], sup[0]), d = rel(sup[3], sup[0]);
+      V3 bxd = cross(bb, d), dxa = cross(d, a), axb = cross(a, bb);
+      double V = dot(a, bxd);
+      if (!(std::fabs(V) > 1e-14 * adot(vabs(a), across(bb, d)))) return false;
+      double aa = dot(a, a), b2 = dot(bb, bb), dd = dot(d, d);
+      off = {(aa * bxd.x + b2 * dxa.x + dd * axb.x) / (2 * V), (aa * bxd.y + b2 * dxa.y + dd * axb.y) / (2 * V),
+             (aa * bxd.z + b2 * dxa.z + dd * axb.z) / (2 * V)};
+    }
+    b.c[0] = p0[0] + off.x; b.c[1] = p0[1] + off.y; b.c[2] = p0[2] + off.z;
+    b.r2 = dot(off, off);
+    return true;
+  }
+
+  // minimum enclosing ball of a facet: smallest circumball of a subset that
+  // encloses the facet; near-equal radii are compared exactly
+  BallFit miniball(const std::vector<uint32_t> &F) const {
+    std::vector<BallFit> cands;
+    const int k = (int)F.size();
+    int64_t s[4];
+    for (int m = 1; m <= std::min(4, k); ++m) {
+      std::vector<int> idx(m);
+      std::iota(idx.begin(), idx.end(), 0);
+      while (true) {
+        for (int q = 0; q < m; ++q) s[q] = F[idx[q]];
+        BallFit b;
+        if (circumball(s, m, b)) cands.push_back(b);
+        int q = m - 1;
+        while (q >= 0 && idx[q] == k - m + q) q--;
+        if (q < 0) break;
+        idx[q]++;
+        for (int r = q + 1; r < m; ++r) idx[r] = idx[r - 1] + 1;
+      }
+    }
+    std::sort(cands.begin(), cands.end(), [](const BallFit &a, const BallFit &b) { return a.r2 < b.r2; });
+    const BallFit *best = nullptr;
+    for (const BallFit &b : cands) {
+      if (best && b.r2 > best->r2 * (1 + 1e-9) + 1e-300) break;
+      bool encloses = true;
+      for (uint32_t x : F)
+        if (side(b.sup, b.m, x) > 0) { encloses = false; break; }
+      if (!encloses) continue;
+      if (!best) { best = &b; continue; }
+      // exact tie-break between two enclosing balls of near-equal radius
+      ExactSupport A, B;
+      A.analyze(ex_, best->sup, best->m);
+      B.analyze(ex_, b.sup, b.m);
+      if (compare_levels(B.lnum, B.lden, A.lnum, A.lden) < 0) best = &b;
+    }
+    if (!best) throw std::runtime_error("no enclosing ball found for a facet");
+    return *best;
+  }
+
+  // closed-ball members outside the facet; strictly inside ones first
+  void extras(const BallFit &b, const std::vector<uint32_t> &F, std::vector<uint32_t> &inside,
+              std::vector<uint32_t> &on) const {
+    inside.clear();
+    on.clear();
+    const double r = std::sqrt(b.r2) * (1 + 1e-9) + 1e-300;
+    grid_.ball(b.c, r, [&](uint32_t x) {
+      if (std::binary_search(F.begin(), F.end(), x)) return;
+      double d2 = 0;
+      for (int a = 0; a < 3; ++a) { double t = xyz_[3 * x + a] - b.c[a]; d2 += t * t; }
+      if (d2 > b.r2 * (1 + 1e-8) + 1e-300) return;
+      int sd = side(b.sup, b.m, x);
+      if (sd < 0) inside.push_back(x);
+      else if (sd == 0) on.push_back(x);
+    });
+  }
+
+  template <class Map>
+  int64_t descend(std::vector<uint32_t> F, const Map &minima) {
+    std::vector<uint32_t> in, on;
+    for (int it = 0; it < 100000; ++it) {
+      auto f = minima.find(F);
+      if (f != minima.end()) return f->second;
+      BallFit b = miniball(F);
+      extras(b, F, in, on);
+      if (in.empty()) {
+        if (on.empty()) throw std::runtime_error("a facet minimum is missing from the catalogue");
+        throw std::runtime_error("degenerate descent: only shell points outside the facet");
+      }
+      const uint32_t x = *std::min_element(in.begin(), in.end());
+      int64_t y = b.sup[0];
+      for (int q = 1; q < b.m; ++q) y = std::min(y, b.sup[q]);
+      F.erase(std::find(F.begin(), F.end(), (uint32_t)y));
+      F.insert(std::upper_bound(F.begin(), F.end(), x), x);
+      steps++;
+    }
+    throw std::runtime_error("a descent did not terminate");
+  }
+
+ private:
+  const std::vector<double> &xyz_;
+  int64_t n_;
+  const ExactCloud &ex_;
+  Grid grid_;
+};
+
+struct Dsu {
+  std::vector<int64_t> p;
+  explicit Dsu(int64_t n) : p(n) { std::iota(p.begin(), p.end(), 0); }
+  int64_t find(int64_t x) {
+    while (p[x] != x) { p[x] = p[p[x]]; x = p[x]; }
+    return x;
+  }
+};
+
+}  // namespace
+
+void tower(Context &c, int k_max, mhgp_tower_info *info) {
+  if (!c.catalog || !c.catalog->valid) throw StateError("build the catalogue first");
+  CatalogBuffers &B = *c.catalog;
+  if (k_max < 1 || k_max > B.k_max) throw std::invalid_argument("k_max exceeds the catalogue's k_max");
+  StageTimer tm(c.stream);
+  const int64_t n = c.n;
+  const int K = (int)std::min<int64_t>(k_max, n);
+  if (!c.tower) c.tower = std::make_unique<TowerState>();
+  TowerState &T = *c.tower;
+  T = TowerState{};
+  const ExactCloud &ex = exact_cloud(c);
+  std::vector<double> xyz(3 * n);
+  canonical_points(c, xyz.data(), nullptr);
+  Descender desc(xyz, n, ex);
+  const std::vector<EventRec> &E = B.host_events;
+  const std::vector<int64_t> &cls = B.level_class;
+  for (int k = 1; k <= K; ++k) {
+    // minima of order k and their carrier DSU
+    std::unordered_map<std::vector<uint32_t>, int64_t, LabelHash> minima;
+    std::vector<int64_t> owner;
+    if (k == 1) {
+      for (int64_t p = 0; p < n; ++p) {
+        minima[{(uint32_t)p}] = p;
+        owner.push_back(T.add(1, 0, -1, 0.0, {}));
+      }
+    } else {
+      for (const EventRec &e : E) {
+        if (((e.meta >> 8) & 0xff) != (uint32_t)k || ((e.meta >> 24) & 1)) continue;
+        std::vector<uint32_t> L;
+        for (int q = 0; q < (int)(e.meta & 0xff); ++q) L.push_back(e.ids[q]);
+        for (int q = 0; q < (int)((e.meta >> 16) & 0xff); ++q) L.push_back(e.interior[q]);
+        std::sort(L.begin(), L.end());
+        minima.emplace(L, (int64_t)owner.size());
+        owner.push_back(-1);
+      }
+    }
+    Dsu dsu((int64_t)owner.size());
+    std::unordered_map<std::vector<uint32_t>, int64_t, LabelHash> arm_cache;
+    for (size_t i = 0; i < E.size();) {
+      size_t j = i;
+      while (j < E.size() && cls[j] == cls[i]) j++;
+      // batch of saddles at one exact level
+      std::unordered_map<int64_t, int64_t> bp;  // batch-local union over carrier roots
+      auto bfind = [&](int64_t x) {
+        while (bp[x] != x) { bp[x] = bp[bp[x]]; x = bp[x]; }
+        return x;
+      };
+      bool any = false;
+      for (size_t e = i; e < j; ++e) {
+        const EventRec &ev = E[e];
+        if (((ev.meta >> 8) & 0xff) != (uint32_t)(k + 1) || ((ev.meta >> 24) & 1)) continue;
+        const int m = ev.meta & 0xff;
+        if (m < 2) continue;
+        any = true;
+        std::vector<uint32_t> L;
+        for (int q = 0; q < m; ++q) L.push_back(ev.ids[q]);
+        for (int q = 0; q < (int)((ev.meta >> 16) & 0xff); ++q) L.push_back(ev.interior[q]);
+        std::sort(L.begin(), L.end());
+        int64_t first = -1;
+        for (int q = 0; q < m; ++q) {
+          std::vector<uint32_t> arm;
+          for (uint32_t x : L)
+            if (x != ev.ids[q]) arm.push_back(x);
+          int64_t carrier;
+          auto hit = arm_cache.find(arm);
+          if (hit != arm_cache.end()) carrier = hit->second;
+          else { carrier = desc.descend(arm, minima); arm_cache.emplace(arm, carrier); }
+          int64_t r = dsu.find(carrier);
+          if (!bp.count(r)) bp[r] = r;
+          if (first < 0) first = r;
+          else { int64_t a = bfind(first), b = bfind(r); if (a != b) bp[b] = a; }
+        }
+      }
+      if (any) {
+        std::unordered_map<int64_t, std::vector<int64_t>> groups;
+        for (auto &kv : bp) groups[bfind(kv.first)].push_back(kv.first);
+        std::vector<std::vector<int64_t>> glist;
+        for (auto &kv : groups) {
+          std::sort(kv.second.begin(), kv.second.end());
+          glist.push_back(kv.second);
+        }
+        std::sort(glist.begin(), glist.end());
+        for (auto &members : glist) {
+          std::vector<int64_t> roots;
+          for (int64_t r : members)
+            if (owner[r] >= 0) roots.push_back(owner[r]);
+          std::sort(roots.begin(), roots.end());
+          roots.erase(std::unique(roots.begin(), roots.end()), roots.end());
+          int64_t node;
+          if (roots.empty()) node = T.add(k, 1, cls[i], E[i].level, {});
+          else if (roots.size() == 1) node = roots[0];
+          else node = T.add(k, 2, cls[i], E[i].level, roots);
+          const int64_t base = members[0];
+          for (size_t q = 1; q < members.size(); ++q) dsu.p[dsu.find(members[q])] = dsu.find(base);
+          owner[dsu.find(base)] = node;
+        }
+      }
+      i = j;
+    }
+  }
+  T.descent_steps = desc.steps;
+  T.orders = K;
+  T.status = 0;
+  T.valid = true;
+  tm.mark("tower");
+  info->node_count = (int64_t)T.order.size();
+  info->child_count = (int64_t)T.children.size();
+  info->descent_steps = T.descent_steps;
+  info->orders = T.orders;
+  info->status = T.status;
+}
+
+void tower_fetch(Context &c, int32_t *order, int32_t *kind, int64_t *level_class, double *level,
+                 int64_t *child_offset, int64_t *children) {
+  if (!c.tower || !c.tower->valid) throw StateError("no tower has been built");
+  const TowerState &T = *c.tower;
+  std::copy(T.order.begin(), T.order.end(), order);
+  std::copy(T.kind.begin(), T.kind.end(), kind);
+  std::copy(T.level_class.begin(), T.level_class.end(), level_class);
+  std::copy(T.level.begin(), T.level.end(), level);
+  std::copy(T.child_offset.begin(), T.child_offset.end(), child_offset);
+  std::copy(T.children.begin(), T.children.end(), children);
+}
+
+}  // namespace mhgp

@@ -1,6 +1,7 @@
 // Host state of the hgp_reduced tower.
 #pragma once
 
+#include <cstdint>
 #include <vector>
 
 #include "stages.hpp"
@@ -9,6 +10,24 @@ namespace mhgp {
 
 struct TowerState {
   bool valid = false;
+  std::vector<int32_t> order, kind;
+  std::vector<int64_t> level_class;
+  std::vector<double> level;
+  std::vector<int64_t> child_offset{0};
+  std::vector<int64_t> children;
+  int64_t descent_steps = 0;
+  int32_t orders = 0;
+  int32_t status = 0;
+
+  int64_t add(int32_t o, int32_t k, int64_t cls, double lev, const std::vector<int64_t> &kids) {
+    order.push_back(o);
+    kind.push_back(k);
+    level_class.push_back(cls);
+    level.push_back(lev);
+    children.insert(children.end(), kids.begin(), kids.end());
+    child_offset.push_back((int64_t)children.size());
+    return (int64_t)order.size() - 1;
+  }
 };
 
 }  // namespace mhgp
