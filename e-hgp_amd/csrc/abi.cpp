@@ -174,3 +174,10 @@ int mhgp_tower_fetch(mhgp_ctx *ctx, int32_t *order, int32_t *kind, int64_t *leve
 }
 
 }  // extern "C"
+
+extern "C" int mhgp_ctx_stats(mhgp_ctx *ctx, mhgp_stats *stats, int reset) {
+  if (!ctx || !stats) return fail(ctx ? &ctx->c : nullptr, MHGP_E_ARGUMENT, "null argument", false);
+  *stats = ctx->c.stats;
+  if (reset) ctx->c.stats = mhgp_stats{};
+  return MHGP_OK;
+}

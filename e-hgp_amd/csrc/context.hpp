@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "../../include/mhgp.h"
 #include "radix_sort.hpp"
 
 namespace mhgp {
@@ -47,6 +48,23 @@ struct Context {
   DevBuf<uint32_t> wit;       // 6 witnesses per internal node (lo xyz, hi xyz)
   DevBuf<uint32_t> counter;
   RadixSorter sorter;
+  mhgp_stats stats{};
+
+  // device time of one launch window on the context stream
+  struct KernelClock {
+    hipEvent_t a = nullptr, b = nullptr;
+    void start(hipStream_t s) {
+      if (!a) { MHGP_HIP(hipEventCreate(&a)); MHGP_HIP(hipEventCreate(&b)); }
+      MHGP_HIP(hipEventRecord(a, s));
+    }
+    void stop(hipStream_t s) { MHGP_HIP(hipEventRecord(b, s)); }
+    double ms() {
+      float v = 0;
+      MHGP_HIP(hipEventSynchronize(b));
+      MHGP_HIP(hipEventElapsedTime(&v, a, b));
+      return v;
+    }
+  } anchor_clock, arm_clock;
 
   std::unique_ptr<CatalogBuffers> catalog;
   std::unique_ptr<EmstBuffers> emst;

@@ -709,9 +709,24 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     P.anchors = B.anchors.p; P.anchor_count = cnt;
     P.big = 0; P.gcap = kGatherCap; P.rcap = kRegionCap;
     tm.mark("catalogue gather");
+    uint64_t ev_before = 0;
+    MHGP_HIP(hipMemcpy(&ev_before, B.counters.p, 8, hipMemcpyDeviceToHost));
+    c.anchor_clock.start(s);
     hipLaunchKernelGGL(k_anchor_lds, dim3(cnt), dim3(kBlock), 0, s, P);
-    tm.mark("catalogue anchors");
+    c.anchor_clock.stop(s);
     MHGP_LAUNCH_CHECK();
+    {
+      uint64_t ev_after = 0;
+      MHGP_HIP(hipMemcpy(&ev_after, B.counters.p, 8, hipMemcpyDeviceToHost));
+      c.stats.anchor_ms += c.anchor_clock.ms();
+      c.stats.anchor_launches++;
+      c.stats.gathered_points += (int64_t)tot;
+      c.stats.events += (int64_t)(ev_after - ev_before);
+      // algorithmic bytes: every gathered neighbour read once (id + FP64 xyz),
+      // every record written once
+      c.stats.anchor_bytes += (int64_t)tot * 28 + (int64_t)(ev_after - ev_before) * (int64_t)sizeof(EventRec);
+    }
+    tm.mark("catalogue anchors");
     std::vector<uint32_t> st(n);
     std::vector<double> need(n), rad(n);
     std::vector<uint32_t> gc(n);
@@ -798,10 +813,11 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
   if (ovf) throw std::runtime_error("LBVH traversal stack overflow in the gather");
   if (ctr[0] > event_cap || ctr[1] > fb_cap || ctr[2] > reg_cap)
     throw std::runtime_error("catalogue queue capacity exceeded");
-  events.resize(ctr[0]);
+  // records stay on the device; only the host's share of the work moves
+  (void)events;
+  B.count = (int64_t)ctr[0];
   fbs.resize(ctr[1]);
   fb_region.resize(ctr[2]);
-  if (ctr[0]) MHGP_HIP(hipMemcpy(events.data(), B.events.p, ctr[0] * sizeof(EventRec), hipMemcpyDeviceToHost));
   if (ctr[1]) MHGP_HIP(hipMemcpy(fbs.data(), B.fallbacks.p, ctr[1] * sizeof(FallbackRec), hipMemcpyDeviceToHost));
   if (ctr[2]) MHGP_HIP(hipMemcpy(fb_region.data(), B.fb_region.p, ctr[2] * 4, hipMemcpyDeviceToHost));
   tm.mark("catalogue download");

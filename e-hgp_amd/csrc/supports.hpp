@@ -52,12 +52,18 @@ struct CatalogBuffers {
   DevBuf<double> cells;           // per cell: dir xyz, cos d, sin d, chord
   DevBuf<double> scratch;         // global slabs for anchors too big for LDS
   DevBuf<uint32_t> anchors;       // anchor list of the current launch
-  // host results, canonical order
-  std::vector<EventRec> host_events;
-  std::vector<int64_t> level_class;
+  // canonical order on the device: records (device emitted + host settled),
+  // their order by (exact level, rank, support) and dense exact level classes
+  int64_t count = 0;
+  DevBuf<uint64_t> sort_keys;
+  DevBuf<uint32_t> order;
+  DevBuf<uint32_t> cls;
+  DevBuf<uint32_t> ambiguous;     // sorted positions whose level is not provably above the previous
   mhgp_catalog_info info{};
 };
 
 int64_t catalog_device_bytes(const CatalogBuffers &b);
+// order.hip: canonical order and level classes of B.events[0, B.count)
+void order_catalogue(Context &c, CatalogBuffers &B);
 
 }  // namespace mhgp

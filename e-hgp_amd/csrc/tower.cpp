@@ -23,6 +23,7 @@
 #include "predicates.hpp"
 #include "supports.hpp"
 #include "timing.hpp"
+#include "tower_gpu.hpp"
 #include "tower_state.hpp"
 
 namespace mhgp {
@@ -251,101 +252,117 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
   if (!c.tower) c.tower = std::make_unique<TowerState>();
   TowerState &T = *c.tower;
   T = TowerState{};
-  const ExactCloud &ex = exact_cloud(c);
-  std::vector<double> xyz(3 * n);
-  canonical_points(c, xyz.data(), nullptr);
-  Descender desc(xyz, n, ex);
-  const std::vector<EventRec> &E = B.host_events;
-  const std::vector<int64_t> &cls = B.level_class;
+  std::unique_ptr<Descender> desc;  // host exact descents, built on demand
+  std::vector<double> xyz;
   for (int k = 1; k <= K; ++k) {
-    // minima of order k and their carrier DSU
-    std::unordered_map<std::vector<uint32_t>, int64_t, LabelHash> minima;
-    std::vector<int64_t> owner;
-    if (k == 1) {
-      for (int64_t p = 0; p < n; ++p) {
-        minima[{(uint32_t)p}] = p;
-        owner.push_back(T.add(1, 0, -1, 0.0, {}));
+    TowerOrderData D;
+    tower_order_device(c, k, D);
+    tm.mark("tower arms (device)");
+    T.descent_steps += D.steps;
+    // carriers the device could not settle: exact host descents
+    bool any_host = false;
+    for (uint32_t x : D.carrier) any_host |= x == kInvalid;
+    if (any_host) {
+      if (!desc) {
+        xyz.resize(3 * n);
+        canonical_points(c, xyz.data(), nullptr);
+        desc = std::make_unique<Descender>(xyz, n, exact_cloud(c));
       }
-    } else {
-      for (const EventRec &e : E) {
-        if (((e.meta >> 8) & 0xff) != (uint32_t)k || ((e.meta >> 24) & 1)) continue;
-        std::vector<uint32_t> L;
-        for (int q = 0; q < (int)(e.meta & 0xff); ++q) L.push_back(e.ids[q]);
-        for (int q = 0; q < (int)((e.meta >> 16) & 0xff); ++q) L.push_back(e.interior[q]);
-        std::sort(L.begin(), L.end());
-        minima.emplace(L, (int64_t)owner.size());
-        owner.push_back(-1);
-      }
+      std::unordered_map<std::vector<uint32_t>, int64_t, LabelHash> minima;
+      for (uint32_t i = 0; i < D.minima; ++i)
+        minima.emplace(std::vector<uint32_t>(D.minima_labels.begin() + (size_t)i * k,
+                                             D.minima_labels.begin() + (size_t)(i + 1) * k), i);
+      for (uint32_t s = 0; s < D.saddles; ++s)
+        for (uint32_t a = D.arm_base[s]; a < D.arm_base[s + 1]; ++a) {
+          if (D.carrier[a] != kInvalid) continue;
+          std::vector<uint32_t> arm;
+          const uint32_t removed = D.sup[4 * s + (a - D.arm_base[s])];
+          for (int q = 0; q < 11; ++q) {
+            uint32_t x = D.saddle_labels[(size_t)s * 11 + q];
+            if (x != kInvalid && x != removed) arm.push_back(x);
+          }
+          std::sort(arm.begin(), arm.end());
+          D.carrier[a] = (uint32_t)desc->descend(arm, minima);
+        }
+      tm.mark("tower arms (host exact)");
     }
-    Dsu dsu((int64_t)owner.size());
-    std::unordered_map<std::vector<uint32_t>, int64_t, LabelHash> arm_cache;
-    for (size_t i = 0; i < E.size();) {
-      size_t j = i;
-      while (j < E.size() && cls[j] == cls[i]) j++;
-      // batch of saddles at one exact level
-      std::unordered_map<int64_t, int64_t> bp;  // batch-local union over carrier roots
+    // carrier DSU; each class may own a reduced root
+    const int64_t M = k == 1 ? n : (int64_t)D.minima;
+    std::vector<int64_t> owner(M, -1);
+    if (k == 1)
+      for (int64_t p = 0; p < n; ++p) owner[p] = T.add(1, 0, -1, 0.0, {});
+    Dsu dsu(M);
+    std::vector<int64_t> touched, bp_parent;
+    std::unordered_map<int64_t, int64_t> bp;
+    for (uint32_t i = 0; i < D.saddles;) {
+      uint32_t j = i;
+      while (j < D.saddles && D.cls[j] == D.cls[i]) j++;
+      if (j == i + 1) {
+        // single saddle at this level: distinct carrier roots are the group
+        std::vector<int64_t> members;
+        for (uint32_t a = D.arm_base[i]; a < D.arm_base[i + 1]; ++a) members.push_back(dsu.find(D.carrier[a]));
+        std::sort(members.begin(), members.end());
+        members.erase(std::unique(members.begin(), members.end()), members.end());
+        std::vector<int64_t> roots;
+        for (int64_t r : members)
+          if (owner[r] >= 0) roots.push_back(owner[r]);
+        std::sort(roots.begin(), roots.end());
+        roots.erase(std::unique(roots.begin(), roots.end()), roots.end());
+        if (members.size() > 1 || roots.empty()) {
+          int64_t node;
+          if (roots.empty()) node = T.add(k, 1, D.cls[i], D.level[i], {});
+          else if (roots.size() == 1) node = roots[0];
+          else node = T.add(k, 2, D.cls[i], D.level[i], roots);
+          for (size_t q = 1; q < members.size(); ++q) dsu.p[members[q]] = members[0];
+          owner[members[0]] = node;
+        }
+        i = j;
+        continue;
+      }
+      // several saddles at one exact level: contract them together
+      bp.clear();
       auto bfind = [&](int64_t x) {
         while (bp[x] != x) { bp[x] = bp[bp[x]]; x = bp[x]; }
         return x;
       };
-      bool any = false;
-      for (size_t e = i; e < j; ++e) {
-        const EventRec &ev = E[e];
-        if (((ev.meta >> 8) & 0xff) != (uint32_t)(k + 1) || ((ev.meta >> 24) & 1)) continue;
-        const int m = ev.meta & 0xff;
-        if (m < 2) continue;
-        any = true;
-        std::vector<uint32_t> L;
-        for (int q = 0; q < m; ++q) L.push_back(ev.ids[q]);
-        for (int q = 0; q < (int)((ev.meta >> 16) & 0xff); ++q) L.push_back(ev.interior[q]);
-        std::sort(L.begin(), L.end());
+      for (uint32_t e = i; e < j; ++e) {
         int64_t first = -1;
-        for (int q = 0; q < m; ++q) {
-          std::vector<uint32_t> arm;
-          for (uint32_t x : L)
-            if (x != ev.ids[q]) arm.push_back(x);
-          int64_t carrier;
-          auto hit = arm_cache.find(arm);
-          if (hit != arm_cache.end()) carrier = hit->second;
-          else { carrier = desc.descend(arm, minima); arm_cache.emplace(arm, carrier); }
-          int64_t r = dsu.find(carrier);
+        for (uint32_t a = D.arm_base[e]; a < D.arm_base[e + 1]; ++a) {
+          int64_t r = dsu.find(D.carrier[a]);
           if (!bp.count(r)) bp[r] = r;
           if (first < 0) first = r;
-          else { int64_t a = bfind(first), b = bfind(r); if (a != b) bp[b] = a; }
+          else { int64_t x = bfind(first), y = bfind(r); if (x != y) bp[y] = x; }
         }
       }
-      if (any) {
-        std::unordered_map<int64_t, std::vector<int64_t>> groups;
-        for (auto &kv : bp) groups[bfind(kv.first)].push_back(kv.first);
-        std::vector<std::vector<int64_t>> glist;
-        for (auto &kv : groups) {
-          std::sort(kv.second.begin(), kv.second.end());
-          glist.push_back(kv.second);
-        }
-        std::sort(glist.begin(), glist.end());
-        for (auto &members : glist) {
-          std::vector<int64_t> roots;
-          for (int64_t r : members)
-            if (owner[r] >= 0) roots.push_back(owner[r]);
-          std::sort(roots.begin(), roots.end());
-          roots.erase(std::unique(roots.begin(), roots.end()), roots.end());
-          int64_t node;
-          if (roots.empty()) node = T.add(k, 1, cls[i], E[i].level, {});
-          else if (roots.size() == 1) node = roots[0];
-          else node = T.add(k, 2, cls[i], E[i].level, roots);
-          const int64_t base = members[0];
-          for (size_t q = 1; q < members.size(); ++q) dsu.p[dsu.find(members[q])] = dsu.find(base);
-          owner[dsu.find(base)] = node;
-        }
+      std::unordered_map<int64_t, std::vector<int64_t>> groups;
+      for (auto &kv : bp) groups[bfind(kv.first)].push_back(kv.first);
+      std::vector<std::vector<int64_t>> glist;
+      for (auto &kv : groups) {
+        std::sort(kv.second.begin(), kv.second.end());
+        glist.push_back(kv.second);
+      }
+      std::sort(glist.begin(), glist.end());
+      for (auto &members : glist) {
+        std::vector<int64_t> roots;
+        for (int64_t r : members)
+          if (owner[r] >= 0) roots.push_back(owner[r]);
+        std::sort(roots.begin(), roots.end());
+        roots.erase(std::unique(roots.begin(), roots.end()), roots.end());
+        int64_t node;
+        if (roots.empty()) node = T.add(k, 1, D.cls[i], D.level[i], {});
+        else if (roots.size() == 1) node = roots[0];
+        else node = T.add(k, 2, D.cls[i], D.level[i], roots);
+        for (size_t q = 1; q < members.size(); ++q) dsu.p[dsu.find(members[q])] = dsu.find(members[0]);
+        owner[dsu.find(members[0])] = node;
       }
       i = j;
     }
+    tm.mark("tower forests (host)");
   }
-  T.descent_steps = desc.steps;
+  T.descent_steps += desc ? desc->steps : 0;
   T.orders = K;
   T.status = 0;
   T.valid = true;
-  tm.mark("tower");
   info->node_count = (int64_t)T.order.size();
   info->child_count = (int64_t)T.children.size();
   info->descent_steps = T.descent_steps;

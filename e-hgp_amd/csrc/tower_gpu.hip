@@ -1,0 +1,438 @@
+// Device arm resolution for the hgp_reduced tower.
+//
+// For order k the minima are the catalogue records of closed rank k and the
+// saddles those of closed rank k+1.  Every strict arm (closed label minus one
+// shell point) is resolved to the minimum of its component:
+//  * the labels of the minima live in an open-addressing hash table;
+//  * an arm that is not a minimum descends: Welzl's algorithm gives the FP64
+//    miniball of the facet, its support and containment are certified with
+//    the predicates of predicates.hpp, an LBVH ball query lists the points of
+//    the closed ball, and the facet trades its least support point for the
+//    least point strictly inside (the miniball strictly shrinks);
+//  * anything the certified filters cannot decide is flagged for the host's
+//    exact descent.
+#include "pipeline.hpp"
+#include "predicates.hpp"
+#include "scan.hpp"
+#include "supports.hpp"
+#include "traverse.hpp"
+#include "tower_gpu.hpp"
+
+namespace mhgp {
+
+namespace {
+
+constexpr int kMaxLabel = 11;
+
+__device__ inline uint64_t label_hash(const uint32_t *ids, int k) {
+  uint64_t h = 0x9e3779b97f4a7c15ull ^ (uint64_t)k;
+  for (int i = 0; i < k; ++i) {
+    h ^= ids[i] + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+    h *= 0xff51afd7ed558ccdull;
+    h ^= h >> 33;
+  }
+  return h | 1ull;  // 0 marks an empty slot
+}
+
+__device__ inline int event_label(const EventRec &e, uint32_t *out) {
+  const int m = e.meta & 0xff, ni = (e.meta >> 16) & 0xff;
+  int k = 0;
+  for (int i = 0; i < m; ++i) out[k++] = e.ids[i];
+  for (int i = 0; i < ni; ++i) out[k++] = e.interior[i];
+  for (int i = 1; i < k; ++i) {  // insertion sort
+    uint32_t v = out[i];
+    int j = i - 1;
+    while (j >= 0 && out[j] > v) { out[j + 1] = out[j]; j--; }
+    out[j + 1] = v;
+  }
+  return k;
+}
+
+__device__ inline int rank_of(const EventRec &e) { return (e.meta >> 8) & 0xff; }
+
+// Select records of one closed rank in canonical order (positions flagged).
+__global__ void k_flag_rank(const EventRec *__restrict__ ev, const uint32_t *__restrict__ order, int64_t n,
+                            int rank, uint32_t *flag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const EventRec &e = ev[order[i]];
+  flag[i] = (rank_of(e) == rank && !((e.meta >> 24) & 1)) ? 1u : 0u;
+}
+
+__global__ void k_compact(const uint32_t *__restrict__ order, const uint32_t *__restrict__ scan, int64_t n,
+                          uint32_t *out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t s = scan[i];
+  uint32_t prev = i ? scan[i - 1] : 0;
+  if (s != prev) out[s - 1] = (uint32_t)i;  // sorted position
+}
+
+__global__ void k_insert_minima(const EventRec *__restrict__ ev, const uint32_t *__restrict__ order,
+                                const uint32_t *__restrict__ pos, int64_t count, int k,
+                                uint32_t *labels, uint64_t *table_keys, uint32_t *table_vals,
+                                uint64_t mask) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint32_t lab[kMaxLabel];
+  int kk = event_label(ev[order[pos[i]]], lab);
+  for (int q = 0; q < k; ++q) labels[(size_t)i * k + q] = q < kk ? lab[q] : kInvalid;
+  uint64_t h = label_hash(lab, k);
+  uint64_t slot = h & mask;
+  while (true) {
+    unsigned long long prev = atomicCAS((unsigned long long *)&table_keys[slot], 0ull, (unsigned long long)h);
+    if (prev == 0ull) { table_vals[slot] = (uint32_t)i; return; }
+    slot = (slot + 1) & mask;
+  }
+}
+
+__device__ int64_t lookup(const uint32_t *F, int k, const uint64_t *table_keys, const uint32_t *table_vals,
+                          uint64_t mask, const uint32_t *labels) {
+  uint64_t h = label_hash(F, k);
+  uint64_t slot = h & mask;
+  while (true) {
+    uint64_t key = table_keys[slot];
+    if (key == 0) return -1;
+    if (key == h) {
+      uint32_t v = ((volatile uint32_t *)table_vals)[slot];
+      bool same = true;
+      for (int q = 0; q < k; ++q) same &= labels[(size_t)v * k + q] == F[q];
+      if (same) return v;
+    }
+    slot = (slot + 1) & mask;
+  }
+}
+
+struct Ball {
+  V3 c;
+  double r2;
+  int m;
+  uint32_t sup[4];  // indices into the facet
+};
+
+__device__ inline V3 P3(const double *X, const double *Y, const double *Z, uint32_t id) { return {X[id], Y[id], Z[id]}; }
+
+// circumball of 0..4 facet points with centre in their affine hull (FP64)
+__device__ bool ball_from(const V3 *pts, const uint32_t *R, int r, Ball &b) {
+  b.m = r;
+  for (int i = 0; i < r; ++i) b.sup[i] = R[i];
+  if (r == 0) { b.c = {0, 0, 0}; b.r2 = -1; return true; }
+  const V3 p0 = pts[R[0]];
+  if (r == 1) { b.c = p0; b.r2 = 0; return true; }
+  const V3 a = sub(pts[R[1]], p0);
+  V3 off;
+  if (r == 2) {
+    off = {0.5 * a.x, 0.5 * a.y, 0.5 * a.z};
+  } else if (r == 3) {
+    const V3 bb = sub(pts[R[2]], p0);
+    double aa = dot(a, a), b2 = dot(bb, bb), ab = dot(a, bb);
+    double D = aa * b2 - ab * ab;
+    if (!(D > 1e-14 * (aa * b2 + ab * ab))) return false;
+    double al = b2 * (aa - ab), be = aa * (b2 - ab);
+    off = {(al * a.x + be * bb.x) / (2 * D), (al * a.y + be * bb.y) / (2 * D), (al * a.z + be * bb.z) / (2 * D)};
+  } else {
+    const V3 bb = sub(pts[R[2]], p0), d = sub(pts[R[3]], p0);
+    V3 bxd = cross(bb, d), dxa = cross(d, a), axb = cross(a, bb);
+    double V = dot(a, bxd);
+    if (!(fabs(V) > 1e-14 * adot(vabs(a), across(bb, d)))) return false;
+    double aa = dot(a, a), b2 = dot(bb, bb), dd = dot(d, d);
+    off = {(aa * bxd.x + b2 * dxa.x + dd * axb.x) / (2 * V), (aa * bxd.y + b2 * dxa.y + dd * axb.y) / (2 * V),
+           (aa * bxd.z + b2 * dxa.z + dd * axb.z) / (2 * V)};
+  }
+  b.c = {p0.x + off.x, p0.y + off.y, p0.z + off.z};
+  b.r2 = dot(off, off);
+  return true;
+}
+
+__device__ inline bool in_ball(const Ball &b, const V3 &p) {
+  if (b.r2 < 0) return false;
+  V3 d = sub(p, b.c);
+  return dot(d, d) <= b.r2 * (1.0 + 1e-12) + 1e-300;
+}
+
+// Welzl's miniball, recursion unrolled by boundary-set size (<= 4).
+__device__ bool welzl(const V3 *pts, int n, uint32_t *R, int r, Ball &out) {
+  Ball b;
+  if (!ball_from(pts, R, r, b)) return false;
+  if (r == 4) { out = b; return true; }
+  for (int i = 0; i < n; ++i) {
+    if (in_ball(b, pts[i])) continue;
+    R[r] = (uint32_t)i;
+    Ball nb;
+    if (!welzl(pts, i, R, r + 1, nb)) return false;
+    b = nb;
+    // restore the boundary prefix for the remaining points
+  }
+  out = b;
+  return true;
+}
+
+// certified side of point q against the support sphere (ids relative to sup[0])
+__device__ int cert_side(const double *X, const double *Y, const double *Z, const uint32_t *sid, int m,
+                         uint32_t q) {
+  for (int i = 0; i < m; ++i)
+    if (sid[i] == q) return 0;
+  if (m == 1) return 1;
+  const V3 o = P3(X, Y, Z, sid[0]);
+  const V3 p = sub(P3(X, Y, Z, q), o);
+  if (m == 2) { PairSphere s; s.init(sub(P3(X, Y, Z, sid[1]), o)); return s.side(p); }
+  if (m == 3) {
+    TriSphere s;
+    s.init(sub(P3(X, Y, Z, sid[1]), o), sub(P3(X, Y, Z, sid[2]), o));
+    return s.side(p);
+  }
+  TetSphere s;
+  s.init(sub(P3(X, Y, Z, sid[1]), o), sub(P3(X, Y, Z, sid[2]), o), sub(P3(X, Y, Z, sid[3]), o));
+  return s.side(p);
+}
+
+// certified well-centring of a support (+1 yes, -1 no, 0 undecided)
+__device__ int cert_well(const double *X, const double *Y, const double *Z, const uint32_t *sid, int m) {
+  if (m <= 2) return 1;
+  const V3 o = P3(X, Y, Z, sid[0]);
+  if (m == 3) {
+    TriSphere s;
+    return s.init(sub(P3(X, Y, Z, sid[1]), o), sub(P3(X, Y, Z, sid[2]), o));
+  }
+  TetSphere s;
+  return s.init(sub(P3(X, Y, Z, sid[1]), o), sub(P3(X, Y, Z, sid[2]), o), sub(P3(X, Y, Z, sid[3]), o));
+}
+
+struct ArmParams {
+  const EventRec *ev;
+  const uint32_t *order;
+  const uint32_t *saddle_pos;  // sorted positions of the saddles of this order
+  const uint32_t *arm_base;    // first arm of each saddle (exclusive scan of sizes)
+  int64_t saddles;
+  int k;
+  const uint64_t *table_keys;
+  const uint32_t *table_vals;
+  uint64_t mask;
+  const uint32_t *labels;
+  const double *X, *Y, *Z;
+  const Node *nodes;
+  const double *LX, *LY, *LZ;
+  const uint32_t *leaf_id;
+  int64_t n;
+  uint32_t *carrier;  // per arm: minimum index, or kInvalid (host)
+  unsigned long long *steps;
+};
+
+__global__ void __launch_bounds__(64) k_resolve_arms(ArmParams A) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= A.saddles) return;
+  const EventRec &e = A.ev[A.order[A.saddle_pos[s]]];
+  uint32_t L[kMaxLabel];
+  const int len = event_label(e, L);
+  const int m = e.meta & 0xff;
+  const int k = A.k;
+  uint64_t local_steps = 0;
+  for (int a = 0; a < m; ++a) {
+    uint32_t F[kMaxLabel];
+    int f = 0;
+    for (int i = 0; i < len; ++i)
+      if (L[i] != e.ids[a]) F[f++] = L[i];
+    uint32_t result = kInvalid;
+    if (k == 1) result = F[0];  // order one: the arm is a point, its own minimum
+    for (int step = 0; step < 4096 && result == kInvalid; ++step) {
+      int64_t hit = lookup(F, k, A.table_keys, A.table_vals, A.mask, A.labels);
+      if (hit >= 0) { result = (uint32_t)hit; break; }
+      // miniball of the facet
+      V3 pts[kMaxLabel];
+      for (int i = 0; i < k; ++i) pts[i] = P3(A.X, A.Y, A.Z, F[i]);
+      uint32_t R[4];
+      Ball b;
+      if (!welzl(pts, k, R, 0, b)) break;
+      uint32_t sid[4];
+      for (int i = 0; i < b.m; ++i) sid[i] = F[b.sup[i]];
+      if (cert_well(A.X, A.Y, A.Z, sid, b.m) <= 0) break;
+      // every facet point must be certainly inside, or a support point
+      bool ok = true;
+      for (int i = 0; i < k && ok; ++i) {
+        bool on_support = false;
+        for (int q = 0; q < b.m; ++q) on_support |= sid[q] == F[i];
+        if (!on_support) ok = cert_side(A.X, A.Y, A.Z, sid, b.m, F[i]) < 0;
+      }
+      if (!ok) break;
+      // closed-ball points outside the facet
+      uint32_t xin = kInvalid;
+      bool undecided = false, shell = false;
+      const double q[3] = {b.c.x, b.c.y, b.c.z};
+      const double R2 = b.r2 * (1.0 + 1e-9) + 1e-300;
+      uint32_t stack[64];
+      int sp = 0;
+      stack[sp++] = 0;
+      while (sp && !undecided) {
+        const Node &N = A.nodes[stack[--sp]];
+        uint32_t ch[2] = {N.left, N.right};
+        for (int c = 0; c < 2; ++c) {
+          uint32_t x = ch[c];
+          if (x & kLeafBit) {
+            uint32_t lp = x & ~kLeafBit;
+            double dx = A.LX[lp] - q[0], dy = A.LY[lp] - q[1], dz = A.LZ[lp] - q[2];
+            if (dx * dx + dy * dy + dz * dz > R2) continue;
+            uint32_t id = A.leaf_id[lp];
+            bool inF = false;
+            for (int i = 0; i < k; ++i) inF |= F[i] == id;
+            if (inF) continue;
+            int sd = cert_side(A.X, A.Y, A.Z, sid, b.m, id);
+            if (sd < 0) xin = min(xin, id);
+            else if (sd == 0) undecided = true;
+          } else if (box_d2(q, A.nodes[x].lo, A.nodes[x].hi) <= R2) {
+            if (sp < 64) stack[sp++] = x; else undecided = true;
+          }
+        }
+      }
+      (void)shell;
+      if (undecided || xin == kInvalid) break;  // host decides (shell points or a missing minimum)
+      uint32_t y = sid[0];
+      for (int i = 1; i < b.m; ++i) y = min(y, sid[i]);
+      // F := F - y + xin, kept sorted
+      int w = 0;
+      uint32_t G[kMaxLabel];
+      bool placed = false;
+      for (int i = 0; i < k; ++i) {
+        if (F[i] == y) continue;
+        if (!placed && xin < F[i]) { G[w++] = xin; placed = true; }
+        G[w++] = F[i];
+      }
+      if (!placed) G[w++] = xin;
+      for (int i = 0; i < k; ++i) F[i] = G[i];
+      local_steps++;
+    }
+    A.carrier[A.arm_base[s] + a] = result;
+  }
+  if (local_steps) atomicAdd(A.steps, (unsigned long long)local_steps);
+}
+
+__global__ void k_arm_sizes(const EventRec *__restrict__ ev, const uint32_t *__restrict__ order,
+                            const uint32_t *__restrict__ pos, int64_t count, uint32_t *sizes) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) sizes[i] = ev[order[pos[i]]].meta & 0xff;
+}
+
+__global__ void k_saddle_meta(const EventRec *__restrict__ ev, const uint32_t *__restrict__ order,
+                              const uint32_t *__restrict__ cls, const uint32_t *__restrict__ pos,
+                              int64_t count, uint32_t *out_cls, double *out_level, uint32_t *out_sup) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const EventRec &e = ev[order[pos[i]]];
+  out_cls[i] = cls[pos[i]];
+  out_level[i] = e.level;
+  for (int q = 0; q < 4; ++q) out_sup[4 * i + q] = e.ids[q];
+}
+
+}  // namespace
+
+void tower_order_device(Context &c, int k, TowerOrderData &out) {
+  CatalogBuffers &B = *c.catalog;
+  hipStream_t s = c.stream;
+  const int64_t n = B.count;
+  out = TowerOrderData{};
+  Scanner sc;
+  DevBuf<uint32_t> flag, minima_pos, saddle_pos;
+  flag.ensure(n);
+  // minima of order k (k >= 2)
+  hipLaunchKernelGGL(k_flag_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.events.p, B.order.p, n, k, flag.p);
+  sc.inclusive(flag.p, n, s);
+  uint32_t M = 0;
+  if (n) MHGP_HIP(hipMemcpy(&M, flag.p + n - 1, 4, hipMemcpyDeviceToHost));
+  minima_pos.ensure(M);
+  hipLaunchKernelGGL(k_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.order.p, flag.p, n, minima_pos.p);
+  out.minima = M;
+  uint64_t cap = 1;
+  while (cap < 2ull * M + 16) cap <<= 1;
+  DevBuf<uint64_t> tkeys;
+  DevBuf<uint32_t> tvals, labels;
+  tkeys.ensure(cap);
+  tvals.ensure(cap);
+  labels.ensure((size_t)M * k + 1);
+  MHGP_HIP(hipMemsetAsync(tkeys.p, 0, cap * 8, s));
+  if (M)
+    hipLaunchKernelGGL(k_insert_minima, dim3(blocks_for(M, 256)), dim3(256), 0, s, B.events.p, B.order.p,
+                       minima_pos.p, (int64_t)M, k, labels.p, tkeys.p, tvals.p, cap - 1);
+  // saddles of order k
+  hipLaunchKernelGGL(k_flag_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.events.p, B.order.p, n, k + 1,
+                     flag.p);
+  sc.inclusive(flag.p, n, s);
+  uint32_t S = 0;
+  if (n) MHGP_HIP(hipMemcpy(&S, flag.p + n - 1, 4, hipMemcpyDeviceToHost));
+  saddle_pos.ensure(S);
+  hipLaunchKernelGGL(k_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.order.p, flag.p, n, saddle_pos.p);
+  out.saddles = S;
+  if (S == 0) return;
+  DevBuf<uint32_t> sizes, carrier, scls, ssup;
+  DevBuf<double> slev;
+  DevBuf<uint64_t> steps;
+  sizes.ensure(S);
+  hipLaunchKernelGGL(k_arm_sizes, dim3(blocks_for(S, 256)), dim3(256), 0, s, B.events.p, B.order.p,
+                     saddle_pos.p, (int64_t)S, sizes.p);
+  sc.inclusive(sizes.p, S, s);
+  uint32_t arms = 0;
+  MHGP_HIP(hipMemcpy(&arms, sizes.p + S - 1, 4, hipMemcpyDeviceToHost));
+  // exclusive bases: shift the inclusive scan by one
+  DevBuf<uint32_t> base;
+  base.ensure(S);
+  MHGP_HIP(hipMemsetAsync(base.p, 0, 4, s));
+  if (S > 1) MHGP_HIP(hipMemcpyAsync(base.p + 1, sizes.p, (S - 1) * 4, hipMemcpyDeviceToDevice, s));
+  carrier.ensure(arms);
+  steps.ensure(1);
+  MHGP_HIP(hipMemsetAsync(steps.p, 0, 8, s));
+  ArmParams A{};
+  A.ev = B.events.p; A.order = B.order.p; A.saddle_pos = saddle_pos.p; A.arm_base = base.p;
+  A.saddles = S; A.k = k;
+  A.table_keys = tkeys.p; A.table_vals = tvals.p; A.mask = cap - 1; A.labels = labels.p;
+  A.X = c.X.p; A.Y = c.Y.p; A.Z = c.Z.p;
+  A.nodes = c.nodes.p; A.LX = c.LX.p; A.LY = c.LY.p; A.LZ = c.LZ.p; A.leaf_id = c.leaf_id.p; A.n = c.n;
+  A.carrier = carrier.p;
+  A.steps = (unsigned long long *)steps.p;
+  c.arm_clock.start(s);
+  hipLaunchKernelGGL(k_resolve_arms, dim3(blocks_for(S, 64)), dim3(64), 0, s, A);
+  c.arm_clock.stop(s);
+  MHGP_LAUNCH_CHECK();
+  c.stats.arm_ms += c.arm_clock.ms();
+  c.stats.arm_launches++;
+  // algorithmic bytes: each saddle record and its order/position entries read
+  // once, each arm's carrier written once and one hash probe (key + value)
+  c.stats.arm_bytes += (int64_t)S * ((int64_t)sizeof(EventRec) + 8) + (int64_t)arms * (4 + 12);
+  scls.ensure(S);
+  slev.ensure(S);
+  ssup.ensure(4 * (size_t)S);
+  hipLaunchKernelGGL(k_saddle_meta, dim3(blocks_for(S, 256)), dim3(256), 0, s, B.events.p, B.order.p, B.cls.p,
+                     saddle_pos.p, (int64_t)S, scls.p, slev.p, ssup.p);
+  out.cls.resize(S);
+  out.level.resize(S);
+  out.sup.resize(4 * (size_t)S);
+  out.arm_base.resize(S + 1);
+  out.carrier.resize(arms);
+  MHGP_HIP(hipMemcpy(out.cls.data(), scls.p, S * 4, hipMemcpyDeviceToHost));
+  MHGP_HIP(hipMemcpy(out.level.data(), slev.p, S * 8, hipMemcpyDeviceToHost));
+  MHGP_HIP(hipMemcpy(out.sup.data(), ssup.p, 4 * (size_t)S * 4, hipMemcpyDeviceToHost));
+  MHGP_HIP(hipMemcpy(out.arm_base.data(), base.p, S * 4, hipMemcpyDeviceToHost));
+  out.arm_base[S] = arms;
+  MHGP_HIP(hipMemcpy(out.carrier.data(), carrier.p, (size_t)arms * 4, hipMemcpyDeviceToHost));
+  uint64_t st = 0;
+  MHGP_HIP(hipMemcpy(&st, steps.p, 8, hipMemcpyDeviceToHost));
+  out.steps = (int64_t)st;
+  bool need_labels = false;
+  for (uint32_t x : out.carrier) need_labels |= x == kInvalid;
+  if (need_labels) {
+    // the host descends unresolved arms; it needs the saddle labels and the minima labels
+    out.minima_labels.resize((size_t)M * k);
+    if (M) MHGP_HIP(hipMemcpy(out.minima_labels.data(), labels.p, (size_t)M * k * 4, hipMemcpyDeviceToHost));
+    out.saddle_labels.resize((size_t)S * kMaxLabel, kInvalid);
+    std::vector<uint32_t> pos(S), ord(n);
+    MHGP_HIP(hipMemcpy(pos.data(), saddle_pos.p, S * 4, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < S; ++i) {
+      uint32_t idx;
+      EventRec e;
+      MHGP_HIP(hipMemcpy(&idx, B.order.p + pos[i], 4, hipMemcpyDeviceToHost));
+      MHGP_HIP(hipMemcpy(&e, B.events.p + idx, sizeof(EventRec), hipMemcpyDeviceToHost));
+      int m = e.meta & 0xff, ni = (e.meta >> 16) & 0xff, t = 0;
+      for (int q = 0; q < m; ++q) out.saddle_labels[(size_t)i * kMaxLabel + t++] = e.ids[q];
+      for (int q = 0; q < ni; ++q) out.saddle_labels[(size_t)i * kMaxLabel + t++] = e.interior[q];
+    }
+  }
+}
+
+}  // namespace mhgp

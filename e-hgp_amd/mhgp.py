@@ -40,6 +40,13 @@ class TowerInfo(ctypes.Structure):
                 ("status", ctypes.c_int32)]
 
 
+class Stats(ctypes.Structure):
+    _fields_ = [("anchor_ms", ctypes.c_double), ("anchor_launches", ctypes.c_int64),
+                ("anchor_bytes", ctypes.c_int64), ("arm_ms", ctypes.c_double),
+                ("arm_launches", ctypes.c_int64), ("arm_bytes", ctypes.c_int64),
+                ("gathered_points", ctypes.c_int64), ("events", ctypes.c_int64)]
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
@@ -64,6 +71,7 @@ def lib() -> ctypes.CDLL:
         L.mhgp_supports_fetch.argtypes = [p, p, p, p, p, p]
         L.mhgp_tower.argtypes = [p, i32, ctypes.POINTER(TowerInfo)]
         L.mhgp_tower_fetch.argtypes = [p, p, p, p, p, p, p]
+        L.mhgp_ctx_stats.argtypes = [p, ctypes.POINTER(Stats), i32]
         _lib = L
     return _lib
 
@@ -72,7 +80,7 @@ EXPORTED = [
     "mhgp_last_error", "mhgp_ctx_create", "mhgp_ctx_destroy", "mhgp_ctx_device_bytes",
     "mhgp_cloud_upload", "mhgp_cloud_upload_device", "mhgp_point_count",
     "mhgp_canonical_points", "mhgp_lbvh_snapshot", "mhgp_knn", "mhgp_emst", "mhgp_supports",
-    "mhgp_supports_fetch", "mhgp_tower", "mhgp_tower_fetch",
+    "mhgp_supports_fetch", "mhgp_tower", "mhgp_tower_fetch", "mhgp_ctx_stats",
 ]
 
 
@@ -128,6 +136,11 @@ class Context:
 
     def device_bytes(self) -> int:
         return int(lib().mhgp_ctx_device_bytes(self._h))
+
+    def stats(self, reset: bool = False) -> Stats:
+        st = Stats()
+        _check(lib().mhgp_ctx_stats(self._h, ctypes.byref(st), 1 if reset else 0))
+        return st
 
     def upload(self, points) -> None:
         a = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))

@@ -135,6 +135,23 @@ int mhgp_tower(mhgp_ctx *ctx, int k_max, mhgp_tower_info *info);
 int mhgp_tower_fetch(mhgp_ctx *ctx, int32_t *order, int32_t *kind, int64_t *level_class,
                      double *level, int64_t *child_offset, int64_t *children);
 
+/* Device-time accounting of the context's hot kernels since the last reset,
+ * measured with HIP events on the context stream.  No reference counterpart:
+ * it mirrors the audit counter blocks the reference attaches to its results
+ * (e.g. MortonLbvhDeviceBuildAudit, morton_lbvh_build.hpp:41-118). */
+typedef struct {
+  double anchor_ms;          /* catalogue anchor kernel(s) */
+  int64_t anchor_launches;
+  int64_t anchor_bytes;      /* algorithmic bytes: gathered points read + records written */
+  double arm_ms;             /* tower arm-resolution kernel(s) */
+  int64_t arm_launches;
+  int64_t arm_bytes;         /* algorithmic bytes: saddle records read + carriers written */
+  int64_t gathered_points;   /* neighbour entries gathered for the anchors */
+  int64_t events;            /* catalogue records written by the device */
+} mhgp_stats;
+
+int mhgp_ctx_stats(mhgp_ctx *ctx, mhgp_stats *stats, int reset);
+
 #ifdef __cplusplus
 }
 #endif

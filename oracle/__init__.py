@@ -163,6 +163,29 @@ def catalog(canonical_points, k_max: int, max_support: int = 4) -> list:
     return out
 
 
+_anchor_cloud = {"key": None, "arr": None}
+
+
+def anchor_catalog(canonical_points, k_max: int, anchor: int, radius: float) -> int:
+    """Exact count of the critical supports whose least PointId is `anchor`,
+    enumerated exhaustively among the points within `radius` of it."""
+    L = lib()
+    if not hasattr(L.or_anchor_catalog, "_typed"):
+        L.or_anchor_prepare.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+        L.or_anchor_catalog.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_double]
+        L.or_anchor_catalog.restype = ctypes.c_int64
+        L.or_anchor_catalog._typed = True
+    a = _pts(canonical_points)
+    key = (a.ctypes.data, len(a))
+    if _anchor_cloud["key"] != key:
+        _check(L.or_anchor_prepare(a.ctypes.data, len(a)), "anchor_prepare")
+        _anchor_cloud["key"], _anchor_cloud["arr"] = key, a
+    r = L.or_anchor_catalog(int(k_max), int(anchor), float(radius))
+    if r < 0:
+        raise OracleError(f"anchor catalogue failed with code {r}")
+    return int(r)
+
+
 def support_info(ids) -> tuple:
     """(well_centred, level) of a support on the cloud of the last catalog() call."""
     arr = np.ascontiguousarray(np.asarray(ids, dtype=np.int64))

@@ -775,3 +775,106 @@ int or_support_info(const int64_t *ids, int m, int *well, char *num, char *den, 
   free(S);
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* anchored exhaustive catalogue (CPU baseline of the timing harness)        */
+/* Enumerates, exactly, every support whose least PointId is the anchor among */
+/* the points within `radius` of it, and classifies the neighbourhood.  A    */
+/* FP64 prefilter skips supports whose circumradius exceeds radius / 2 or    */
+/* that are clearly not well-centred; every kept decision is exact.          */
+/* ------------------------------------------------------------------------ */
+
+static ecloud g_anc_cl;
+static int g_anc_valid;
+
+int or_anchor_prepare(const double *xyz, int64_t n) {
+  if (g_anc_valid) { ecloud_free(&g_anc_cl); g_anc_valid = 0; }
+  int rc = ecloud_init(&g_anc_cl, xyz, n);
+  if (rc == OR_OK) g_anc_valid = 1;
+  return rc;
+}
+
+static int f_circum(const double *X, const int64_t *ids, int m, double *r2, int *well) {
+  const double *p = X + 3 * ids[0];
+  double a[3], b[3], d[3];
+  for (int k = 0; k < 3; ++k) a[k] = X[3 * ids[1] + k] - p[k];
+  if (m == 2) { *r2 = 0.25 * (a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); *well = 1; return 1; }
+  for (int k = 0; k < 3; ++k) b[k] = X[3 * ids[2] + k] - p[k];
+  double aa = a[0] * a[0] + a[1] * a[1] + a[2] * a[2], bb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+  double ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+  if (m == 3) {
+    double D = aa * bb - ab * ab;
+    if (!(D > 1e-12 * aa * bb)) return 0;
+    double al = bb * (aa - ab), be = aa * (bb - ab);
+    double c[3];
+    for (int k = 0; k < 3; ++k) c[k] = (al * a[k] + be * b[k]) / (2 * D);
+    *r2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+    *well = al > -1e-9 * fabs(al) && be > -1e-9 * fabs(be) && 2 * D - al - be > -1e-9 * (2 * D);
+    return 1;
+  }
+  for (int k = 0; k < 3; ++k) d[k] = X[3 * ids[3] + k] - p[k];
+  double bxd[3] = {b[1] * d[2] - b[2] * d[1], b[2] * d[0] - b[0] * d[2], b[0] * d[1] - b[1] * d[0]};
+  double dxa[3] = {d[1] * a[2] - d[2] * a[1], d[2] * a[0] - d[0] * a[2], d[0] * a[1] - d[1] * a[0]};
+  double axb[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+  double V = a[0] * bxd[0] + a[1] * bxd[1] + a[2] * bxd[2];
+  if (fabs(V) < 1e-12 * sqrt(aa * bb * (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]))) return 0;
+  double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  double N[3];
+  for (int k = 0; k < 3; ++k) N[k] = aa * bxd[k] + bb * dxa[k] + dd * axb[k];
+  *r2 = (N[0] * N[0] + N[1] * N[1] + N[2] * N[2]) / (4 * V * V);
+  double nv = N[0] * bxd[0] + N[1] * bxd[1] + N[2] * bxd[2];
+  double nw = N[0] * dxa[0] + N[1] * dxa[1] + N[2] * dxa[2];
+  double nz = N[0] * axb[0] + N[1] * axb[1] + N[2] * axb[2];
+  double tol = 1e-9 * (fabs(nv) + fabs(nw) + fabs(nz) + 2 * V * V);
+  *well = nv > -tol && nw > -tol && nz > -tol && 2 * V * V - nv - nw - nz > -tol;
+  return 1;
+}
+
+int64_t or_anchor_catalog(int k_max, int64_t anchor, double radius) {
+  if (!g_anc_valid) return OR_ARGUMENT;
+  const ecloud *cl = &g_anc_cl;
+  const double *X = cl->X;
+  const int64_t n = cl->n;
+  int64_t keff = k_max < n ? k_max : n;
+  int64_t smax = keff + 1 < n ? keff + 1 : n;
+  int64_t *nb = malloc(sizeof(int64_t) * (size_t)n);
+  int64_t m = 0;
+  for (int64_t j = 0; j < n; ++j) {
+    double s = 0;
+    for (int k = 0; k < 3; ++k) { double t = X[3 * j + k] - X[3 * anchor + k]; s += t * t; }
+    if (s <= radius * radius && j != anchor) nb[m++] = j;
+  }
+  sup_t *S = malloc(sizeof(sup_t));
+  int64_t events = 0;
+  const double rmax2 = 0.25 * radius * radius;
+  int64_t ids[4];
+  ids[0] = anchor;
+  for (int sz = 2; sz <= 4 && sz <= smax; ++sz) {
+    int64_t c[3] = {0, 1, 2};
+    int k = sz - 1;
+    if (m < k) break;
+    int done = 0;
+    while (!done) {
+      int ok = 1;
+      for (int q = 0; q < k; ++q) { ids[q + 1] = nb[c[q]]; ok &= nb[c[q]] > anchor; }
+      double r2; int well;
+      if (ok && f_circum(X, ids, sz, &r2, &well) && well && r2 <= rmax2 &&
+          support_analyze(S, cl, ids, sz) && S->well) {
+        int nint = 0, shell = 0;
+        for (int64_t t = 0; t < m && nint + sz <= smax; ++t) {
+          int64_t x = nb[t];
+          int on = 0;
+          for (int q = 1; q < sz; ++q) on |= ids[q] == x;
+          if (on) continue;
+          int sd = support_side(S, cl, x);
+          if (sd < 0) nint++; else if (sd == 0) shell++;
+        }
+        if (nint + sz <= smax && shell == 0) events++;
+      }
+      next_comb(c, k, m, &done);
+    }
+  }
+  free(S);
+  free(nb);
+  return events;
+}
