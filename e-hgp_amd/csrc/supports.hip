@@ -248,6 +248,8 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
     double terr = (kCert * PA / fabs(Pw) + kCert * HA / fabs(H) + 1e-15) * fabs(tau);
     if (sp < 0) {
       cin[side]++;
+      // both sides saturated and the triangle out of rank: nothing left here
+      if (cin[0] >= K && cin[1] >= K && total_in + 3 > smax) return;
       // behind the opposite side: inside until the sweep key reaches -tau
       int o = 1 - side;
       if (nbehind[o] < 16) behind[o][nbehind[o]++] = -tau - terr;
@@ -308,6 +310,30 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
   }
 }
 
+// Cells whose union of closed tangent balls (radius rho'_C) contains gathered
+// point i: cos of the smallest angle between its direction and the cell.
+__device__ bool member_mask(const Params &P, const Work &w, uint32_t i, uint64_t mask[6]) {
+  const double x = w.gx[i], y = w.gy[i], z = w.gz[i];
+  const double dist = sqrt(x * x + y * y + z * z);
+  bool any = false;
+  for (int k = 0; k < 6; ++k) mask[k] = 0;
+  for (int c = 0; c < kCells; ++c) {
+    const double *cd = P.cells + 8 * c;
+    double ct = (x * cd[0] + y * cd[1] + z * cd[2]) / dist;
+    double cm;  // cos(max(theta - delta, 0))
+    if (ct >= cd[3]) cm = 1.0;
+    else {
+      double st = sqrt(fmax(0.0, 1.0 - ct * ct));
+      cm = ct * cd[3] + st * cd[4];
+    }
+    if (dist <= 2.0 * w.rho[c] * cm * (1.0 + 1e-9) + 1e-300) {
+      mask[c >> 6] |= 1ull << (c & 63);
+      any = true;
+    }
+  }
+  return any;
+}
+
 __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   __shared__ uint32_t sM, sK, sflag, sregion;
   __shared__ double smax_rho;
@@ -316,13 +342,48 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   const uint64_t off = P.goff[u];
   const double ux = P.X[u], uy = P.Y[u], uz = P.Z[u];
   const double Rg = P.radius[u];
+  __shared__ uint32_t scan_buf[kBlock];
   if (t == 0) { sM = 0; sK = 0; sflag = 0; sregion = kInvalid; }
-  for (uint32_t i = t; i < G; i += blockDim.x) {
-    uint32_t id = P.glist[off + i];
-    w.gid[i] = id;
-    w.gx[i] = P.X[id] - ux;
-    w.gy[i] = P.Y[id] - uy;
-    w.gz[i] = P.Z[id] - uz;
+  if (G <= 4 * blockDim.x) {
+    // load in increasing distance from the anchor (rank sort in LDS), so that
+    // every scan below meets the points most likely to be inside first
+    for (uint32_t i = t; i < G; i += blockDim.x) {
+      uint32_t id = P.glist[off + i];
+      double dx = P.X[id] - ux, dy = P.Y[id] - uy, dz = P.Z[id] - uz;
+      w.gx[i] = dx * dx + dy * dy + dz * dz;
+    }
+    __syncthreads();
+    uint32_t rank[4];
+    for (int r = 0; r < 4; ++r) {
+      uint32_t i = t + r * blockDim.x;
+      rank[r] = 0;
+      if (i >= G) continue;
+      const double di = w.gx[i];
+      uint32_t cnt = 0;
+      for (uint32_t j = 0; j < G; ++j) {
+        const double dj = w.gx[j];
+        cnt += (dj < di) || (dj == di && j < i);
+      }
+      rank[r] = cnt;
+    }
+    __syncthreads();
+    for (int r = 0; r < 4; ++r) {
+      uint32_t i = t + r * blockDim.x;
+      if (i >= G) continue;
+      uint32_t id = P.glist[off + i];
+      w.gid[rank[r]] = id;
+      w.gx[rank[r]] = P.X[id] - ux;
+      w.gy[rank[r]] = P.Y[id] - uy;
+      w.gz[rank[r]] = P.Z[id] - uz;
+    }
+  } else {
+    for (uint32_t i = t; i < G; i += blockDim.x) {
+      uint32_t id = P.glist[off + i];
+      w.gid[i] = id;
+      w.gx[i] = P.X[id] - ux;
+      w.gy[i] = P.Y[id] - uy;
+      w.gz[i] = P.Z[id] - uz;
+    }
   }
   __syncthreads();
   // 1. direction-cell certificate
@@ -377,30 +438,47 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   __syncthreads();
   if (sflag) return;
   // 2. region membership masks
-  for (uint32_t i = t; i < G; i += blockDim.x) {
-    double x = w.gx[i], y = w.gy[i], z = w.gz[i];
-    double dist = sqrt(x * x + y * y + z * z);
-    uint64_t mask[6] = {0, 0, 0, 0, 0, 0};
-    bool any = false;
-    for (int c = 0; c < kCells; ++c) {
-      const double *cd = P.cells + 8 * c;
-      double ct = (x * cd[0] + y * cd[1] + z * cd[2]) / dist;
-      double cm;  // cos(max(theta - delta, 0))
-      if (ct >= cd[3]) cm = 1.0;
-      else {
-        double st = sqrt(fmax(0.0, 1.0 - ct * ct));
-        cm = ct * cd[3] + st * cd[4];
-      }
-      if (dist <= 2.0 * w.rho[c] * cm * (1.0 + 1e-9) + 1e-300) {
-        mask[c >> 6] |= 1ull << (c & 63);
-        any = true;
+  if (G <= 4 * blockDim.x) {
+    // ordered compaction: members keep the gathered (distance) order
+    const uint32_t chunk = (G + blockDim.x - 1) / blockDim.x;
+    const uint32_t b0 = t * chunk;
+    uint64_t mk[4][6];
+    bool mem[4];
+    uint32_t cnt = 0;
+    for (uint32_t r = 0; r < 4; ++r) {
+      mem[r] = false;
+      if (r < chunk && b0 + r < G) {
+        mem[r] = member_mask(P, w, b0 + r, mk[r]);
+        cnt += mem[r];
       }
     }
-    if (any) {
-      uint32_t slot = atomicAdd(&sM, 1u);
+    scan_buf[t] = cnt;
+    __syncthreads();
+    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+      uint32_t v = t >= o ? scan_buf[t - o] : 0;
+      __syncthreads();
+      scan_buf[t] += v;
+      __syncthreads();
+    }
+    uint32_t slot = scan_buf[t] - cnt;
+    for (uint32_t r = 0; r < 4; ++r) {
+      if (!mem[r]) continue;
       if (slot < P.rcap) {
-        w.ridx[slot] = i;
-        for (int k = 0; k < 6; ++k) w.rmask[6 * slot + k] = mask[k];
+        w.ridx[slot] = b0 + r;
+        for (int k = 0; k < 6; ++k) w.rmask[6 * slot + k] = mk[r][k];
+      }
+      slot++;
+    }
+    if (t == (int)blockDim.x - 1) sM = scan_buf[t];
+  } else {
+    for (uint32_t i = t; i < G; i += blockDim.x) {
+      uint64_t mask[6];
+      if (member_mask(P, w, i, mask)) {
+        uint32_t slot = atomicAdd(&sM, 1u);
+        if (slot < P.rcap) {
+          w.ridx[slot] = i;
+          for (int k = 0; k < 6; ++k) w.rmask[6 * slot + k] = mask[k];
+        }
       }
     }
   }

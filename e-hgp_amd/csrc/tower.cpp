@@ -299,22 +299,30 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
       while (j < D.saddles && D.cls[j] == D.cls[i]) j++;
       if (j == i + 1) {
         // single saddle at this level: distinct carrier roots are the group
-        std::vector<int64_t> members;
-        for (uint32_t a = D.arm_base[i]; a < D.arm_base[i + 1]; ++a) members.push_back(dsu.find(D.carrier[a]));
-        std::sort(members.begin(), members.end());
-        members.erase(std::unique(members.begin(), members.end()), members.end());
-        std::vector<int64_t> roots;
-        for (int64_t r : members)
-          if (owner[r] >= 0) roots.push_back(owner[r]);
-        std::sort(roots.begin(), roots.end());
-        roots.erase(std::unique(roots.begin(), roots.end()), roots.end());
-        if (members.size() > 1 || roots.empty()) {
+        int64_t members[4];
+        int nm = 0;
+        for (uint32_t a = D.arm_base[i]; a < D.arm_base[i + 1]; ++a) {
+          int64_t r = dsu.find(D.carrier[a]);
+          bool seen = false;
+          for (int q = 0; q < nm; ++q) seen |= members[q] == r;
+          if (!seen) members[nm++] = r;
+        }
+        if (nm > 1) {
+          int64_t roots[4];
+          int nr = 0;
+          for (int q = 0; q < nm; ++q)
+            if (owner[members[q]] >= 0) roots[nr++] = owner[members[q]];
+          std::sort(roots, roots + nr);
+          nr = (int)(std::unique(roots, roots + nr) - roots);
           int64_t node;
-          if (roots.empty()) node = T.add(k, 1, D.cls[i], D.level[i], {});
-          else if (roots.size() == 1) node = roots[0];
-          else node = T.add(k, 2, D.cls[i], D.level[i], roots);
-          for (size_t q = 1; q < members.size(); ++q) dsu.p[members[q]] = members[0];
+          if (nr == 0) node = T.add(k, 1, D.cls[i], D.level[i], {});
+          else if (nr == 1) node = roots[0];
+          else node = T.add(k, 2, D.cls[i], D.level[i], std::vector<int64_t>(roots, roots + nr));
+          for (int q = 1; q < nm; ++q) dsu.p[members[q]] = members[0];
           owner[members[0]] = node;
+        } else if (owner[members[0]] < 0) {
+          // all arms in one latent minimum cannot happen in general position
+          owner[members[0]] = T.add(k, 1, D.cls[i], D.level[i], {});
         }
         i = j;
         continue;

@@ -47,6 +47,13 @@ class Stats(ctypes.Structure):
                 ("gathered_points", ctypes.c_int64), ("events", ctypes.c_int64)]
 
 
+class RequestResult(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("event_count", ctypes.c_int64), ("node_count", ctypes.c_int64),
+                ("child_count", ctypes.c_int64), ("queue_ms", ctypes.c_double),
+                ("service_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
@@ -72,6 +79,13 @@ def lib() -> ctypes.CDLL:
         L.mhgp_tower.argtypes = [p, i32, ctypes.POINTER(TowerInfo)]
         L.mhgp_tower_fetch.argtypes = [p, p, p, p, p, p, p]
         L.mhgp_ctx_stats.argtypes = [p, ctypes.POINTER(Stats), i32]
+        L.mhgp_server_create.argtypes = [i32, i32, i64, ctypes.POINTER(p)]
+        L.mhgp_server_submit.argtypes = [p, p, i64, i32, ctypes.POINTER(i64)]
+        L.mhgp_server_wait.argtypes = [p, i64, ctypes.POINTER(RequestResult)]
+        L.mhgp_server_fetch.argtypes = [p, i64, p, p, p, p, p, p]
+        L.mhgp_server_release.argtypes = [p, i64]
+        L.mhgp_server_destroy.argtypes = [p]
+        L.mhgp_server_destroy.restype = None
         _lib = L
     return _lib
 
@@ -81,6 +95,8 @@ EXPORTED = [
     "mhgp_cloud_upload", "mhgp_cloud_upload_device", "mhgp_point_count",
     "mhgp_canonical_points", "mhgp_lbvh_snapshot", "mhgp_knn", "mhgp_emst", "mhgp_supports",
     "mhgp_supports_fetch", "mhgp_tower", "mhgp_tower_fetch", "mhgp_ctx_stats",
+    "mhgp_server_create", "mhgp_server_submit", "mhgp_server_wait", "mhgp_server_fetch",
+    "mhgp_server_release", "mhgp_server_destroy",
 ]
 
 
@@ -211,3 +227,61 @@ class Context:
                                       _ptr(off), _ptr(kids)))
         return {"order": order, "kind": kind, "level_class": cls, "level": level,
                 "child_offset": off, "children": kids[: info.child_count], "info": info}
+
+
+def _tower_arrays(node_count: int, child_count: int):
+    return (np.zeros(node_count, dtype=np.int32), np.zeros(node_count, dtype=np.int32),
+            np.zeros(node_count, dtype=np.int64), np.zeros(node_count, dtype=np.float64),
+            np.zeros(node_count + 1, dtype=np.int64), np.zeros(max(child_count, 1), dtype=np.int64))
+
+
+class Server:
+    """Continuous-batching server on one GPU: worker slots with their own
+    streams, memory admission under a cap (default 80% of HBM)."""
+
+    def __init__(self, device: int = 0, slots: int = 4, memory_cap_bytes: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().mhgp_server_create(device, slots, memory_cap_bytes, ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            lib().mhgp_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, points: np.ndarray, k_max: int) -> int:
+        xyz = np.ascontiguousarray(points, dtype=np.float64)
+        if xyz.ndim != 2 or xyz.shape[1] != 3:
+            raise ValueError("points must be (n, 3)")
+        t = ctypes.c_int64()
+        _check(lib().mhgp_server_submit(self._h, _ptr(xyz), xyz.shape[0], k_max, ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int) -> RequestResult:
+        r = RequestResult()
+        _check(lib().mhgp_server_wait(self._h, ticket, ctypes.byref(r)))
+        return r
+
+    def fetch(self, ticket: int, result: RequestResult) -> dict:
+        if result.code != 0:
+            raise MhgpError(result.code, f"request {ticket} failed")
+        order, kind, cls, level, off, kids = _tower_arrays(result.node_count, result.child_count)
+        _check(lib().mhgp_server_fetch(self._h, ticket, _ptr(order), _ptr(kind), _ptr(cls),
+                                       _ptr(level), _ptr(off), _ptr(kids)))
+        return {"order": order, "kind": kind, "level_class": cls, "level": level,
+                "child_offset": off, "children": kids[: result.child_count]}
+
+    def release(self, ticket: int) -> None:
+        _check(lib().mhgp_server_release(self._h, ticket))

@@ -152,6 +152,38 @@ typedef struct {
 
 int mhgp_ctx_stats(mhgp_ctx *ctx, mhgp_stats *stats, int reset);
 
+/* Continuous-batching server: a pool of worker slots (each one context and
+ * one stream) that runs upload -> EMST -> catalogue -> tower for submitted
+ * clouds.  A request is admitted when a slot is free and its predicted
+ * device memory fits under memory_cap_bytes (<= 0: 80% of the device's
+ * HBM); a request larger than the cap runs alone.  The reference has no
+ * server; this is the serving loop that sits above its launchers. */
+typedef struct mhgp_server mhgp_server;
+
+typedef struct {
+  int32_t code;         /* MHGP_OK or the error of the failed stage */
+  int32_t status;       /* 0 exact, 1 conditional */
+  int64_t event_count;  /* catalogue records */
+  int64_t node_count;   /* tower nodes (sizes for mhgp_server_fetch) */
+  int64_t child_count;
+  double queue_ms;      /* submit -> admitted */
+  double service_ms;    /* admitted -> tower on the host */
+  double latency_ms;    /* submit -> tower on the host */
+} mhgp_request_result;
+
+int mhgp_server_create(int device, int slots, int64_t memory_cap_bytes, mhgp_server **out);
+/* Copies xyz (n x 3 binary64); k_max in 1..10. */
+int mhgp_server_submit(mhgp_server *srv, const double *xyz, int64_t n, int k_max,
+                       int64_t *ticket);
+/* Blocks until the request has finished. */
+int mhgp_server_wait(mhgp_server *srv, int64_t ticket, mhgp_request_result *result);
+/* Tower of a finished request, laid out as mhgp_tower_fetch. */
+int mhgp_server_fetch(mhgp_server *srv, int64_t ticket, int32_t *order, int32_t *kind,
+                      int64_t *level_class, double *level, int64_t *child_offset,
+                      int64_t *children);
+int mhgp_server_release(mhgp_server *srv, int64_t ticket);
+void mhgp_server_destroy(mhgp_server *srv);
+
 #ifdef __cplusplus
 }
 #endif
