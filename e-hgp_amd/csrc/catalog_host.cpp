@@ -103,12 +103,12 @@ void supports(Context &c, int k_max, mhgp_catalog_info *info) {
       if ((uint64_t)total > B.events.cap) {
         DevBuf<EventRec> grown;
         grown.ensure(total);
-        MHGP_HIP(hipMemcpy(grown.p, B.events.p, B.count * sizeof(EventRec), hipMemcpyDeviceToDevice));
+        copy_wait(grown.p, B.events.p, B.count * sizeof(EventRec), hipMemcpyDeviceToDevice, c.stream);
         std::swap(grown.p, B.events.p);
         std::swap(grown.cap, B.events.cap);
       }
-      MHGP_HIP(hipMemcpy(B.events.p + B.count, settled.data(), settled.size() * sizeof(EventRec),
-                         hipMemcpyHostToDevice));
+      copy_wait(B.events.p + B.count, settled.data(), settled.size() * sizeof(EventRec),
+                         hipMemcpyHostToDevice, c.stream);
       B.count = total;
       for (const EventRec &e : settled) diagnostics += (e.meta >> 24) & 1;
     }
@@ -131,9 +131,9 @@ void supports_fetch(Context &c, uint32_t *ids, uint32_t *meta, uint32_t *interio
   std::vector<EventRec> ev(n);
   std::vector<uint32_t> order(n), cls(n);
   if (n) {
-    MHGP_HIP(hipMemcpy(ev.data(), B.events.p, n * sizeof(EventRec), hipMemcpyDeviceToHost));
-    MHGP_HIP(hipMemcpy(order.data(), B.order.p, n * 4, hipMemcpyDeviceToHost));
-    MHGP_HIP(hipMemcpy(cls.data(), B.cls.p, n * 4, hipMemcpyDeviceToHost));
+    copy_wait(ev.data(), B.events.p, n * sizeof(EventRec), hipMemcpyDeviceToHost, c.stream);
+    copy_wait(order.data(), B.order.p, n * 4, hipMemcpyDeviceToHost, c.stream);
+    copy_wait(cls.data(), B.cls.p, n * 4, hipMemcpyDeviceToHost, c.stream);
   }
   const int width = std::max(B.s_max - 1, 1);
   for (int64_t i = 0; i < n; ++i) {

@@ -79,4 +79,12 @@ __host__ __device__ inline uint64_t dbits(double x) {
 
 inline int blocks_for(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
+// Copy on the context stream and wait for it.  Context streams are created
+// non-blocking, so a plain hipMemcpy (null stream) would not be ordered after
+// the kernels queued on them.
+inline void copy_wait(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s) {
+  MHGP_HIP(hipMemcpyAsync(dst, src, bytes, kind, s));
+  MHGP_HIP(hipStreamSynchronize(s));
+}
+
 }  // namespace mhgp

@@ -326,8 +326,8 @@ void emst_host(Context &c, uint32_t *edges_out) {
     if (any) {
       std::vector<uint32_t> label(n);
       std::vector<uint64_t> ce(n);
-      MHGP_HIP(hipMemcpy(label.data(), B.label.p, n * 4, hipMemcpyDeviceToHost));
-      MHGP_HIP(hipMemcpy(ce.data(), B.comp_e.p, n * 8, hipMemcpyDeviceToHost));
+      copy_wait(label.data(), B.label.p, n * 4, hipMemcpyDeviceToHost, s);
+      copy_wait(ce.data(), B.comp_e.p, n * 8, hipMemcpyDeviceToHost, s);
       const ExactCloud &ex = exact_cloud(c);
       const std::vector<double> &xyz = c.host->xyz;
       for (int64_t l = 0; l < n; ++l) {
@@ -342,7 +342,7 @@ void emst_host(Context &c, uint32_t *edges_out) {
         }
         ce[l] = best;
       }
-      MHGP_HIP(hipMemcpy(B.comp_e.p, ce.data(), n * 8, hipMemcpyHostToDevice));
+      copy_wait(B.comp_e.p, ce.data(), n * 8, hipMemcpyHostToDevice, s);
     }
     hipLaunchKernelGGL(k_hook, dim3(G), dim3(T), 0, s, n, B.label.p, B.comp_e.p, hook.p);
     hipLaunchKernelGGL(k_emit, dim3(G), dim3(T), 0, s, n, B.label.p, B.comp_e.p, hook.p,
@@ -365,7 +365,7 @@ void emst_host(Context &c, uint32_t *edges_out) {
   StageTimer tm(s);
   // kappa order: FP64 squared length, exact comparison inside rounding ties
   std::vector<uint32_t> e(2 * (n - 1));
-  MHGP_HIP(hipMemcpy(e.data(), B.edges.p, e.size() * 4, hipMemcpyDeviceToHost));
+  copy_wait(e.data(), B.edges.p, e.size() * 4, hipMemcpyDeviceToHost, s);
   std::vector<double> xyz(3 * n);
   if (c.host) xyz = c.host->xyz; else canonical_points(c, xyz.data(), nullptr);
   std::vector<std::pair<double, uint64_t>> ord(n - 1);

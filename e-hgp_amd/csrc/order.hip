@@ -83,7 +83,7 @@ void order_catalogue(Context &c, CatalogBuffers &B) {
   if (na > cap) throw std::runtime_error("too many ambiguous level neighbours");
   if (na) {
     std::vector<uint32_t> pos(na);
-    MHGP_HIP(hipMemcpy(pos.data(), B.ambiguous.p, na * 4, hipMemcpyDeviceToHost));
+    copy_wait(pos.data(), B.ambiguous.p, na * 4, hipMemcpyDeviceToHost, s);
     std::sort(pos.begin(), pos.end());
     const ExactCloud &ex = exact_cloud(c);
     for (size_t a = 0; a < pos.size();) {
@@ -93,10 +93,10 @@ void order_catalogue(Context &c, CatalogBuffers &B) {
       const int64_t lo = pos[a] - 1, hi = pos[b];
       const int64_t len = hi - lo + 1;
       std::vector<uint32_t> idx(len);
-      MHGP_HIP(hipMemcpy(idx.data(), B.order.p + lo, len * 4, hipMemcpyDeviceToHost));
+      copy_wait(idx.data(), B.order.p + lo, len * 4, hipMemcpyDeviceToHost, s);
       std::vector<EventRec> recs(len);
       for (int64_t k = 0; k < len; ++k)
-        MHGP_HIP(hipMemcpy(&recs[k], B.events.p + idx[k], sizeof(EventRec), hipMemcpyDeviceToHost));
+        copy_wait(&recs[k], B.events.p + idx[k], sizeof(EventRec), hipMemcpyDeviceToHost, s);
       std::vector<BigInt> num(len), den(len);
       for (int64_t k = 0; k < len; ++k) exact_level_of(ex, recs[k], num[k], den[k]);
       std::vector<int64_t> perm(len);
@@ -119,11 +119,11 @@ void order_catalogue(Context &c, CatalogBuffers &B) {
       if (lo == 0) links[0] = 1;
       else {
         uint32_t l0;
-        MHGP_HIP(hipMemcpy(&l0, B.cls.p + lo, 4, hipMemcpyDeviceToHost));
+        copy_wait(&l0, B.cls.p + lo, 4, hipMemcpyDeviceToHost, s);
         links[0] = l0;  // the link into the run was provable
       }
-      MHGP_HIP(hipMemcpy(B.order.p + lo, new_idx.data(), len * 4, hipMemcpyHostToDevice));
-      MHGP_HIP(hipMemcpy(B.cls.p + lo, links.data(), len * 4, hipMemcpyHostToDevice));
+      copy_wait(B.order.p + lo, new_idx.data(), len * 4, hipMemcpyHostToDevice, s);
+      copy_wait(B.cls.p + lo, links.data(), len * 4, hipMemcpyHostToDevice, s);
       a = b + 1;
     }
   }

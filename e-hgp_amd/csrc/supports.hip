@@ -719,7 +719,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
   if (B.cells.cap < 8 * kCells) {
     auto cells = make_cells();
     B.cells.ensure(8 * kCells);
-    MHGP_HIP(hipMemcpy(B.cells.p, cells.data(), cells.size() * 8, hipMemcpyHostToDevice));
+    copy_wait(B.cells.p, cells.data(), cells.size() * 8, hipMemcpyHostToDevice, s);
   }
   // initial gather radius: twice the distance to the min(32, n-1)-th neighbour
   const int kq = (int)std::min<int64_t>(32, n - 1);
@@ -736,11 +736,11 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
   // anchors in Morton order: neighbouring workgroups share cache lines
   {
     std::vector<uint32_t> lid(n);
-    MHGP_HIP(hipMemcpy(lid.data(), c.leaf_id.p, n * 4, hipMemcpyDeviceToHost));
+    copy_wait(lid.data(), c.leaf_id.p, n * 4, hipMemcpyDeviceToHost, s);
     anchors = lid;
   }
   double bounds[6];
-  MHGP_HIP(hipMemcpy(bounds, c.bounds.p, sizeof(bounds), hipMemcpyDeviceToHost));
+  copy_wait(bounds, c.bounds.p, sizeof(bounds), hipMemcpyDeviceToHost, s);
   uint64_t event_cap = std::max<uint64_t>(1024, (uint64_t)n * (smax >= 11 ? 640 : 64 * smax));
   uint64_t fb_cap = std::max<uint64_t>(4096, (uint64_t)n / 4);
   uint64_t reg_cap = std::max<uint64_t>(1 << 20, (uint64_t)n * 64);
@@ -788,14 +788,14 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     P.big = 0; P.gcap = kGatherCap; P.rcap = kRegionCap;
     tm.mark("catalogue gather");
     uint64_t ev_before = 0;
-    MHGP_HIP(hipMemcpy(&ev_before, B.counters.p, 8, hipMemcpyDeviceToHost));
+    copy_wait(&ev_before, B.counters.p, 8, hipMemcpyDeviceToHost, s);
     c.anchor_clock.start(s);
     hipLaunchKernelGGL(k_anchor_lds, dim3(cnt), dim3(kBlock), 0, s, P);
     c.anchor_clock.stop(s);
     MHGP_LAUNCH_CHECK();
     {
       uint64_t ev_after = 0;
-      MHGP_HIP(hipMemcpy(&ev_after, B.counters.p, 8, hipMemcpyDeviceToHost));
+      copy_wait(&ev_after, B.counters.p, 8, hipMemcpyDeviceToHost, s);
       c.stats.anchor_ms += c.anchor_clock.ms();
       c.stats.anchor_launches++;
       c.stats.gathered_points += (int64_t)tot;
@@ -861,7 +861,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
         if (again.empty()) break;
         // regather the widened big anchors
         const uint32_t ac = (uint32_t)again.size();
-        MHGP_HIP(hipMemcpy(B.radius.p, rad.data(), n * 8, hipMemcpyHostToDevice));
+        copy_wait(B.radius.p, rad.data(), n * 8, hipMemcpyHostToDevice, s);
         MHGP_HIP(hipMemcpyAsync(B.anchors.p, again.data(), ac * 4, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_gather<false>, dim3(blocks_for(ac, 128)), dim3(128), 0, s, n, c.nodes.p,
                            c.LX.p, c.LY.p, c.LZ.p, c.leaf_id.p, c.leaf_pos.p, B.anchors.p, ac,
@@ -880,14 +880,14 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
       }
       big.clear();
     }
-    MHGP_HIP(hipMemcpy(B.radius.p, rad.data(), n * 8, hipMemcpyHostToDevice));
+    copy_wait(B.radius.p, rad.data(), n * 8, hipMemcpyHostToDevice, s);
     pending = next;
   }
   tm.mark("catalogue big anchors");
   uint32_t ovf = 0;
   uint64_t ctr[8];
-  MHGP_HIP(hipMemcpy(&ovf, overflow.p, 4, hipMemcpyDeviceToHost));
-  MHGP_HIP(hipMemcpy(ctr, B.counters.p, sizeof(ctr), hipMemcpyDeviceToHost));
+  copy_wait(&ovf, overflow.p, 4, hipMemcpyDeviceToHost, s);
+  copy_wait(ctr, B.counters.p, sizeof(ctr), hipMemcpyDeviceToHost, s);
   if (ovf) throw std::runtime_error("LBVH traversal stack overflow in the gather");
   if (ctr[0] > event_cap || ctr[1] > fb_cap || ctr[2] > reg_cap)
     throw std::runtime_error("catalogue queue capacity exceeded");
@@ -896,8 +896,8 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
   B.count = (int64_t)ctr[0];
   fbs.resize(ctr[1]);
   fb_region.resize(ctr[2]);
-  if (ctr[1]) MHGP_HIP(hipMemcpy(fbs.data(), B.fallbacks.p, ctr[1] * sizeof(FallbackRec), hipMemcpyDeviceToHost));
-  if (ctr[2]) MHGP_HIP(hipMemcpy(fb_region.data(), B.fb_region.p, ctr[2] * 4, hipMemcpyDeviceToHost));
+  if (ctr[1]) copy_wait(fbs.data(), B.fallbacks.p, ctr[1] * sizeof(FallbackRec), hipMemcpyDeviceToHost, s);
+  if (ctr[2]) copy_wait(fb_region.data(), B.fb_region.p, ctr[2] * 4, hipMemcpyDeviceToHost, s);
   tm.mark("catalogue download");
   info->candidate_count = (int64_t)ctr[3];
   info->anchors_widened = widened;

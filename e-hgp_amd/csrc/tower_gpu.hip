@@ -150,18 +150,33 @@ __device__ inline bool in_ball(const Ball &b, const V3 &p) {
   return dot(d, d) <= b.r2 * (1.0 + 1e-12) + 1e-300;
 }
 
-// Welzl's miniball, recursion unrolled by boundary-set size (<= 4).
-__device__ bool welzl(const V3 *pts, int n, uint32_t *R, int r, Ball &out) {
+// Welzl's miniball in its fixed-dimension loop form: level r of the loop nest
+// is the recursion level with r boundary points (at most 4 in 3-D), visited in
+// the same order as the recursive algorithm.  No recursion, so the kernel's
+// private stack has a static size.
+__device__ bool welzl(const V3 *pts, int n, Ball &out) {
+  uint32_t R[4];
   Ball b;
-  if (!ball_from(pts, R, r, b)) return false;
-  if (r == 4) { out = b; return true; }
+  ball_from(pts, R, 0, b);
   for (int i = 0; i < n; ++i) {
     if (in_ball(b, pts[i])) continue;
-    R[r] = (uint32_t)i;
-    Ball nb;
-    if (!welzl(pts, i, R, r + 1, nb)) return false;
-    b = nb;
-    // restore the boundary prefix for the remaining points
+    R[0] = (uint32_t)i;
+    if (!ball_from(pts, R, 1, b)) return false;
+    for (int j = 0; j < i; ++j) {
+      if (in_ball(b, pts[j])) continue;
+      R[1] = (uint32_t)j;
+      if (!ball_from(pts, R, 2, b)) return false;
+      for (int l = 0; l < j; ++l) {
+        if (in_ball(b, pts[l])) continue;
+        R[2] = (uint32_t)l;
+        if (!ball_from(pts, R, 3, b)) return false;
+        for (int q = 0; q < l; ++q) {
+          if (in_ball(b, pts[q])) continue;
+          R[3] = (uint32_t)q;
+          if (!ball_from(pts, R, 4, b)) return false;
+        }
+      }
+    }
   }
   out = b;
   return true;
@@ -240,9 +255,8 @@ __global__ void __launch_bounds__(64) k_resolve_arms(ArmParams A) {
       // miniball of the facet
       V3 pts[kMaxLabel];
       for (int i = 0; i < k; ++i) pts[i] = P3(A.X, A.Y, A.Z, F[i]);
-      uint32_t R[4];
       Ball b;
-      if (!welzl(pts, k, R, 0, b)) break;
+      if (!welzl(pts, k, b)) break;
       uint32_t sid[4];
       for (int i = 0; i < b.m; ++i) sid[i] = F[b.sup[i]];
       if (cert_well(A.X, A.Y, A.Z, sid, b.m) <= 0) break;
@@ -336,7 +350,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   hipLaunchKernelGGL(k_flag_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.events.p, B.order.p, n, k, flag.p);
   sc.inclusive(flag.p, n, s);
   uint32_t M = 0;
-  if (n) MHGP_HIP(hipMemcpy(&M, flag.p + n - 1, 4, hipMemcpyDeviceToHost));
+  if (n) copy_wait(&M, flag.p + n - 1, 4, hipMemcpyDeviceToHost, s);
   minima_pos.ensure(M);
   hipLaunchKernelGGL(k_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.order.p, flag.p, n, minima_pos.p);
   out.minima = M;
@@ -356,7 +370,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
                      flag.p);
   sc.inclusive(flag.p, n, s);
   uint32_t S = 0;
-  if (n) MHGP_HIP(hipMemcpy(&S, flag.p + n - 1, 4, hipMemcpyDeviceToHost));
+  if (n) copy_wait(&S, flag.p + n - 1, 4, hipMemcpyDeviceToHost, s);
   saddle_pos.ensure(S);
   hipLaunchKernelGGL(k_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.order.p, flag.p, n, saddle_pos.p);
   out.saddles = S;
@@ -369,7 +383,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
                      saddle_pos.p, (int64_t)S, sizes.p);
   sc.inclusive(sizes.p, S, s);
   uint32_t arms = 0;
-  MHGP_HIP(hipMemcpy(&arms, sizes.p + S - 1, 4, hipMemcpyDeviceToHost));
+  copy_wait(&arms, sizes.p + S - 1, 4, hipMemcpyDeviceToHost, s);
   // exclusive bases: shift the inclusive scan by one
   DevBuf<uint32_t> base;
   base.ensure(S);
@@ -405,29 +419,29 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   out.sup.resize(4 * (size_t)S);
   out.arm_base.resize(S + 1);
   out.carrier.resize(arms);
-  MHGP_HIP(hipMemcpy(out.cls.data(), scls.p, S * 4, hipMemcpyDeviceToHost));
-  MHGP_HIP(hipMemcpy(out.level.data(), slev.p, S * 8, hipMemcpyDeviceToHost));
-  MHGP_HIP(hipMemcpy(out.sup.data(), ssup.p, 4 * (size_t)S * 4, hipMemcpyDeviceToHost));
-  MHGP_HIP(hipMemcpy(out.arm_base.data(), base.p, S * 4, hipMemcpyDeviceToHost));
+  copy_wait(out.cls.data(), scls.p, S * 4, hipMemcpyDeviceToHost, s);
+  copy_wait(out.level.data(), slev.p, S * 8, hipMemcpyDeviceToHost, s);
+  copy_wait(out.sup.data(), ssup.p, 4 * (size_t)S * 4, hipMemcpyDeviceToHost, s);
+  copy_wait(out.arm_base.data(), base.p, S * 4, hipMemcpyDeviceToHost, s);
   out.arm_base[S] = arms;
-  MHGP_HIP(hipMemcpy(out.carrier.data(), carrier.p, (size_t)arms * 4, hipMemcpyDeviceToHost));
+  copy_wait(out.carrier.data(), carrier.p, (size_t)arms * 4, hipMemcpyDeviceToHost, s);
   uint64_t st = 0;
-  MHGP_HIP(hipMemcpy(&st, steps.p, 8, hipMemcpyDeviceToHost));
+  copy_wait(&st, steps.p, 8, hipMemcpyDeviceToHost, s);
   out.steps = (int64_t)st;
   bool need_labels = false;
   for (uint32_t x : out.carrier) need_labels |= x == kInvalid;
   if (need_labels) {
     // the host descends unresolved arms; it needs the saddle labels and the minima labels
     out.minima_labels.resize((size_t)M * k);
-    if (M) MHGP_HIP(hipMemcpy(out.minima_labels.data(), labels.p, (size_t)M * k * 4, hipMemcpyDeviceToHost));
+    if (M) copy_wait(out.minima_labels.data(), labels.p, (size_t)M * k * 4, hipMemcpyDeviceToHost, s);
     out.saddle_labels.resize((size_t)S * kMaxLabel, kInvalid);
     std::vector<uint32_t> pos(S), ord(n);
-    MHGP_HIP(hipMemcpy(pos.data(), saddle_pos.p, S * 4, hipMemcpyDeviceToHost));
+    copy_wait(pos.data(), saddle_pos.p, S * 4, hipMemcpyDeviceToHost, s);
     for (uint32_t i = 0; i < S; ++i) {
       uint32_t idx;
       EventRec e;
-      MHGP_HIP(hipMemcpy(&idx, B.order.p + pos[i], 4, hipMemcpyDeviceToHost));
-      MHGP_HIP(hipMemcpy(&e, B.events.p + idx, sizeof(EventRec), hipMemcpyDeviceToHost));
+      copy_wait(&idx, B.order.p + pos[i], 4, hipMemcpyDeviceToHost, s);
+      copy_wait(&e, B.events.p + idx, sizeof(EventRec), hipMemcpyDeviceToHost, s);
       int m = e.meta & 0xff, ni = (e.meta >> 16) & 0xff, t = 0;
       for (int q = 0; q < m; ++q) out.saddle_labels[(size_t)i * kMaxLabel + t++] = e.ids[q];
       for (int q = 0; q < ni; ++q) out.saddle_labels[(size_t)i * kMaxLabel + t++] = e.interior[q];
