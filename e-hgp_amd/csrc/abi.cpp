@@ -42,6 +42,8 @@ int guarded(mhgp::Context *c, F &&f) {
     return fail(c, MHGP_E_ARGUMENT, e.what(), false);
   } catch (const mhgp::StateError &e) {
     return fail(c, MHGP_E_STATE, e.what(), false);
+  } catch (const mhgp::UnsupportedError &e) {
+    return fail(c, MHGP_E_UNSUPPORTED, e.what(), false);
   } catch (const mhgp::HipError &e) {
     return fail(c, MHGP_E_DEVICE, e.what(), true);
   } catch (const std::exception &e) {

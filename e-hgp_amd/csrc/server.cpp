@@ -122,6 +122,8 @@ void Server::worker_main() {
       status = MHGP_E_NONFINITE;
     } catch (const std::invalid_argument &e) {
       status = MHGP_E_ARGUMENT;
+    } catch (const UnsupportedError &e) {
+      status = MHGP_E_UNSUPPORTED;
     } catch (const HipError &e) {
       status = MHGP_E_DEVICE;
       std::lock_guard<std::mutex> g(mu);

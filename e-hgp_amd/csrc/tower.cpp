@@ -246,6 +246,12 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
   if (!c.catalog || !c.catalog->valid) throw StateError("build the catalogue first");
   CatalogBuffers &B = *c.catalog;
   if (k_max < 1 || k_max > B.k_max) throw std::invalid_argument("k_max exceeds the catalogue's k_max");
+  // The direct construction needs every relevant sphere to have shell == support;
+  // the reference refuses the journal otherwise (decision
+  // no_journal_relevant_extra_shell_diagnostics, direct_morse_event_journal.cpp:587-592).
+  if (B.info.diagnostic_count > 0)
+    throw UnsupportedError("the catalogue has " + std::to_string(B.info.diagnostic_count) +
+                           " relevant extra-shell diagnostics: the direct Morse tower is not defined");
   StageTimer tm(c.stream);
   const int64_t n = c.n;
   const int K = (int)std::min<int64_t>(k_max, n);

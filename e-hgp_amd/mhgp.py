@@ -25,6 +25,7 @@ class MhgpError(RuntimeError):
 
 
 E_ARGUMENT, E_DUPLICATE, E_NONFINITE, E_DEVICE, E_POISONED, E_STATE = -1, -2, -3, -4, -5, -6
+E_CAPACITY, E_UNSUPPORTED, E_INTERNAL = -7, -8, -9
 
 
 class CatalogInfo(ctypes.Structure):

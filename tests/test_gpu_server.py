@@ -27,7 +27,7 @@ def _clouds():
         (workloads.uniform_cloud(300, seed=11), 4),
         (workloads.jittered_lattice(2000, side=13, sigma=0.1, seed=3), 10),
         (workloads.uniform_cloud(900, seed=12), 10),
-        (workloads.gaussian_mixture(1500, blobs=5, seed=4), 10),
+        (workloads.gaussian_mixture(1500, blobs=5, snap_bits=None, seed=4), 10),
         (workloads.uniform_cloud(64, seed=13), 3),
     ]
 
@@ -62,3 +62,29 @@ def test_server_memory_cap_serialises_and_reports_errors():
         assert r2.code == mhgp.E_DUPLICATE
         _same(srv.fetch(t1, r1), srv.fetch(t3, r3))
         _same(srv.fetch(t1, r1), _plain(good, 5))
+
+
+def _grid_cloud():
+    """3 x 3 x 3 integer grid: many cospherical quadruples, so the catalogue
+    holds extra-shell diagnostics and the direct tower does not exist."""
+    g = np.arange(3, dtype=np.float64)
+    return np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+
+
+def test_degenerate_cloud_refused_alike():
+    pts = _grid_cloud()
+    print("server degenerate grid n=27 k=3", flush=True)
+    with mhgp.Context(0) as ctx:
+        ctx.upload(pts)
+        info = ctx.supports(3, fetch=False)
+        assert info.diagnostic_count > 0
+        with pytest.raises(mhgp.MhgpError) as err:
+            ctx.tower(3)
+        assert err.value.code == mhgp.E_UNSUPPORTED
+    with mhgp.Server(0, slots=2) as srv:
+        t_bad = srv.submit(pts, 3)
+        t_good = srv.submit(workloads.uniform_cloud(200, seed=5), 3)
+        assert srv.wait(t_bad).code == mhgp.E_UNSUPPORTED
+        r = srv.wait(t_good)
+        assert r.code == 0
+        _same(srv.fetch(t_good, r), _plain(workloads.uniform_cloud(200, seed=5), 3))
