@@ -72,6 +72,73 @@ def cpu_baseline(pts, k_max, budget_s=20.0):
     }
 
 
+def copy_calibration(mib: int) -> None:
+    """A plain elementwise kernel of known traffic (reads and writes `mib` MiB
+    of float32 once each), profiled beside the catalogue kernel to scale the
+    FETCH_SIZE / WRITE_SIZE counters into bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.add(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4, "launches": 3}))
+
+
+def served_stream(k_values=(4, 10), count=48, rate_hz=None, slots=4, seed=7):
+    """Poisson stream of clouds (sizes log-uniform in [2 000, 50 000]) through
+    the continuous-batching server, against the same clouds run one after
+    another on one context (the plain path)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(2_000), np.log(50_000), size=count)).astype(int)
+    ks = rng.choice(np.asarray(k_values), size=count)
+    clouds = [workloads.jittered_lattice(int(m), side=int(np.ceil((m * 1.3) ** (1 / 3))), sigma=0.1,
+                                         seed=1000 + i) for i, m in enumerate(sizes)]
+    ctx = mhgp.Context(0)
+    run_step(ctx, clouds[0], int(ks[0]))  # warm
+    t0 = time.perf_counter()
+    plain_lat = []
+    for pts, k in zip(clouds, ks):
+        a = time.perf_counter()
+        run_step(ctx, pts, int(k))
+        plain_lat.append(1e3 * (time.perf_counter() - a))
+    plain_s = time.perf_counter() - t0
+    ctx.close()
+    plain_rate = count / plain_s
+    rate = rate_hz or 0.9 * plain_rate * min(slots, 2)
+    gaps = rng.exponential(1.0 / rate, size=count)
+    srv = mhgp.Server(0, slots=slots)
+    warm = srv.submit(clouds[0], int(ks[0]))
+    srv.wait(warm)
+    srv.release(warm)
+    t0 = time.perf_counter()
+    tickets, due = [], t0
+    for pts, k, g in zip(clouds, ks, gaps):
+        due += g
+        while time.perf_counter() < due:
+            time.sleep(min(0.001, max(0.0, due - time.perf_counter())))
+        tickets.append(srv.submit(pts, int(k)))
+    lat = []
+    for t in tickets:
+        r = srv.wait(t)
+        if r.code != 0:
+            raise mhgp.MhgpError(r.code, "served request failed")
+        lat.append(r.latency_ms)
+        srv.release(t)
+    served_s = time.perf_counter() - t0
+    srv.close()
+    return {
+        "clouds": count, "slots": slots, "offered_rate_hz": round(rate, 2),
+        "served_clouds_per_s": round(count / served_s, 3),
+        "plain_clouds_per_s": round(plain_rate, 3),
+        "served_p50_ms": round(float(np.percentile(lat, 50)), 2),
+        "served_p99_ms": round(float(np.percentile(lat, 99)), 2),
+        "plain_p99_ms": round(float(np.percentile(plain_lat, 99)), 2),
+    }
+
+
 def traffic_from_profile():
     path = os.path.join(ROOT, "profiles", "anchor_kernel_c2_traffic.json")
     if not os.path.exists(path):
@@ -88,7 +155,13 @@ def main() -> None:
     ap.add_argument("--n", type=int, default=50_000)
     ap.add_argument("--kmax", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--served", action="store_true", help="add the served-stream leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain kernel of known traffic (counter scaling)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
@@ -152,6 +225,8 @@ def main() -> None:
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(pts, args.kmax),
     }
     ctx.close()
+    if args.served:
+        line["config"]["served"] = served_stream()
     print(json.dumps(line))
 
 

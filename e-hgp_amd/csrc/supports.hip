@@ -205,10 +205,12 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
                              uint32_t wi, uint32_t *region_slot) {
   const uint64_t *mv = w.rmask;  // masks indexed by region slot, looked up by caller
   (void)mv;
-  TriSphere f;
-  const int wc = f.init(gpt(w, vi), gpt(w, wi));
+  // the sphere is built from its points in PointId order, so its FP64 level
+  // (and every later tetrahedron's) does not depend on the enumeration order
   uint32_t ids[2] = {w.gid[vi], w.gid[wi]};
   if (ids[0] > ids[1]) { uint32_t t = ids[0]; ids[0] = ids[1]; ids[1] = t; uint32_t q = vi; vi = wi; wi = q; }
+  TriSphere f;
+  const int wc = f.init(gpt(w, vi), gpt(w, wi));
   if (!f.nondegenerate()) {
     // collinear or undecided: a triangle cannot be a support; an undecided
     // face goes to the host together with its apexes

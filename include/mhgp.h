@@ -111,7 +111,9 @@ int mhgp_supports(mhgp_ctx *ctx, int k_max, mhgp_catalog_info *info);
  * (level, rank, support).  ids: 4 per event (unused = UINT32_MAX);
  * meta: size | rank << 8 | interior_count << 16 | degenerate << 24;
  * interior: s_max-1 ids per event (unused = UINT32_MAX);
- * level: FP64 value nearest the exact squared radius;
+ * level: FP64 approximation of the exact squared radius (relative error
+ *   below 1e-12, a deterministic function of the support; exact order and
+ *   equality are carried by level_class);
  * level_class: dense rank of the exact squared radius (equal classes share it). */
 int mhgp_supports_fetch(mhgp_ctx *ctx, uint32_t *ids, uint32_t *meta, uint32_t *interior,
                         double *level, int64_t *level_class);
