@@ -58,7 +58,9 @@ struct CatalogBuffers {
   DevBuf<uint64_t> sort_keys;
   DevBuf<uint32_t> order;
   DevBuf<uint32_t> cls;
-  DevBuf<uint32_t> ambiguous;     // sorted positions whose level is not provably above the previous
+  DevBuf<uint32_t> ambiguous;     // open-run starts, and the runs left to the host (start, length)
+  int64_t open_runs = 0;          // runs the FP64 intervals left open
+  int64_t host_runs = 0;          // of those, runs ordered exactly on the host
   mhgp_catalog_info info{};
 };
 
