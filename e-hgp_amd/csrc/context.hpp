@@ -48,6 +48,7 @@ struct Context {
   DevBuf<uint32_t> wit;       // 6 witnesses per internal node (lo xyz, hi xyz)
   DevBuf<uint32_t> counter;
   RadixSorter sorter;
+  Scanner scanner;
   mhgp_stats stats{};
 
   // device time of one launch window on the context stream

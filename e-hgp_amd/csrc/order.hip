@@ -256,8 +256,7 @@ void order_catalogue(Context &c, CatalogBuffers &B) {
     MHGP_HIP(hipStreamSynchronize(s));
   }
   tm.mark("catalogue level runs (host exact)");
-  Scanner sc;
-  sc.inclusive(B.cls.p, n, s);
+  c.scanner.inclusive(B.cls.p, n, s);
   hipLaunchKernelGGL(k_class_from_scan, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.cls.p, n);
   MHGP_LAUNCH_CHECK();
   tm.mark("catalogue level classes");

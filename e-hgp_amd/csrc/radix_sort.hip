@@ -2,8 +2,9 @@
 //
 // Pass structure (one pass per digit):
 //   1. tile histograms: each 1024-key tile counts its digits in LDS;
-//   2. one-workgroup scan over [digit][tile] giving every tile its base offset
-//      for every digit (digit-major, so equal digits keep tile order);
+//   2. multi-block inclusive scan over [digit][tile]; the entry before
+//      (digit, tile) is that tile's base offset for the digit (digit-major,
+//      so equal digits keep tile order);
 //   3. scatter: one 64-lane wavefront per tile walks its keys in order, ranks
 //      equal digits inside each 64-key chunk with 8 ballots, and keeps a
 //      per-digit running count in LDS, which makes the pass stable.
@@ -32,42 +33,18 @@ __global__ void rs_histogram(const uint64_t *__restrict__ keys, int64_t n, int s
   for (int d = threadIdx.x; d < kDigits; d += blockDim.x) counts[(size_t)d * tiles + blockIdx.x] = h[d];
 }
 
-// Exclusive scan of counts (digit-major) in place; flags[0] = 1 when one digit
-// holds every key (the pass is the identity permutation).
-__global__ void rs_scan(uint32_t *__restrict__ counts, int64_t total, int64_t n, int tiles,
-                        uint32_t *__restrict__ flags) {
-  __shared__ uint32_t part[1024];
+// flags[0] = 1 when one digit holds every key (the pass is the identity
+// permutation); incl is the inclusive scan of the digit-major counts.
+__global__ void rs_identity(const uint32_t *__restrict__ incl, int64_t n, int tiles, uint32_t *flags) {
   __shared__ uint32_t full;
-  const int t = threadIdx.x;
-  if (t == 0) full = 0;
-  const int64_t per = (total + blockDim.x - 1) / blockDim.x;
-  const int64_t b = t * per, e = (b + per < total) ? b + per : total;
-  uint32_t s = 0;
-  for (int64_t i = b; i < e; ++i) s += counts[i];
-  part[t] = s;
+  const int d = threadIdx.x;
+  if (d == 0) full = 0;
   __syncthreads();
-  // Hillis-Steele inclusive scan over the per-thread partial sums.
-  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-    uint32_t v = (t >= off) ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  uint32_t run = part[t] - s;
-  for (int64_t i = b; i < e; ++i) {
-    uint32_t c = counts[i];
-    counts[i] = run;
-    run += c;
-  }
+  const uint32_t lo = d == 0 ? 0u : incl[(size_t)d * tiles - 1];
+  const uint32_t hi = incl[(size_t)(d + 1) * tiles - 1];
+  if (hi - lo == (uint32_t)n) full = 1;
   __syncthreads();
-  // a digit holding all keys shows as (offset(d) == 0 and offset(d+1) == n)
-  for (int d = t; d < kDigits; d += blockDim.x) {
-    uint32_t lo = counts[(size_t)d * tiles];
-    uint32_t hi = (d + 1 < kDigits) ? counts[(size_t)(d + 1) * tiles] : (uint32_t)n;
-    if (hi - lo == (uint32_t)n) full = 1;
-  }
-  __syncthreads();
-  if (t == 0) flags[0] = full;
+  if (d == 0) flags[0] = full;
 }
 
 __global__ void __launch_bounds__(64) rs_scatter(const uint64_t *__restrict__ kin,
@@ -86,7 +63,11 @@ __global__ void __launch_bounds__(64) rs_scatter(const uint64_t *__restrict__ ki
     }
     return;
   }
-  for (int d = lane; d < kDigits; d += 64) run[d] = offsets[(size_t)d * tiles + blockIdx.x];
+  // exclusive offset of (digit, tile) = inclusive scan at the previous entry
+  for (int d = lane; d < kDigits; d += 64) {
+    const size_t e = (size_t)d * tiles + blockIdx.x;
+    run[d] = e ? offsets[e - 1] : 0u;
+  }
   __syncthreads();
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int c = 0; c < kTile; c += 64) {
@@ -131,8 +112,8 @@ void RadixSorter::sort(uint64_t *keys, uint32_t *vals, int64_t n, int begin_bit,
   for (int shift = begin_bit; shift < end_bit; shift += 8) {
     hipLaunchKernelGGL(rs_histogram, dim3(tiles), dim3(256), 0, stream, ka, n, shift, counts.p,
                        tiles);
-    hipLaunchKernelGGL(rs_scan, dim3(1), dim3(1024), 0, stream, counts.p,
-                       (int64_t)kDigits * tiles, n, tiles, flags.p);
+    scanner.inclusive(counts.p, (int64_t)kDigits * tiles, stream);
+    hipLaunchKernelGGL(rs_identity, dim3(1), dim3(kDigits), 0, stream, counts.p, n, tiles, flags.p);
     hipLaunchKernelGGL(rs_scatter, dim3(tiles), dim3(64), 0, stream, ka, va, kb, vb, n, shift,
                        counts.p, tiles, flags.p);
     MHGP_LAUNCH_CHECK();

@@ -1,6 +1,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "scan.hpp"
 
 namespace mhgp {
 
@@ -10,6 +11,7 @@ struct RadixSorter {
   DevBuf<uint32_t> tmp_vals;
   DevBuf<uint32_t> counts;
   DevBuf<uint32_t> flags;
+  Scanner scanner;
   void sort(uint64_t *keys, uint32_t *vals, int64_t n, int begin_bit, int end_bit,
             hipStream_t stream);
 };

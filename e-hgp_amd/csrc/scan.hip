@@ -1,5 +1,7 @@
 // Three-phase inclusive scan: per-block scans with block totals, a scan of
 // the totals (recursively), then a uniform add of the block offsets.
+#include <stdexcept>
+
 #include "scan.hpp"
 
 namespace mhgp {
@@ -45,19 +47,17 @@ __global__ void k_add_offsets(uint32_t *data, int64_t n, const uint32_t *sums) {
 
 }  // namespace
 
-void Scanner::inclusive(uint32_t *data, int64_t n, hipStream_t s) {
+void Scanner::inclusive(uint32_t *data, int64_t n, hipStream_t s, int depth) {
   if (n <= 0) return;
+  if (depth >= 4) throw std::runtime_error("scan too deep");
   const int64_t blocks = (n + kTileItems - 1) / kTileItems;
-  DevBuf<uint32_t> local;
-  uint32_t *bs = local.ensure(blocks);
+  uint32_t *bs = level[depth].ensure(blocks);
   hipLaunchKernelGGL(k_block_scan, dim3(blocks), dim3(kScanBlock), 0, s, data, n, bs);
   if (blocks > 1) {
-    Scanner inner;
-    inner.inclusive(bs, blocks, s);
+    inclusive(bs, blocks, s, depth + 1);
     hipLaunchKernelGGL(k_add_offsets, dim3(blocks), dim3(256), 0, s, data, n, bs);
   }
   MHGP_LAUNCH_CHECK();
-  MHGP_HIP(hipStreamSynchronize(s));  // local buffers die with this scope
 }
 
 }  // namespace mhgp

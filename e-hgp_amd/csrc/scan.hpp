@@ -5,10 +5,10 @@
 namespace mhgp {
 
 // In-place inclusive prefix sum of n uint32 values (three-phase, any n).
+// Stream-ordered: the level buffers persist between calls, nothing waits.
 struct Scanner {
-  DevBuf<uint32_t> sums;
-  DevBuf<uint32_t> sums2;
-  void inclusive(uint32_t *data, int64_t n, hipStream_t s);
+  DevBuf<uint32_t> level[4];
+  void inclusive(uint32_t *data, int64_t n, hipStream_t s, int depth = 0);
 };
 
 }  // namespace mhgp

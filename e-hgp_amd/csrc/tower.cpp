@@ -16,6 +16,8 @@
 //    root grows silently, two or more roots create a multifusion.
 #include <algorithm>
 #include <cmath>
+#include <exception>
+#include <thread>
 #include <numeric>
 #include <unordered_map>
 
@@ -242,6 +244,91 @@ struct Dsu {
 
 }  // namespace
 
+// Level-ordered contraction of one order: carrier DSU over the minima, each
+// class owning at most one reduced root.  Node ids are local to the order.
+void contract_order(int k, int64_t n, TowerOrderData &D, TowerState &F) {
+  // carrier DSU; each class may own a reduced root
+  const int64_t M = k == 1 ? n : (int64_t)D.minima;
+  std::vector<int64_t> owner(M, -1);
+  if (k == 1)
+    for (int64_t p = 0; p < n; ++p) owner[p] = F.add(1, 0, -1, 0.0, {});
+  Dsu dsu(M);
+  std::vector<int64_t> touched, bp_parent;
+  std::unordered_map<int64_t, int64_t> bp;
+  for (uint32_t i = 0; i < D.saddles;) {
+    uint32_t j = i;
+    while (j < D.saddles && D.cls[j] == D.cls[i]) j++;
+    if (j == i + 1) {
+      // single saddle at this level: distinct carrier roots are the group
+      int64_t members[4];
+      int nm = 0;
+      for (uint32_t a = D.arm_base[i]; a < D.arm_base[i + 1]; ++a) {
+        int64_t r = dsu.find(D.carrier[a]);
+        bool seen = false;
+        for (int q = 0; q < nm; ++q) seen |= members[q] == r;
+        if (!seen) members[nm++] = r;
+      }
+      if (nm > 1) {
+        int64_t roots[4];
+        int nr = 0;
+        for (int q = 0; q < nm; ++q)
+          if (owner[members[q]] >= 0) roots[nr++] = owner[members[q]];
+        std::sort(roots, roots + nr);
+        nr = (int)(std::unique(roots, roots + nr) - roots);
+        int64_t node;
+        if (nr == 0) node = F.add(k, 1, D.cls[i], D.level[i], {});
+        else if (nr == 1) node = roots[0];
+        else node = F.add(k, 2, D.cls[i], D.level[i], std::vector<int64_t>(roots, roots + nr));
+        for (int q = 1; q < nm; ++q) dsu.p[members[q]] = members[0];
+        owner[members[0]] = node;
+      } else if (owner[members[0]] < 0) {
+        // all arms in one latent minimum cannot happen in general position
+        owner[members[0]] = F.add(k, 1, D.cls[i], D.level[i], {});
+      }
+      i = j;
+      continue;
+    }
+    // several saddles at one exact level: contract them together
+    bp.clear();
+    auto bfind = [&](int64_t x) {
+      while (bp[x] != x) { bp[x] = bp[bp[x]]; x = bp[x]; }
+      return x;
+    };
+    for (uint32_t e = i; e < j; ++e) {
+      int64_t first = -1;
+      for (uint32_t a = D.arm_base[e]; a < D.arm_base[e + 1]; ++a) {
+        int64_t r = dsu.find(D.carrier[a]);
+        if (!bp.count(r)) bp[r] = r;
+        if (first < 0) first = r;
+        else { int64_t x = bfind(first), y = bfind(r); if (x != y) bp[y] = x; }
+      }
+    }
+    std::unordered_map<int64_t, std::vector<int64_t>> groups;
+    for (auto &kv : bp) groups[bfind(kv.first)].push_back(kv.first);
+    std::vector<std::vector<int64_t>> glist;
+    for (auto &kv : groups) {
+      std::sort(kv.second.begin(), kv.second.end());
+      glist.push_back(kv.second);
+    }
+    std::sort(glist.begin(), glist.end());
+    for (auto &members : glist) {
+      std::vector<int64_t> roots;
+      for (int64_t r : members)
+        if (owner[r] >= 0) roots.push_back(owner[r]);
+      std::sort(roots.begin(), roots.end());
+      roots.erase(std::unique(roots.begin(), roots.end()), roots.end());
+      int64_t node;
+      if (roots.empty()) node = F.add(k, 1, D.cls[i], D.level[i], {});
+      else if (roots.size() == 1) node = roots[0];
+      else node = F.add(k, 2, D.cls[i], D.level[i], roots);
+      for (size_t q = 1; q < members.size(); ++q) dsu.p[dsu.find(members[q])] = dsu.find(members[0]);
+      owner[dsu.find(members[0])] = node;
+    }
+    i = j;
+  }
+  D = TowerOrderData{};  // release the order's arrays
+}
+
 void tower(Context &c, int k_max, mhgp_tower_info *info) {
   if (!c.catalog || !c.catalog->valid) throw StateError("build the catalogue first");
   CatalogBuffers &B = *c.catalog;
@@ -260,8 +347,19 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
   T = TowerState{};
   std::unique_ptr<Descender> desc;  // host exact descents, built on demand
   std::vector<double> xyz;
+  std::vector<TowerOrderData> Ds(K);
+  std::vector<TowerState> forests(K);
+  std::vector<std::exception_ptr> errors(K);
+  std::vector<std::thread> workers;
+  struct Joiner {
+    std::vector<std::thread> &w;
+    ~Joiner() {
+      for (auto &t : w)
+        if (t.joinable()) t.join();
+    }
+  } joiner{workers};
   for (int k = 1; k <= K; ++k) {
-    TowerOrderData D;
+    TowerOrderData &D = Ds[k - 1];
     tower_order_device(c, k, D);
     tm.mark("tower arms (device)");
     T.descent_steps += D.steps;
@@ -292,86 +390,30 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
         }
       tm.mark("tower arms (host exact)");
     }
-    // carrier DSU; each class may own a reduced root
-    const int64_t M = k == 1 ? n : (int64_t)D.minima;
-    std::vector<int64_t> owner(M, -1);
-    if (k == 1)
-      for (int64_t p = 0; p < n; ++p) owner[p] = T.add(1, 0, -1, 0.0, {});
-    Dsu dsu(M);
-    std::vector<int64_t> touched, bp_parent;
-    std::unordered_map<int64_t, int64_t> bp;
-    for (uint32_t i = 0; i < D.saddles;) {
-      uint32_t j = i;
-      while (j < D.saddles && D.cls[j] == D.cls[i]) j++;
-      if (j == i + 1) {
-        // single saddle at this level: distinct carrier roots are the group
-        int64_t members[4];
-        int nm = 0;
-        for (uint32_t a = D.arm_base[i]; a < D.arm_base[i + 1]; ++a) {
-          int64_t r = dsu.find(D.carrier[a]);
-          bool seen = false;
-          for (int q = 0; q < nm; ++q) seen |= members[q] == r;
-          if (!seen) members[nm++] = r;
-        }
-        if (nm > 1) {
-          int64_t roots[4];
-          int nr = 0;
-          for (int q = 0; q < nm; ++q)
-            if (owner[members[q]] >= 0) roots[nr++] = owner[members[q]];
-          std::sort(roots, roots + nr);
-          nr = (int)(std::unique(roots, roots + nr) - roots);
-          int64_t node;
-          if (nr == 0) node = T.add(k, 1, D.cls[i], D.level[i], {});
-          else if (nr == 1) node = roots[0];
-          else node = T.add(k, 2, D.cls[i], D.level[i], std::vector<int64_t>(roots, roots + nr));
-          for (int q = 1; q < nm; ++q) dsu.p[members[q]] = members[0];
-          owner[members[0]] = node;
-        } else if (owner[members[0]] < 0) {
-          // all arms in one latent minimum cannot happen in general position
-          owner[members[0]] = T.add(k, 1, D.cls[i], D.level[i], {});
-        }
-        i = j;
-        continue;
+    // the order's contraction runs on the host while the device resolves the next order
+    workers.emplace_back([&, k] {
+      try {
+        contract_order(k, n, Ds[k - 1], forests[k - 1]);
+      } catch (...) {
+        errors[k - 1] = std::current_exception();
       }
-      // several saddles at one exact level: contract them together
-      bp.clear();
-      auto bfind = [&](int64_t x) {
-        while (bp[x] != x) { bp[x] = bp[bp[x]]; x = bp[x]; }
-        return x;
-      };
-      for (uint32_t e = i; e < j; ++e) {
-        int64_t first = -1;
-        for (uint32_t a = D.arm_base[e]; a < D.arm_base[e + 1]; ++a) {
-          int64_t r = dsu.find(D.carrier[a]);
-          if (!bp.count(r)) bp[r] = r;
-          if (first < 0) first = r;
-          else { int64_t x = bfind(first), y = bfind(r); if (x != y) bp[y] = x; }
-        }
-      }
-      std::unordered_map<int64_t, std::vector<int64_t>> groups;
-      for (auto &kv : bp) groups[bfind(kv.first)].push_back(kv.first);
-      std::vector<std::vector<int64_t>> glist;
-      for (auto &kv : groups) {
-        std::sort(kv.second.begin(), kv.second.end());
-        glist.push_back(kv.second);
-      }
-      std::sort(glist.begin(), glist.end());
-      for (auto &members : glist) {
-        std::vector<int64_t> roots;
-        for (int64_t r : members)
-          if (owner[r] >= 0) roots.push_back(owner[r]);
-        std::sort(roots.begin(), roots.end());
-        roots.erase(std::unique(roots.begin(), roots.end()), roots.end());
-        int64_t node;
-        if (roots.empty()) node = T.add(k, 1, D.cls[i], D.level[i], {});
-        else if (roots.size() == 1) node = roots[0];
-        else node = T.add(k, 2, D.cls[i], D.level[i], roots);
-        for (size_t q = 1; q < members.size(); ++q) dsu.p[dsu.find(members[q])] = dsu.find(members[0]);
-        owner[dsu.find(members[0])] = node;
-      }
-      i = j;
-    }
-    tm.mark("tower forests (host)");
+    });
+  }
+  for (auto &t : workers) t.join();
+  for (auto &e : errors)
+    if (e) std::rethrow_exception(e);
+  tm.mark("tower forests (host)");
+  // concatenate the orders; child ids move by the nodes of the orders below
+  for (int k = 1; k <= K; ++k) {
+    const TowerState &F = forests[k - 1];
+    const int64_t base = (int64_t)T.order.size();
+    const int64_t cbase = (int64_t)T.children.size();
+    T.order.insert(T.order.end(), F.order.begin(), F.order.end());
+    T.kind.insert(T.kind.end(), F.kind.begin(), F.kind.end());
+    T.level_class.insert(T.level_class.end(), F.level_class.begin(), F.level_class.end());
+    T.level.insert(T.level.end(), F.level.begin(), F.level.end());
+    for (size_t i = 1; i < F.child_offset.size(); ++i) T.child_offset.push_back(cbase + F.child_offset[i]);
+    for (int64_t x : F.children) T.children.push_back(base + x);
   }
   T.descent_steps += desc ? desc->steps : 0;
   T.orders = K;

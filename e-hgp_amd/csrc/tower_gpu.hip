@@ -343,12 +343,11 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   hipStream_t s = c.stream;
   const int64_t n = B.count;
   out = TowerOrderData{};
-  Scanner sc;
   DevBuf<uint32_t> flag, minima_pos, saddle_pos;
   flag.ensure(n);
   // minima of order k (k >= 2)
   hipLaunchKernelGGL(k_flag_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.events.p, B.order.p, n, k, flag.p);
-  sc.inclusive(flag.p, n, s);
+  c.scanner.inclusive(flag.p, n, s);
   uint32_t M = 0;
   if (n) copy_wait(&M, flag.p + n - 1, 4, hipMemcpyDeviceToHost, s);
   minima_pos.ensure(M);
@@ -368,7 +367,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   // saddles of order k
   hipLaunchKernelGGL(k_flag_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.events.p, B.order.p, n, k + 1,
                      flag.p);
-  sc.inclusive(flag.p, n, s);
+  c.scanner.inclusive(flag.p, n, s);
   uint32_t S = 0;
   if (n) copy_wait(&S, flag.p + n - 1, 4, hipMemcpyDeviceToHost, s);
   saddle_pos.ensure(S);
@@ -381,7 +380,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   sizes.ensure(S);
   hipLaunchKernelGGL(k_arm_sizes, dim3(blocks_for(S, 256)), dim3(256), 0, s, B.events.p, B.order.p,
                      saddle_pos.p, (int64_t)S, sizes.p);
-  sc.inclusive(sizes.p, S, s);
+  c.scanner.inclusive(sizes.p, S, s);
   uint32_t arms = 0;
   copy_wait(&arms, sizes.p + S - 1, 4, hipMemcpyDeviceToHost, s);
   // exclusive bases: shift the inclusive scan by one
