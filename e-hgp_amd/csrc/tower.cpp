@@ -234,9 +234,9 @@ class Descender {
 };
 
 struct Dsu {
-  std::vector<int64_t> p;
-  explicit Dsu(int64_t n) : p(n) { std::iota(p.begin(), p.end(), 0); }
-  int64_t find(int64_t x) {
+  std::vector<uint32_t> p;
+  explicit Dsu(int64_t n) : p(n) { std::iota(p.begin(), p.end(), 0u); }
+  uint32_t find(uint32_t x) {
     while (p[x] != x) { p[x] = p[p[x]]; x = p[x]; }
     return x;
   }
@@ -249,7 +249,7 @@ struct Dsu {
 void contract_order(int k, int64_t n, TowerOrderData &D, TowerState &F) {
   // carrier DSU; each class may own a reduced root
   const int64_t M = k == 1 ? n : (int64_t)D.minima;
-  std::vector<int64_t> owner(M, -1);
+  std::vector<int32_t> owner(M, -1);  // forest node of a component root (-1: none)
   if (k == 1)
     for (int64_t p = 0; p < n; ++p) owner[p] = F.add(1, 0, -1, 0.0, {});
   Dsu dsu(M);
@@ -258,6 +258,12 @@ void contract_order(int k, int64_t n, TowerOrderData &D, TowerState &F) {
   for (uint32_t i = 0; i < D.saddles;) {
     uint32_t j = i;
     while (j < D.saddles && D.cls[j] == D.cls[i]) j++;
+    // the DSU walk is latency bound: touch the carriers a few saddles ahead
+    if (i + 8 < D.saddles)
+      for (uint32_t a = D.arm_base[i + 8]; a < D.arm_base[i + 9]; ++a) {
+        __builtin_prefetch(&dsu.p[D.carrier[a]]);
+        __builtin_prefetch(&owner[D.carrier[a]]);
+      }
     if (j == i + 1) {
       // single saddle at this level: distinct carrier roots are the group
       int64_t members[4];
@@ -358,7 +364,9 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
         if (t.joinable()) t.join();
     }
   } joiner{workers};
-  for (int k = 1; k <= K; ++k) {
+  // highest order first: its contraction is the longest and overlaps the
+  // device work of all lower orders
+  for (int k = K; k >= 1; --k) {
     TowerOrderData &D = Ds[k - 1];
     tower_order_device(c, k, D);
     tm.mark("tower arms (device)");

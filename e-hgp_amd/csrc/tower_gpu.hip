@@ -112,26 +112,27 @@ struct Ball {
 
 __device__ inline V3 P3(const double *X, const double *Y, const double *Z, uint32_t id) { return {X[id], Y[id], Z[id]}; }
 
-// circumball of 0..4 facet points with centre in their affine hull (FP64)
-__device__ bool ball_from(const V3 *pts, const uint32_t *R, int r, Ball &b) {
+// circumball of 0..4 facet points with centre in their affine hull (FP64);
+// point i of the facet is pts[i * stride]
+__device__ bool ball_from(const V3 *pts, int stride, const uint32_t *R, int r, Ball &b) {
   b.m = r;
   for (int i = 0; i < r; ++i) b.sup[i] = R[i];
   if (r == 0) { b.c = {0, 0, 0}; b.r2 = -1; return true; }
-  const V3 p0 = pts[R[0]];
+  const V3 p0 = pts[R[0] * stride];
   if (r == 1) { b.c = p0; b.r2 = 0; return true; }
-  const V3 a = sub(pts[R[1]], p0);
+  const V3 a = sub(pts[R[1] * stride], p0);
   V3 off;
   if (r == 2) {
     off = {0.5 * a.x, 0.5 * a.y, 0.5 * a.z};
   } else if (r == 3) {
-    const V3 bb = sub(pts[R[2]], p0);
+    const V3 bb = sub(pts[R[2] * stride], p0);
     double aa = dot(a, a), b2 = dot(bb, bb), ab = dot(a, bb);
     double D = aa * b2 - ab * ab;
     if (!(D > 1e-14 * (aa * b2 + ab * ab))) return false;
     double al = b2 * (aa - ab), be = aa * (b2 - ab);
     off = {(al * a.x + be * bb.x) / (2 * D), (al * a.y + be * bb.y) / (2 * D), (al * a.z + be * bb.z) / (2 * D)};
   } else {
-    const V3 bb = sub(pts[R[2]], p0), d = sub(pts[R[3]], p0);
+    const V3 bb = sub(pts[R[2] * stride], p0), d = sub(pts[R[3] * stride], p0);
     V3 bxd = cross(bb, d), dxa = cross(d, a), axb = cross(a, bb);
     double V = dot(a, bxd);
     if (!(fabs(V) > 1e-14 * adot(vabs(a), across(bb, d)))) return false;
@@ -154,26 +155,26 @@ __device__ inline bool in_ball(const Ball &b, const V3 &p) {
 // is the recursion level with r boundary points (at most 4 in 3-D), visited in
 // the same order as the recursive algorithm.  No recursion, so the kernel's
 // private stack has a static size.
-__device__ bool welzl(const V3 *pts, int n, Ball &out) {
+__device__ bool welzl(const V3 *pts, int stride, int n, Ball &out) {
   uint32_t R[4];
   Ball b;
-  ball_from(pts, R, 0, b);
+  ball_from(pts, stride, R, 0, b);
   for (int i = 0; i < n; ++i) {
-    if (in_ball(b, pts[i])) continue;
+    if (in_ball(b, pts[i * stride])) continue;
     R[0] = (uint32_t)i;
-    if (!ball_from(pts, R, 1, b)) return false;
+    if (!ball_from(pts, stride, R, 1, b)) return false;
     for (int j = 0; j < i; ++j) {
-      if (in_ball(b, pts[j])) continue;
+      if (in_ball(b, pts[j * stride])) continue;
       R[1] = (uint32_t)j;
-      if (!ball_from(pts, R, 2, b)) return false;
+      if (!ball_from(pts, stride, R, 2, b)) return false;
       for (int l = 0; l < j; ++l) {
-        if (in_ball(b, pts[l])) continue;
+        if (in_ball(b, pts[l * stride])) continue;
         R[2] = (uint32_t)l;
-        if (!ball_from(pts, R, 3, b)) return false;
+        if (!ball_from(pts, stride, R, 3, b)) return false;
         for (int q = 0; q < l; ++q) {
-          if (in_ball(b, pts[q])) continue;
+          if (in_ball(b, pts[q * stride])) continue;
           R[3] = (uint32_t)q;
-          if (!ball_from(pts, R, 4, b)) return false;
+          if (!ball_from(pts, stride, R, 4, b)) return false;
         }
       }
     }
@@ -182,43 +183,47 @@ __device__ bool welzl(const V3 *pts, int n, Ball &out) {
   return true;
 }
 
-// certified side of point q against the support sphere (ids relative to sup[0])
-__device__ int cert_side(const double *X, const double *Y, const double *Z, const uint32_t *sid, int m,
-                         uint32_t q) {
-  for (int i = 0; i < m; ++i)
-    if (sid[i] == q) return 0;
-  if (m == 1) return 1;
-  const V3 o = P3(X, Y, Z, sid[0]);
-  const V3 p = sub(P3(X, Y, Z, q), o);
-  if (m == 2) { PairSphere s; s.init(sub(P3(X, Y, Z, sid[1]), o)); return s.side(p); }
-  if (m == 3) {
-    TriSphere s;
-    s.init(sub(P3(X, Y, Z, sid[1]), o), sub(P3(X, Y, Z, sid[2]), o));
-    return s.side(p);
+// Sphere of a miniball support (1..4 canonical ids), built once per descent
+// step; side() is the certified position of a point (0: on a support point
+// or undecided, which sends the arm to the host).
+struct SupSphere {
+  int m;
+  uint32_t sid[4];
+  V3 o;
+  union {
+    PairSphere ps;
+    TriSphere ts;
+    TetSphere tt;
+  };
+  // certified well-centring: +1 yes, -1 no, 0 undecided
+  __device__ int init(const double *X, const double *Y, const double *Z, const uint32_t *ids, int mm) {
+    m = mm;
+    for (int i = 0; i < 4; ++i) sid[i] = i < m ? ids[i] : kInvalid;
+    o = P3(X, Y, Z, sid[0]);
+    if (m == 2) { ps.init(sub(P3(X, Y, Z, sid[1]), o)); return 1; }
+    if (m == 3) return ts.init(sub(P3(X, Y, Z, sid[1]), o), sub(P3(X, Y, Z, sid[2]), o));
+    if (m == 4)
+      return tt.init(sub(P3(X, Y, Z, sid[1]), o), sub(P3(X, Y, Z, sid[2]), o), sub(P3(X, Y, Z, sid[3]), o));
+    return 1;
   }
-  TetSphere s;
-  s.init(sub(P3(X, Y, Z, sid[1]), o), sub(P3(X, Y, Z, sid[2]), o), sub(P3(X, Y, Z, sid[3]), o));
-  return s.side(p);
-}
-
-// certified well-centring of a support (+1 yes, -1 no, 0 undecided)
-__device__ int cert_well(const double *X, const double *Y, const double *Z, const uint32_t *sid, int m) {
-  if (m <= 2) return 1;
-  const V3 o = P3(X, Y, Z, sid[0]);
-  if (m == 3) {
-    TriSphere s;
-    return s.init(sub(P3(X, Y, Z, sid[1]), o), sub(P3(X, Y, Z, sid[2]), o));
+  __device__ int side(const double *X, const double *Y, const double *Z, uint32_t q) const {
+    for (int i = 0; i < m; ++i)
+      if (sid[i] == q) return 0;
+    if (m == 1) return 1;
+    const V3 p = sub(P3(X, Y, Z, q), o);
+    if (m == 2) return ps.side(p);
+    if (m == 3) return ts.side(p);
+    return tt.side(p);
   }
-  TetSphere s;
-  return s.init(sub(P3(X, Y, Z, sid[1]), o), sub(P3(X, Y, Z, sid[2]), o), sub(P3(X, Y, Z, sid[3]), o));
-}
+};
 
 struct ArmParams {
   const EventRec *ev;
   const uint32_t *order;
   const uint32_t *saddle_pos;  // sorted positions of the saddles of this order
   const uint32_t *arm_base;    // first arm of each saddle (exclusive scan of sizes)
-  int64_t saddles;
+  const uint32_t *arm_saddle;  // saddle of each arm
+  int64_t arms;
   int k;
   const uint64_t *table_keys;
   const uint32_t *table_vals;
@@ -233,90 +238,106 @@ struct ArmParams {
   unsigned long long *steps;
 };
 
-__global__ void __launch_bounds__(64) k_resolve_arms(ArmParams A) {
-  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= A.saddles) return;
+constexpr int kArmBlock = 64;
+constexpr int kArmStack = 40;  // LBVH depth is 18 at 50k points, 32 at 200k; overflow goes to the host
+
+// One thread per strict arm: hash lookup of the facet, else certified
+// descents until a minimum's label is found.
+__global__ void __launch_bounds__(kArmBlock) k_resolve_arms(ArmParams A) {
+  // traversal stacks, interleaved by lane so equal depths hit distinct banks
+  __shared__ uint32_t stk[kArmStack * kArmBlock];
+  const int lane = threadIdx.x;
+  const int64_t arm = (int64_t)blockIdx.x * kArmBlock + lane;
+  if (arm >= A.arms) return;
+  const uint32_t s = A.arm_saddle[arm];
+  const int j = (int)(arm - A.arm_base[s]);
   const EventRec &e = A.ev[A.order[A.saddle_pos[s]]];
   uint32_t L[kMaxLabel];
   const int len = event_label(e, L);
-  const int m = e.meta & 0xff;
   const int k = A.k;
+  uint32_t F[kMaxLabel];
+  int f = 0;
+  for (int i = 0; i < len; ++i)
+    if (L[i] != e.ids[j]) F[f++] = L[i];
+  uint32_t result = kInvalid;
   uint64_t local_steps = 0;
-  for (int a = 0; a < m; ++a) {
-    uint32_t F[kMaxLabel];
-    int f = 0;
-    for (int i = 0; i < len; ++i)
-      if (L[i] != e.ids[a]) F[f++] = L[i];
-    uint32_t result = kInvalid;
-    if (k == 1) result = F[0];  // order one: the arm is a point, its own minimum
-    for (int step = 0; step < 4096 && result == kInvalid; ++step) {
-      int64_t hit = lookup(F, k, A.table_keys, A.table_vals, A.mask, A.labels);
-      if (hit >= 0) { result = (uint32_t)hit; break; }
-      // miniball of the facet
-      V3 pts[kMaxLabel];
-      for (int i = 0; i < k; ++i) pts[i] = P3(A.X, A.Y, A.Z, F[i]);
-      Ball b;
-      if (!welzl(pts, k, b)) break;
-      uint32_t sid[4];
-      for (int i = 0; i < b.m; ++i) sid[i] = F[b.sup[i]];
-      if (cert_well(A.X, A.Y, A.Z, sid, b.m) <= 0) break;
-      // every facet point must be certainly inside, or a support point
-      bool ok = true;
-      for (int i = 0; i < k && ok; ++i) {
-        bool on_support = false;
-        for (int q = 0; q < b.m; ++q) on_support |= sid[q] == F[i];
-        if (!on_support) ok = cert_side(A.X, A.Y, A.Z, sid, b.m, F[i]) < 0;
-      }
-      if (!ok) break;
-      // closed-ball points outside the facet
-      uint32_t xin = kInvalid;
-      bool undecided = false, shell = false;
-      const double q[3] = {b.c.x, b.c.y, b.c.z};
-      const double R2 = b.r2 * (1.0 + 1e-9) + 1e-300;
-      uint32_t stack[64];
-      int sp = 0;
-      stack[sp++] = 0;
-      while (sp && !undecided) {
-        const Node &N = A.nodes[stack[--sp]];
-        uint32_t ch[2] = {N.left, N.right};
-        for (int c = 0; c < 2; ++c) {
-          uint32_t x = ch[c];
-          if (x & kLeafBit) {
-            uint32_t lp = x & ~kLeafBit;
-            double dx = A.LX[lp] - q[0], dy = A.LY[lp] - q[1], dz = A.LZ[lp] - q[2];
-            if (dx * dx + dy * dy + dz * dz > R2) continue;
-            uint32_t id = A.leaf_id[lp];
-            bool inF = false;
-            for (int i = 0; i < k; ++i) inF |= F[i] == id;
-            if (inF) continue;
-            int sd = cert_side(A.X, A.Y, A.Z, sid, b.m, id);
-            if (sd < 0) xin = min(xin, id);
-            else if (sd == 0) undecided = true;
-          } else if (box_d2(q, A.nodes[x].lo, A.nodes[x].hi) <= R2) {
-            if (sp < 64) stack[sp++] = x; else undecided = true;
-          }
+  if (k == 1) result = F[0];  // order one: the arm is a point, its own minimum
+  for (int step = 0; step < 4096 && result == kInvalid; ++step) {
+    const int64_t hit = lookup(F, k, A.table_keys, A.table_vals, A.mask, A.labels);
+    if (hit >= 0) { result = (uint32_t)hit; break; }
+    // miniball of the facet
+    V3 pts[kMaxLabel];
+    for (int i = 0; i < k; ++i) pts[i] = P3(A.X, A.Y, A.Z, F[i]);
+    Ball b;
+    if (!welzl(pts, 1, k, b)) break;
+    uint32_t sid[4];
+    for (int i = 0; i < b.m; ++i) sid[i] = F[b.sup[i]];
+    SupSphere S;
+    if (S.init(A.X, A.Y, A.Z, sid, b.m) <= 0) break;
+    // every facet point must be certainly inside, or a support point
+    bool ok = true;
+    for (int i = 0; i < k && ok; ++i) {
+      bool on_support = false;
+      for (int q = 0; q < b.m; ++q) on_support |= sid[q] == F[i];
+      if (!on_support) ok = S.side(A.X, A.Y, A.Z, F[i]) < 0;
+    }
+    if (!ok) break;
+    // closed-ball points outside the facet
+    uint32_t xin = kInvalid;
+    bool undecided = false;
+    const double q[3] = {b.c.x, b.c.y, b.c.z};
+    const double R2 = b.r2 * (1.0 + 1e-9) + 1e-300;
+    int sp = 0;
+    stk[lane] = 0;
+    sp = 1;
+    while (sp && !undecided) {
+      const Node &N = A.nodes[stk[(--sp) * kArmBlock + lane]];
+      const uint32_t ch[2] = {N.left, N.right};
+      for (int c = 0; c < 2; ++c) {
+        const uint32_t x = ch[c];
+        if (x & kLeafBit) {
+          const uint32_t lp = x & ~kLeafBit;
+          const double dx = A.LX[lp] - q[0], dy = A.LY[lp] - q[1], dz = A.LZ[lp] - q[2];
+          if (dx * dx + dy * dy + dz * dz > R2) continue;
+          const uint32_t id = A.leaf_id[lp];
+          bool inF = false;
+          for (int i = 0; i < k; ++i) inF |= F[i] == id;
+          if (inF) continue;
+          const int sd = S.side(A.X, A.Y, A.Z, id);
+          if (sd < 0) xin = min(xin, id);
+          else if (sd == 0) undecided = true;
+        } else if (box_d2(q, A.nodes[x].lo, A.nodes[x].hi) <= R2) {
+          if (sp < kArmStack) stk[(sp++) * kArmBlock + lane] = x;
+          else undecided = true;
         }
       }
-      (void)shell;
-      if (undecided || xin == kInvalid) break;  // host decides (shell points or a missing minimum)
-      uint32_t y = sid[0];
-      for (int i = 1; i < b.m; ++i) y = min(y, sid[i]);
-      // F := F - y + xin, kept sorted
-      int w = 0;
-      uint32_t G[kMaxLabel];
-      bool placed = false;
-      for (int i = 0; i < k; ++i) {
-        if (F[i] == y) continue;
-        if (!placed && xin < F[i]) { G[w++] = xin; placed = true; }
-        G[w++] = F[i];
-      }
-      if (!placed) G[w++] = xin;
-      for (int i = 0; i < k; ++i) F[i] = G[i];
-      local_steps++;
     }
-    A.carrier[A.arm_base[s] + a] = result;
+    if (undecided || xin == kInvalid) break;  // host decides (shell points or a missing minimum)
+    uint32_t y = sid[0];
+    for (int i = 1; i < b.m; ++i) y = min(y, sid[i]);
+    // F := F - y + xin, kept sorted
+    int w = 0;
+    uint32_t G[kMaxLabel];
+    bool placed = false;
+    for (int i = 0; i < k; ++i) {
+      if (F[i] == y) continue;
+      if (!placed && xin < F[i]) { G[w++] = xin; placed = true; }
+      G[w++] = F[i];
+    }
+    if (!placed) G[w++] = xin;
+    for (int i = 0; i < k; ++i) F[i] = G[i];
+    local_steps++;
   }
+  A.carrier[arm] = result;
   if (local_steps) atomicAdd(A.steps, (unsigned long long)local_steps);
+}
+
+__global__ void k_arm_owner(const uint32_t *__restrict__ base, const uint32_t *__restrict__ sizes_incl,
+                            int64_t count, uint32_t *arm_saddle) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t b = base[i], e = sizes_incl[i];
+  for (uint32_t a = b; a < e; ++a) arm_saddle[a] = (uint32_t)i;
 }
 
 __global__ void k_arm_sizes(const EventRec *__restrict__ ev, const uint32_t *__restrict__ order,
@@ -389,18 +410,22 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   MHGP_HIP(hipMemsetAsync(base.p, 0, 4, s));
   if (S > 1) MHGP_HIP(hipMemcpyAsync(base.p + 1, sizes.p, (S - 1) * 4, hipMemcpyDeviceToDevice, s));
   carrier.ensure(arms);
+  DevBuf<uint32_t> arm_saddle;
+  arm_saddle.ensure(arms);
+  hipLaunchKernelGGL(k_arm_owner, dim3(blocks_for(S, 256)), dim3(256), 0, s, base.p, sizes.p, (int64_t)S,
+                     arm_saddle.p);
   steps.ensure(1);
   MHGP_HIP(hipMemsetAsync(steps.p, 0, 8, s));
   ArmParams A{};
   A.ev = B.events.p; A.order = B.order.p; A.saddle_pos = saddle_pos.p; A.arm_base = base.p;
-  A.saddles = S; A.k = k;
+  A.arm_saddle = arm_saddle.p; A.arms = arms; A.k = k;
   A.table_keys = tkeys.p; A.table_vals = tvals.p; A.mask = cap - 1; A.labels = labels.p;
   A.X = c.X.p; A.Y = c.Y.p; A.Z = c.Z.p;
   A.nodes = c.nodes.p; A.LX = c.LX.p; A.LY = c.LY.p; A.LZ = c.LZ.p; A.leaf_id = c.leaf_id.p; A.n = c.n;
   A.carrier = carrier.p;
   A.steps = (unsigned long long *)steps.p;
   c.arm_clock.start(s);
-  hipLaunchKernelGGL(k_resolve_arms, dim3(blocks_for(S, 64)), dim3(64), 0, s, A);
+  hipLaunchKernelGGL(k_resolve_arms, dim3(blocks_for(arms, kArmBlock)), dim3(kArmBlock), 0, s, A);
   c.arm_clock.stop(s);
   MHGP_LAUNCH_CHECK();
   c.stats.arm_ms += c.arm_clock.ms();
