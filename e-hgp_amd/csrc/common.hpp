@@ -53,6 +53,29 @@ struct DevBuf {
   size_t bytes() const { return cap * sizeof(T); }
 };
 
+// Page-locked host buffer (hipHostMalloc), grown on demand and reused, for
+// large device-to-host results.
+struct PinnedBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf &) = delete;
+  PinnedBuf &operator=(const PinnedBuf &) = delete;
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void *ensure(size_t bytes) {
+    if (bytes > cap) {
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      cap = 0;
+      MHGP_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+      cap = bytes;
+    }
+    return p;
+  }
+};
+
 // Internal LBVH node: FP64 box (exact, coordinates are binary64), children,
 // leaf range.  64 bytes, one cache line per node.  A child index with the high
 // bit set denotes a leaf position in Morton order.

@@ -71,6 +71,7 @@ struct Context {
   std::unique_ptr<EmstBuffers> emst;
   std::unique_ptr<HostMirror> host;
   std::unique_ptr<TowerState> tower;
+  PinnedBuf tower_pinned[11];  // per tower order: saddle results on the host
 
   Context();
   ~Context();

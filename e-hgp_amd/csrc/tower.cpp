@@ -373,7 +373,7 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
     T.descent_steps += D.steps;
     // carriers the device could not settle: exact host descents
     bool any_host = false;
-    for (uint32_t x : D.carrier) any_host |= x == kInvalid;
+    for (uint32_t a = 0; a < D.arms; ++a) any_host |= D.carrier[a] == kInvalid;
     if (any_host) {
       if (!desc) {
         xyz.resize(3 * n);

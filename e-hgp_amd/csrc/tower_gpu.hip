@@ -348,13 +348,28 @@ __global__ void k_arm_sizes(const EventRec *__restrict__ ev, const uint32_t *__r
 
 __global__ void k_saddle_meta(const EventRec *__restrict__ ev, const uint32_t *__restrict__ order,
                               const uint32_t *__restrict__ cls, const uint32_t *__restrict__ pos,
-                              int64_t count, uint32_t *out_cls, double *out_level, uint32_t *out_sup) {
+                              int64_t count, uint32_t *out_cls, double *out_level) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   const EventRec &e = ev[order[pos[i]]];
   out_cls[i] = cls[pos[i]];
   out_level[i] = e.level;
-  for (int q = 0; q < 4; ++q) out_sup[4 * i + q] = e.ids[q];
+}
+
+// Closed labels (support then interior, kInvalid padded to 11) and supports
+// of the saddles, for the host's exact descents.
+__global__ void k_saddle_labels(const EventRec *__restrict__ ev, const uint32_t *__restrict__ order,
+                                const uint32_t *__restrict__ pos, int64_t count, uint32_t *labels,
+                                uint32_t *sup) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const EventRec &e = ev[order[pos[i]]];
+  const int m = e.meta & 0xff, ni = (e.meta >> 16) & 0xff;
+  int t = 0;
+  for (int q = 0; q < m; ++q) labels[(size_t)i * kMaxLabel + t++] = e.ids[q];
+  for (int q = 0; q < ni; ++q) labels[(size_t)i * kMaxLabel + t++] = e.interior[q];
+  while (t < kMaxLabel) labels[(size_t)i * kMaxLabel + t++] = kInvalid;
+  for (int q = 0; q < 4; ++q) sup[4 * i + q] = e.ids[q];
 }
 
 }  // namespace
@@ -395,7 +410,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   hipLaunchKernelGGL(k_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.order.p, flag.p, n, saddle_pos.p);
   out.saddles = S;
   if (S == 0) return;
-  DevBuf<uint32_t> sizes, carrier, scls, ssup;
+  DevBuf<uint32_t> sizes, carrier, scls;
   DevBuf<double> slev;
   DevBuf<uint64_t> steps;
   sizes.ensure(S);
@@ -435,41 +450,44 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   c.stats.arm_bytes += (int64_t)S * ((int64_t)sizeof(EventRec) + 8) + (int64_t)arms * (4 + 12);
   scls.ensure(S);
   slev.ensure(S);
-  ssup.ensure(4 * (size_t)S);
   hipLaunchKernelGGL(k_saddle_meta, dim3(blocks_for(S, 256)), dim3(256), 0, s, B.events.p, B.order.p, B.cls.p,
-                     saddle_pos.p, (int64_t)S, scls.p, slev.p, ssup.p);
-  out.cls.resize(S);
-  out.level.resize(S);
-  out.sup.resize(4 * (size_t)S);
-  out.arm_base.resize(S + 1);
-  out.carrier.resize(arms);
-  copy_wait(out.cls.data(), scls.p, S * 4, hipMemcpyDeviceToHost, s);
-  copy_wait(out.level.data(), slev.p, S * 8, hipMemcpyDeviceToHost, s);
-  copy_wait(out.sup.data(), ssup.p, 4 * (size_t)S * 4, hipMemcpyDeviceToHost, s);
-  copy_wait(out.arm_base.data(), base.p, S * 4, hipMemcpyDeviceToHost, s);
-  out.arm_base[S] = arms;
-  copy_wait(out.carrier.data(), carrier.p, (size_t)arms * 4, hipMemcpyDeviceToHost, s);
+                     saddle_pos.p, (int64_t)S, scls.p, slev.p);
+  // page-locked results: cls | arm_base | carrier | level
+  const size_t o_base = (size_t)S * 4, o_car = o_base + ((size_t)S + 1) * 4;
+  const size_t o_lev = (o_car + (size_t)arms * 4 + 7) & ~(size_t)7;
+  char *h = (char *)c.tower_pinned[k].ensure(o_lev + (size_t)S * 8);
+  uint32_t *h_cls = (uint32_t *)h, *h_base = (uint32_t *)(h + o_base), *h_car = (uint32_t *)(h + o_car);
+  double *h_lev = (double *)(h + o_lev);
+  MHGP_HIP(hipMemcpyAsync(h_cls, scls.p, (size_t)S * 4, hipMemcpyDeviceToHost, s));
+  MHGP_HIP(hipMemcpyAsync(h_base, base.p, (size_t)S * 4, hipMemcpyDeviceToHost, s));
+  MHGP_HIP(hipMemcpyAsync(h_car, carrier.p, (size_t)arms * 4, hipMemcpyDeviceToHost, s));
+  MHGP_HIP(hipMemcpyAsync(h_lev, slev.p, (size_t)S * 8, hipMemcpyDeviceToHost, s));
   uint64_t st = 0;
-  copy_wait(&st, steps.p, 8, hipMemcpyDeviceToHost, s);
+  MHGP_HIP(hipMemcpyAsync(&st, steps.p, 8, hipMemcpyDeviceToHost, s));
+  MHGP_HIP(hipStreamSynchronize(s));
+  h_base[S] = arms;
+  out.cls = h_cls;
+  out.arm_base = h_base;
+  out.carrier = h_car;
+  out.level = h_lev;
+  out.arms = arms;
   out.steps = (int64_t)st;
   bool need_labels = false;
-  for (uint32_t x : out.carrier) need_labels |= x == kInvalid;
+  for (uint32_t a = 0; a < arms; ++a) need_labels |= h_car[a] == kInvalid;
   if (need_labels) {
-    // the host descends unresolved arms; it needs the saddle labels and the minima labels
+    // the host descends unresolved arms; it needs the saddle labels, their
+    // supports and the minima labels
     out.minima_labels.resize((size_t)M * k);
     if (M) copy_wait(out.minima_labels.data(), labels.p, (size_t)M * k * 4, hipMemcpyDeviceToHost, s);
-    out.saddle_labels.resize((size_t)S * kMaxLabel, kInvalid);
-    std::vector<uint32_t> pos(S), ord(n);
-    copy_wait(pos.data(), saddle_pos.p, S * 4, hipMemcpyDeviceToHost, s);
-    for (uint32_t i = 0; i < S; ++i) {
-      uint32_t idx;
-      EventRec e;
-      copy_wait(&idx, B.order.p + pos[i], 4, hipMemcpyDeviceToHost, s);
-      copy_wait(&e, B.events.p + idx, sizeof(EventRec), hipMemcpyDeviceToHost, s);
-      int m = e.meta & 0xff, ni = (e.meta >> 16) & 0xff, t = 0;
-      for (int q = 0; q < m; ++q) out.saddle_labels[(size_t)i * kMaxLabel + t++] = e.ids[q];
-      for (int q = 0; q < ni; ++q) out.saddle_labels[(size_t)i * kMaxLabel + t++] = e.interior[q];
-    }
+    DevBuf<uint32_t> slab, ssup;
+    slab.ensure((size_t)S * kMaxLabel);
+    ssup.ensure(4 * (size_t)S);
+    hipLaunchKernelGGL(k_saddle_labels, dim3(blocks_for(S, 256)), dim3(256), 0, s, B.events.p, B.order.p,
+                       saddle_pos.p, (int64_t)S, slab.p, ssup.p);
+    out.saddle_labels.resize((size_t)S * kMaxLabel);
+    out.sup.resize(4 * (size_t)S);
+    MHGP_HIP(hipMemcpyAsync(out.saddle_labels.data(), slab.p, (size_t)S * kMaxLabel * 4, hipMemcpyDeviceToHost, s));
+    copy_wait(out.sup.data(), ssup.p, 4 * (size_t)S * 4, hipMemcpyDeviceToHost, s);
   }
 }
 
