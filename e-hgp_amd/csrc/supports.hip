@@ -199,12 +199,161 @@ __device__ void process_tet(const Params &P, const Work &w, uint32_t u, uint32_t
   emit_event(P, u, ids, 4, nint, interior, t.level(), t.level_err());
 }
 
-// Face (u, v, w): its triangle event and the apexes swept first by the
-// pencil of spheres through its circumcircle, on each side.
+// Smallest sweep keys of the outside points ahead of one side: tau with its
+// error bound and the region slot.  Insertion runs over static indices, so
+// the list stays in registers.
+struct SweepList {
+  static constexpr int kCap = 8;  // >= s_max - 3 for s_max <= 11
+  double hi[kCap];                // upper key bounds, ascending
+  float err[kCap];                // key error bounds (key = hi - err)
+  uint32_t slot[kCap];
+  double evicted_lo;  // smallest lower key bound of a point not in the list
+  __device__ void init() {
+#pragma unroll
+    for (int j = 0; j < kCap; ++j) { hi[j] = INFINITY; err[j] = 0.f; slot[j] = kInvalid; }
+    evicted_lo = INFINITY;
+  }
+  __device__ void insert(double t, double e, uint32_t s) {
+    const double h = t + e;
+    if (!(h < hi[kCap - 1])) {
+      evicted_lo = fmin(evicted_lo, t - e);
+      return;
+    }
+    evicted_lo = fmin(evicted_lo, hi[kCap - 1] - 2.0 * (double)err[kCap - 1]);
+    const float ef = (float)e * (1.0f + 1e-6f);
+#pragma unroll
+    for (int j = kCap - 1; j > 0; --j) {
+      const bool shift = h < hi[j - 1];
+      const bool place = !shift && h < hi[j];
+      hi[j] = shift ? hi[j - 1] : (place ? h : hi[j]);
+      err[j] = shift ? err[j - 1] : (place ? ef : err[j]);
+      slot[j] = shift ? slot[j - 1] : (place ? s : slot[j]);
+    }
+    if (h < hi[0]) { hi[0] = h; err[0] = ef; slot[0] = s; }
+  }
+  // upper key bound of the need-th entry (need in 1..kCap)
+  __device__ double threshold(int need) const {
+    double r = INFINITY;
+#pragma unroll
+    for (int j = 0; j < kCap; ++j) r = (j == need - 1) ? hi[j] : r;
+    return r;
+  }
+};
+
+// Largest lower key bounds of the inside points behind one side (a subset
+// of them only weakens the apex pruning bound, never invalidates it).
+struct BehindList {
+  static constexpr int kCap = 4;
+  double v[kCap];
+  __device__ void init() {
+#pragma unroll
+    for (int j = 0; j < kCap; ++j) v[j] = -INFINITY;
+  }
+  __device__ void insert(double x) {
+#pragma unroll
+    for (int j = kCap - 1; j > 0; --j) v[j] = x > v[j - 1] ? v[j - 1] : (x > v[j] ? x : v[j]);
+    v[0] = fmax(v[0], x);
+  }
+  __device__ int count_above(double x) const {
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < kCap; ++j) c += v[j] > x;
+    return c;
+  }
+};
+
+// Sweep key of a point against the face (tau = P / |H|) and its error bound.
+__device__ inline void sweep_key(double Pw, double PA, double H, double HA, double &tau, double &terr) {
+  const double rH = 1.0 / fabs(H);
+  tau = Pw * rH;
+  terr = (kCert * PA + kCert * HA * fabs(tau)) * rH * (1.0 + 1e-12) + 1e-15 * fabs(tau);
+}
+
+// Apexes of one side of a face: the outside points ahead of the side whose
+// sweep key is among the s_max-3 smallest (fewer when inside points are
+// already ahead), each checked against the inside points behind the side.
+// One pass over the region fills the side's sweep list; the candidates are
+// packed into a mask before any tetrahedron is classified.
+__device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uint32_t M, const TriSphere &f,
+                                 uint32_t vi, uint32_t wi, uint32_t wid, int side, int cin_side, int K,
+                                 uint32_t *region_slot) {
+  const int need = K - cin_side;  // slots left for points swept before z
+  if (need <= 0) return;
+  SweepList top;
+  BehindList beh;
+  top.init();
+  beh.init();
+  bool undecided = false;
+#pragma unroll 1
+  for (uint32_t s = 0; s < M; ++s) {
+    const uint32_t gi = w.ridx[s];
+    if (gi == vi || gi == wi) continue;
+    const V3 p = gpt(w, gi);
+    double Pw, PA, H, HA;
+    f.power(p, Pw, PA);
+    f.height(p, H, HA);
+    const int sp = cert_sign(Pw, PA), sh = cert_sign(H, HA);
+    undecided |= sp == 0 || sh == 0;
+    const bool ahead = side == 0 ? sh > 0 : sh < 0;
+    double tau, terr;
+    sweep_key(Pw, PA, H, HA, tau, terr);
+    if (sp < 0) {
+      // inside the triangle's sphere, behind this side: inside the pencil's
+      // spheres until the sweep key reaches -tau
+      if (!ahead) beh.insert(-tau - terr);
+    } else if (ahead) {
+      top.insert(tau, terr, s);
+    }
+  }
+  // the face pass certified every sign with the same arithmetic
+  if (undecided) return;
+  const double thr = top.threshold(need);
+  if (top.evicted_lo <= thr) {
+    // a point outside the list may still qualify (near-equal keys): full scan
+#pragma unroll 1
+    for (uint32_t s = 0; s < M; ++s) {
+      const uint32_t zi = w.ridx[s];
+      if (zi == vi || zi == wi || w.gid[zi] <= wid) continue;
+      const V3 p = gpt(w, zi);
+      double Pw, PA, H, HA;
+      f.power(p, Pw, PA);
+      f.height(p, H, HA);
+      if (cert_sign(Pw, PA) <= 0) continue;
+      const int sh = cert_sign(H, HA);
+      if ((side == 0 && sh <= 0) || (side == 1 && sh >= 0)) continue;
+      double tau, terr;
+      sweep_key(Pw, PA, H, HA, tau, terr);
+      if (tau - terr > thr) continue;
+      if (cin_side + beh.count_above(tau + terr) >= K) continue;
+      process_tet(P, w, u, M, f, vi, wi, zi, region_slot);
+    }
+    return;
+  }
+  uint32_t cand = 0, pack[3] = {0u, 0u, 0u};
+#pragma unroll
+  for (int j = 0; j < SweepList::kCap; ++j) {
+    const double terr = (double)top.err[j], h = top.hi[j];
+    // inside points behind: counted while their key exceeds z's
+    const bool ok = top.slot[j] != kInvalid && h - 2.0 * terr <= thr && cin_side + beh.count_above(h) < K;
+    cand |= ok ? (1u << j) : 0u;
+    pack[j / 3] |= (ok ? top.slot[j] : 0u) << (10 * (j % 3));
+  }
+  while (cand) {
+    const int j = __ffs(cand) - 1;
+    cand &= cand - 1;
+    const uint32_t word = j < 3 ? pack[0] : (j < 6 ? pack[1] : pack[2]);
+    const uint32_t zi = w.ridx[(word >> (10 * (j % 3))) & 1023u];
+    if (w.gid[zi] <= wid) continue;
+    process_tet(P, w, u, M, f, vi, wi, zi, region_slot);
+  }
+}
+
+// Face (u, v, w): its triangle event, then the apexes of each side.  The
+// first pass certifies every side and insideness, builds the triangle's
+// interior and counts the inside points ahead of each side; each side then
+// takes one pass of its own.
 __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_t M, uint32_t vi,
                              uint32_t wi, uint32_t *region_slot) {
-  const uint64_t *mv = w.rmask;  // masks indexed by region slot, looked up by caller
-  (void)mv;
   // the sphere is built from its points in PointId order, so its FP64 level
   // (and every later tetrahedron's) does not depend on the enumeration order
   uint32_t ids[2] = {w.gid[vi], w.gid[wi]};
@@ -219,62 +368,34 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
   }
   const uint32_t wid = ids[1];
   const int smax = P.s_max;
-  // pass 1: sides and insideness of every region point, triangle interior,
-  // the (s_max-3) smallest sweep keys of outside points on each side, and
-  // the keys of inside points behind each side.
-  int cin[2] = {0, 0};        // inside S3, ahead of side s (H > 0 for s = 0)
+  const int K = smax - 3;  // an apex must be among the first K points swept
+  int cin0 = 0, cin1 = 0;  // inside the triangle's sphere, ahead of side 0 / 1
   int total_in = 0;
   uint32_t tri_int[kMaxInterior];
-  bool undecided = false;
-  const int K = smax - 3;     // apex must be among the first K sweeps
-  double top[2][8];           // smallest upper bounds of tau for outside points ahead
-  int ntop[2] = {0, 0};
-  double behind[2][16];       // lower bounds of tau for inside points behind side s
-  int nbehind[2] = {0, 0};
-  bool behind_over[2] = {false, false};
-  for (uint32_t s = 0; s < M && !undecided; ++s) {
-    uint32_t gi = w.ridx[s];
+#pragma unroll 1
+  for (uint32_t s = 0; s < M; ++s) {
+    const uint32_t gi = w.ridx[s];
     if (gi == vi || gi == wi) continue;
-    V3 p = gpt(w, gi);
+    const V3 p = gpt(w, gi);
     double Pw, PA, H, HA;
     f.power(p, Pw, PA);
     f.height(p, H, HA);
-    int sp = cert_sign(Pw, PA), sh = cert_sign(H, HA);
-    if (sp == 0 || sh == 0) { undecided = true; break; }
-    if (sp < 0) {
-      if (total_in < kMaxInterior) tri_int[total_in] = w.gid[gi];
-      total_in++;
+    const int sp = cert_sign(Pw, PA), sh = cert_sign(H, HA);
+    // every sign the side passes use is certified here, before anything is emitted
+    if (sp == 0 || sh == 0) {
+      emit_fallback(P, w, u, 3, ids, 3, M, region_slot);
+      return;
     }
-    int side = sh > 0 ? 0 : 1;  // side the point is ahead of
-    double tau = Pw / fabs(H);  // sweep key towards its own side
-    double terr = (kCert * PA / fabs(Pw) + kCert * HA / fabs(H) + 1e-15) * fabs(tau);
-    if (sp < 0) {
-      cin[side]++;
-      // both sides saturated and the triangle out of rank: nothing left here
-      if (cin[0] >= K && cin[1] >= K && total_in + 3 > smax) return;
-      // behind the opposite side: inside until the sweep key reaches -tau
-      int o = 1 - side;
-      if (nbehind[o] < 16) behind[o][nbehind[o]++] = -tau - terr;
-      else behind_over[o] = true;
-    } else {
-      double hi = tau + terr;
-      if (ntop[side] < K) {
-        int j = ntop[side]++;
-        while (j > 0 && top[side][j - 1] > hi) { top[side][j] = top[side][j - 1]; j--; }
-        top[side][j] = hi;
-      } else if (K > 0 && hi < top[side][K - 1]) {
-        int j = K - 1;
-        while (j > 0 && top[side][j - 1] > hi) { top[side][j] = top[side][j - 1]; j--; }
-        top[side][j] = hi;
-      }
-    }
-  }
-  if (undecided) {
-    emit_fallback(P, w, u, 3, ids, 3, M, region_slot);
-    return;
+    if (sp > 0) continue;
+    if (total_in < kMaxInterior) tri_int[total_in] = w.gid[gi];
+    total_in++;
+    cin0 += sh > 0;
+    cin1 += sh < 0;
+    // both sides saturated and the triangle out of rank: nothing left here
+    if (cin0 >= K && cin1 >= K && total_in + 3 > smax) return;
   }
   // triangle event
-  if (wc != 0 && wc > 0 && total_in + 3 <= smax) {
+  if (wc > 0 && total_in + 3 <= smax) {
     atomicAdd(&P.counters[3], 1ull);
     sort_small(tri_int, total_in);
     emit_event(P, u, ids, 3, total_in, tri_int, f.level(), f.level_err());
@@ -282,34 +403,9 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
     emit_fallback(P, w, u, 3, ids, 3, M, region_slot);
     return;
   }
-  // apexes: z with id > w, outside S3, ahead of side s, with fewer than
-  // s_max-3 points certainly swept before it
-  for (int side = 0; side < 2; ++side) {
-    const int need = K - cin[side];  // slots left for points swept before z
-    if (need <= 0) continue;
-    const double thr = ntop[side] >= need ? top[side][need - 1] : INFINITY;
-    for (uint32_t s = 0; s < M; ++s) {
-      uint32_t zi = w.ridx[s];
-      if (zi == vi || zi == wi || w.gid[zi] <= wid) continue;
-      V3 p = gpt(w, zi);
-      double Pw, PA, H, HA;
-      f.power(p, Pw, PA);
-      f.height(p, H, HA);
-      if (cert_sign(Pw, PA) <= 0) continue;
-      int sh = cert_sign(H, HA);
-      if ((side == 0 && sh <= 0) || (side == 1 && sh >= 0)) continue;
-      double tau = Pw / fabs(H);
-      double terr = (kCert * PA / fabs(Pw) + kCert * HA / fabs(H) + 1e-15) * tau;
-      double lo = tau - terr;
-      if (lo > thr) continue;  // need points certainly swept before z
-      // inside points behind: counted while their key exceeds z's
-      int lb = cin[side];
-      if (!behind_over[side])
-        for (int b = 0; b < nbehind[side]; ++b) lb += behind[side][b] > tau + terr;
-      if (lb >= K) continue;
-      process_tet(P, w, u, M, f, vi, wi, zi, region_slot);
-    }
-  }
+  if (K <= 0) return;
+  face_side_apexes(P, w, u, M, f, vi, wi, wid, 0, cin0, K, region_slot);
+  face_side_apexes(P, w, u, M, f, vi, wi, wid, 1, cin1, K, region_slot);
 }
 
 // Cells whose union of closed tangent balls (radius rho'_C) contains gathered
