@@ -88,6 +88,7 @@ struct Work {
   uint32_t *ridx;
   uint64_t *rmask;  // 6 words per member
   uint32_t *kidx;
+  float4 *fc;       // FP32 offset and distance per gathered point (certificate and masks)
 };
 
 __device__ inline V3 gpt(const Work &w, uint32_t i) { return {w.gx[i], w.gy[i], w.gz[i]}; }
@@ -408,29 +409,46 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
   face_side_apexes(P, w, u, M, f, vi, wi, wid, 1, cin1, K, region_slot);
 }
 
+// Cell direction and angular radius in FP32, rounded so that the cell only
+// grows (cos delta down, sin delta up): every FP32 test below errs towards a
+// larger certificate radius and a larger region.
+struct CellF {
+  float dx, dy, dz, cosd, sind;
+};
+__device__ inline CellF cell_f(const double *cd) {
+  return {(float)cd[0], (float)cd[1], (float)cd[2], (float)cd[3] * (1.0f - 1e-6f),
+          (float)cd[4] * (1.0f + 1e-6f) + 1e-7f};
+}
+
 // Cells whose union of closed tangent balls (radius rho'_C) contains gathered
 // point i: cos of the smallest angle between its direction and the cell.
+// FP32 with relative and absolute slack; a point may only be included too
+// often, which costs work but never a support.
 __device__ bool member_mask(const Params &P, const Work &w, uint32_t i, uint64_t mask[6]) {
-  const double x = w.gx[i], y = w.gy[i], z = w.gz[i];
-  const double dist = sqrt(x * x + y * y + z * z);
-  bool any = false;
+  const float4 q = w.fc[i];
   for (int k = 0; k < 6; ++k) mask[k] = 0;
+  if (!(q.w > 0.f)) {  // distance lost to underflow: member everywhere
+    for (int k = 0; k < 6; ++k) mask[k] = ~0ull;
+    return true;
+  }
+  const float inv = 1.0f / q.w;
+  bool any = false;
   for (int c = 0; c < kCells; ++c) {
-    const double *cd = P.cells + 8 * c;
-    double ct = (x * cd[0] + y * cd[1] + z * cd[2]) / dist;
-    double cm;  // cos(max(theta - delta, 0))
-    if (ct >= cd[3]) cm = 1.0;
-    else {
-      double st = sqrt(fmax(0.0, 1.0 - ct * ct));
-      cm = ct * cd[3] + st * cd[4];
-    }
-    if (dist <= 2.0 * w.rho[c] * cm * (1.0 + 1e-9) + 1e-300) {
+    const CellF cf = cell_f(P.cells + 8 * c);
+    const float ct = (q.x * cf.dx + q.y * cf.dy + q.z * cf.dz) * inv;
+    float cm;  // cos(max(theta - delta, 0))
+    if (ct >= cf.cosd) cm = 1.0f;
+    else cm = ct * cf.cosd + sqrtf(fmaxf(0.0f, 1.0f - ct * ct)) * cf.sind;
+    const float rho = (float)w.rho[c];
+    if (q.w <= 2.0f * rho * (cm + 1e-5f) * (1.0f + 1e-5f)) {
       mask[c >> 6] |= 1ull << (c & 63);
       any = true;
     }
   }
   return any;
 }
+
+constexpr int kCoverCap = 10;  // s_max - 1 <= 10
 
 __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   __shared__ uint32_t sM, sK, sflag, sregion;
@@ -484,34 +502,41 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     }
   }
   __syncthreads();
-  // 1. direction-cell certificate
+  // FP32 copy of the offsets with their lengths (rounded up), for the
+  // certificate and the masks
+  for (uint32_t i = t; i < G; i += blockDim.x) {
+    const double x = w.gx[i], y = w.gy[i], z = w.gz[i];
+    w.fc[i] = make_float4((float)x, (float)y, (float)z, (float)sqrt(x * x + y * y + z * z) * (1.0f + 1e-6f));
+  }
+  __syncthreads();
+  // 1. direction-cell certificate (FP32; every rounding errs towards larger radii)
   const int need_cover = P.s_max - 1;
   for (int c = t; c < kCells; c += blockDim.x) {
     const double *cd = P.cells + 8 * c;
-    const double dx = cd[0], dy = cd[1], dz = cd[2], cosd = cd[3], sind = cd[4], chord = cd[5];
-    double best[12];
-    int nb = 0;
+    const CellF cf = cell_f(cd);
+    const double chord = cd[5];
+    float best[kCoverCap];
+#pragma unroll
+    for (int j = 0; j < kCoverCap; ++j) best[j] = INFINITY;
     for (uint32_t i = 0; i < G; ++i) {
-      double x = w.gx[i], y = w.gy[i], z = w.gz[i];
-      double dist = sqrt(x * x + y * y + z * z);
-      double ct = (x * dx + y * dy + z * dz) / dist;
-      if (ct <= 0.0) continue;
-      double st = sqrt(fmax(0.0, 1.0 - ct * ct));
-      double cs = ct * cosd - st * sind;  // cos(theta + delta)
-      if (cs < 1e-6) continue;
-      double r = dist / (2.0 * cs);
-      if (nb < need_cover) {
-        int j = nb++;
-        while (j > 0 && best[j - 1] > r) { best[j] = best[j - 1]; j--; }
-        best[j] = r;
-      } else if (r < best[need_cover - 1]) {
-        int j = need_cover - 1;
-        while (j > 0 && best[j - 1] > r) { best[j] = best[j - 1]; j--; }
-        best[j] = r;
-      }
+      const float4 q = w.fc[i];
+      if (!(q.w > 0.f)) continue;  // skipping a point only enlarges the radius
+      const float ct = (q.x * cf.dx + q.y * cf.dy + q.z * cf.dz) / q.w;
+      if (ct <= 0.0f) continue;
+      const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+      const float cs = ct * cf.cosd - st * cf.sind;  // cos(theta + delta)
+      if (cs < 0.01f) continue;  // ill-conditioned cover: skipped
+      const float r = q.w / (2.0f * cs);
+      if (!(r < best[kCoverCap - 1])) continue;
+#pragma unroll
+      for (int j = kCoverCap - 1; j > 0; --j) best[j] = r < best[j - 1] ? best[j - 1] : (r < best[j] ? r : best[j]);
+      best[0] = fminf(best[0], r);
     }
-    double rc = nb >= need_cover ? best[need_cover - 1] * (1.0 + 1e-9) : INFINITY;
-    double dir[3] = {dx, dy, dz};
+    float bk = INFINITY;
+#pragma unroll
+    for (int j = 0; j < kCoverCap; ++j) bk = j == need_cover - 1 ? best[j] : bk;
+    const double rc = (double)bk * (1.0 + 1e-4);
+    double dir[3] = {cd[0], cd[1], cd[2]};
     double uu[3] = {ux, uy, uz};
     double e = INFINITY;
     for (int a = 0; a < 3; ++a) {
@@ -633,7 +658,10 @@ __global__ void __launch_bounds__(kBlock) k_anchor_lds(Params P) {
   __shared__ uint32_t gid[kGatherCap];
   __shared__ double rho[kCells];
   __shared__ uint32_t ridx[kRegionCap];
-  __shared__ uint64_t rmask[6 * kRegionCap];
+  // the FP32 offsets (16 B per gathered point) live in the mask area until
+  // the masks are written
+  __shared__ __align__(16) uint64_t rmask[6 * kRegionCap];
+  static_assert(sizeof(uint64_t) * 6 * kRegionCap >= sizeof(float4) * kGatherCap, "fc alias");
   __shared__ uint32_t kidx[kRegionCap];
   const uint32_t a = blockIdx.x;
   if (a >= P.anchor_count) return;
@@ -642,8 +670,14 @@ __global__ void __launch_bounds__(kBlock) k_anchor_lds(Params P) {
     if (threadIdx.x == 0) P.status[u] = ST_BIG;
     return;
   }
-  Work w{gx, gy, gz, gid, rho, ridx, rmask, kidx};
+  Work w{gx, gy, gz, gid, rho, ridx, rmask, kidx, reinterpret_cast<float4 *>(rmask)};
   anchor_work(P, w, u);
+}
+
+// Offset (in doubles, even) of the FP32 offsets in a global slab of gather
+// capacity G; the slab holds big_fc_offset(G) + 2G doubles.
+__host__ __device__ inline uint64_t big_fc_offset(uint64_t G) {
+  return (3 * G + kCells + 6 * G + (3 * G + 1) / 2 + 1 + 1) & ~1ull;
 }
 
 // Anchors whose gather or region exceeds LDS: the same work on global slabs.
@@ -662,6 +696,7 @@ __global__ void __launch_bounds__(kBlock) k_anchor_big(Params P) {
   w.gid = (uint32_t *)(slab + 3 * G + kCells + 6 * G);
   w.ridx = w.gid + G;
   w.kidx = w.ridx + G;
+  w.fc = reinterpret_cast<float4 *>(slab + big_fc_offset(G));
   anchor_work(P, w, u);
 }
 
@@ -930,7 +965,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
         uint32_t gmax = 0;
         for (uint32_t u : big) gmax = std::max(gmax, gc[u]);
         gmax = std::max<uint32_t>(gmax, 64);
-        const uint64_t slab = 3ull * gmax + kCells + 6ull * gmax + (3ull * gmax + 1) / 2 + 1;
+        const uint64_t slab = big_fc_offset(gmax) + 2ull * gmax;
         const uint32_t chunk = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(bc, (1ull << 30) / (slab * 8)));
         B.scratch.ensure(slab * chunk);
         // the CSR of the big anchors is still valid from this iteration's gather

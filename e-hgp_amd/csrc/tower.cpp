@@ -250,8 +250,9 @@ void contract_order(int k, int64_t n, TowerOrderData &D, TowerState &F) {
   // carrier DSU; each class may own a reduced root
   const int64_t M = k == 1 ? n : (int64_t)D.minima;
   std::vector<int32_t> owner(M, -1);  // forest node of a component root (-1: none)
+  F.reserve(k == 1 ? (size_t)n + D.saddles / 8 : D.saddles / 8);
   if (k == 1)
-    for (int64_t p = 0; p < n; ++p) owner[p] = F.add(1, 0, -1, 0.0, {});
+    for (int64_t p = 0; p < n; ++p) owner[p] = F.add(1, 0, -1, 0.0);
   Dsu dsu(M);
   std::vector<int64_t> touched, bp_parent;
   std::unordered_map<int64_t, int64_t> bp;
@@ -268,8 +269,11 @@ void contract_order(int k, int64_t n, TowerOrderData &D, TowerState &F) {
       // single saddle at this level: distinct carrier roots are the group
       int64_t members[4];
       int nm = 0;
+      uint32_t last = kInvalid;
       for (uint32_t a = D.arm_base[i]; a < D.arm_base[i + 1]; ++a) {
-        int64_t r = dsu.find(D.carrier[a]);
+        if (D.carrier[a] == last) continue;  // same carrier, same root
+        last = D.carrier[a];
+        int64_t r = dsu.find(last);
         bool seen = false;
         for (int q = 0; q < nm; ++q) seen |= members[q] == r;
         if (!seen) members[nm++] = r;
@@ -282,14 +286,14 @@ void contract_order(int k, int64_t n, TowerOrderData &D, TowerState &F) {
         std::sort(roots, roots + nr);
         nr = (int)(std::unique(roots, roots + nr) - roots);
         int64_t node;
-        if (nr == 0) node = F.add(k, 1, D.cls[i], D.level[i], {});
+        if (nr == 0) node = F.add(k, 1, D.cls[i], D.level[i]);
         else if (nr == 1) node = roots[0];
-        else node = F.add(k, 2, D.cls[i], D.level[i], std::vector<int64_t>(roots, roots + nr));
+        else node = F.add(k, 2, D.cls[i], D.level[i], roots, nr);
         for (int q = 1; q < nm; ++q) dsu.p[members[q]] = members[0];
         owner[members[0]] = node;
       } else if (owner[members[0]] < 0) {
         // all arms in one latent minimum cannot happen in general position
-        owner[members[0]] = F.add(k, 1, D.cls[i], D.level[i], {});
+        owner[members[0]] = F.add(k, 1, D.cls[i], D.level[i]);
       }
       i = j;
       continue;
@@ -324,7 +328,7 @@ void contract_order(int k, int64_t n, TowerOrderData &D, TowerState &F) {
       std::sort(roots.begin(), roots.end());
       roots.erase(std::unique(roots.begin(), roots.end()), roots.end());
       int64_t node;
-      if (roots.empty()) node = F.add(k, 1, D.cls[i], D.level[i], {});
+      if (roots.empty()) node = F.add(k, 1, D.cls[i], D.level[i]);
       else if (roots.size() == 1) node = roots[0];
       else node = F.add(k, 2, D.cls[i], D.level[i], roots);
       for (size_t q = 1; q < members.size(); ++q) dsu.p[dsu.find(members[q])] = dsu.find(members[0]);

@@ -19,14 +19,25 @@ struct TowerState {
   int32_t orders = 0;
   int32_t status = 0;
 
-  int64_t add(int32_t o, int32_t k, int64_t cls, double lev, const std::vector<int64_t> &kids) {
+  int64_t add(int32_t o, int32_t k, int64_t cls, double lev, const int64_t *kids = nullptr, size_t nk = 0) {
     order.push_back(o);
     kind.push_back(k);
     level_class.push_back(cls);
     level.push_back(lev);
-    children.insert(children.end(), kids.begin(), kids.end());
+    children.insert(children.end(), kids, kids + nk);
     child_offset.push_back((int64_t)children.size());
     return (int64_t)order.size() - 1;
+  }
+  int64_t add(int32_t o, int32_t k, int64_t cls, double lev, const std::vector<int64_t> &kids) {
+    return add(o, k, cls, lev, kids.data(), kids.size());
+  }
+  void reserve(size_t nodes) {
+    order.reserve(nodes);
+    kind.reserve(nodes);
+    level_class.reserve(nodes);
+    level.reserve(nodes);
+    child_offset.reserve(nodes + 1);
+    children.reserve(2 * nodes);
   }
 };
 
