@@ -275,9 +275,21 @@ __device__ inline void sweep_key(double Pw, double PA, double H, double HA, doub
 // already ahead), each checked against the inside points behind the side.
 // One pass over the region fills the side's sweep list; the candidates are
 // packed into a mask before any tetrahedron is classified.
+// Capsule holding every sphere of a face's pencil that can be critical:
+// centres c3 + t n with |t| <= T, radii <= Rb.  A point farther than T + Rb
+// from c3 is outside all of them (checked with a relative margin).
+struct Capsule {
+  V3 c3;
+  double bound2;  // (T + Rb)^2 with margin; INFINITY disables the test
+  __device__ bool outside(const V3 &p) const {
+    const V3 d = sub(p, c3);
+    return dot(d, d) > bound2;
+  }
+};
+
 __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uint32_t M, const TriSphere &f,
-                                 uint32_t vi, uint32_t wi, uint32_t wid, int side, int cin_side, int K,
-                                 uint32_t *region_slot) {
+                                 const Capsule &cap, uint32_t vi, uint32_t wi, uint32_t wid, int side,
+                                 int cin_side, int K, uint32_t *region_slot) {
   const int need = K - cin_side;  // slots left for points swept before z
   if (need <= 0) return;
   SweepList top;
@@ -290,6 +302,9 @@ __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uin
     const uint32_t gi = w.ridx[s];
     if (gi == vi || gi == wi) continue;
     const V3 p = gpt(w, gi);
+    // outside every feasible sphere of the pencil: it can neither be an apex
+    // nor be swept before one (dropping it only loosens the threshold)
+    if (cap.outside(p)) continue;
     double Pw, PA, H, HA;
     f.power(p, Pw, PA);
     f.height(p, H, HA);
@@ -316,6 +331,7 @@ __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uin
       const uint32_t zi = w.ridx[s];
       if (zi == vi || zi == wi || w.gid[zi] <= wid) continue;
       const V3 p = gpt(w, zi);
+      if (cap.outside(p)) continue;
       double Pw, PA, H, HA;
       f.power(p, Pw, PA);
       f.height(p, H, HA);
@@ -354,7 +370,7 @@ __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uin
 // interior and counts the inside points ahead of each side; each side then
 // takes one pass of its own.
 __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_t M, uint32_t vi,
-                             uint32_t wi, uint32_t *region_slot) {
+                             uint32_t wi, double rmax, uint32_t *region_slot) {
   // the sphere is built from its points in PointId order, so its FP64 level
   // (and every later tetrahedron's) does not depend on the enumeration order
   uint32_t ids[2] = {w.gid[vi], w.gid[wi]};
@@ -370,6 +386,21 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
   const uint32_t wid = ids[1];
   const int smax = P.s_max;
   const int K = smax - 3;  // an apex must be among the first K points swept
+  // pencil capsule: a critical sphere through u, v, w has its direction in a
+  // cell shared by v and w, so its radius is at most rmax; its centre is
+  // c3 + t n with r3^2 + t^2 <= rmax^2.  Only well-conditioned faces use it.
+  Capsule cap;
+  cap.c3 = {0.0, 0.0, 0.0};
+  cap.bound2 = INFINITY;
+  if (f.D > 1e-6 * f.DA) {
+    const double s2 = 0.5 / f.D;
+    cap.c3 = {(f.al * f.a.x + f.be * f.b.x) * s2, (f.al * f.a.y + f.be * f.b.y) * s2,
+              (f.al * f.a.z + f.be * f.b.z) * s2};
+    const double r3 = sqrt(dot(cap.c3, cap.c3));
+    const double T = sqrt(fmax(0.0, rmax * rmax - r3 * r3));
+    const double Rb = fmax(rmax, r3);
+    cap.bound2 = (T + Rb) * (T + Rb) * (1.0 + 1e-6);
+  }
   int cin0 = 0, cin1 = 0;  // inside the triangle's sphere, ahead of side 0 / 1
   int total_in = 0;
   uint32_t tri_int[kMaxInterior];
@@ -378,6 +409,7 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
     const uint32_t gi = w.ridx[s];
     if (gi == vi || gi == wi) continue;
     const V3 p = gpt(w, gi);
+    if (cap.outside(p)) continue;  // outside the triangle's sphere and every feasible one
     double Pw, PA, H, HA;
     f.power(p, Pw, PA);
     f.height(p, H, HA);
@@ -405,8 +437,8 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
     return;
   }
   if (K <= 0) return;
-  face_side_apexes(P, w, u, M, f, vi, wi, wid, 0, cin0, K, region_slot);
-  face_side_apexes(P, w, u, M, f, vi, wi, wid, 1, cin1, K, region_slot);
+  face_side_apexes(P, w, u, M, f, cap, vi, wi, wid, 0, cin0, K, region_slot);
+  face_side_apexes(P, w, u, M, f, cap, vi, wi, wid, 1, cin1, K, region_slot);
 }
 
 // Cell direction and angular radius in FP32, rounded so that the cell only
@@ -647,7 +679,7 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     }
     // prune only well-conditioned faces (FP64 r2 is then accurate to ~1e-10)
     if (D > 1e-6 * (aa * bb + fabs(ab) * fabs(ab)) && r2 > rmax * rmax * (1.0 + 1e-6)) continue;
-    process_face(P, w, u, M, vi, wi, &sregion);
+    process_face(P, w, u, M, vi, wi, rmax, &sregion);
   }
   __syncthreads();
   if (t == 0) P.status[u] = ST_DONE;
