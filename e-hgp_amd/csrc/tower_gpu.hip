@@ -241,30 +241,58 @@ struct ArmParams {
 constexpr int kArmBlock = 64;
 constexpr int kArmStack = 40;  // LBVH depth is 18 at 50k points, 32 at 200k; overflow goes to the host
 
-// One thread per strict arm: hash lookup of the facet, else certified
-// descents until a minimum's label is found.
-__global__ void __launch_bounds__(kArmBlock) k_resolve_arms(ArmParams A) {
-  // traversal stacks, interleaved by lane so equal depths hit distinct banks
-  __shared__ uint32_t stk[kArmStack * kArmBlock];
-  const int lane = threadIdx.x;
-  const int64_t arm = (int64_t)blockIdx.x * kArmBlock + lane;
-  if (arm >= A.arms) return;
+// Facet of an arm: the saddle's closed label without the arm's shell point.
+__device__ inline void arm_facet(const ArmParams &A, int64_t arm, uint32_t *F) {
   const uint32_t s = A.arm_saddle[arm];
   const int j = (int)(arm - A.arm_base[s]);
   const EventRec &e = A.ev[A.order[A.saddle_pos[s]]];
   uint32_t L[kMaxLabel];
   const int len = event_label(e, L);
-  const int k = A.k;
-  uint32_t F[kMaxLabel];
   int f = 0;
   for (int i = 0; i < len; ++i)
     if (L[i] != e.ids[j]) F[f++] = L[i];
+}
+
+// One thread per strict arm: the facet is a minimum (hash hit) or the arm
+// is queued for descents.  Splitting the two keeps the descending lanes of a
+// wave together.
+__global__ void k_arm_lookup(ArmParams A, uint32_t *queue, unsigned long long *qcount) {
+  const int64_t arm = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (arm >= A.arms) return;
+  uint32_t F[kMaxLabel];
+  arm_facet(A, arm, F);
+  if (A.k == 1) {  // order one: the arm is a point, its own minimum
+    A.carrier[arm] = F[0];
+    return;
+  }
+  const int64_t hit = lookup(F, A.k, A.table_keys, A.table_vals, A.mask, A.labels);
+  if (hit >= 0) {
+    A.carrier[arm] = (uint32_t)hit;
+    return;
+  }
+  A.carrier[arm] = kInvalid;
+  queue[atomicAdd(qcount, 1ull)] = (uint32_t)arm;
+}
+
+// One thread per queued arm: certified descents until a minimum's label is
+// found (the facet itself is not one).
+__global__ void __launch_bounds__(kArmBlock) k_resolve_arms(ArmParams A, const uint32_t *queue, uint32_t nq) {
+  // traversal stacks, interleaved by lane so equal depths hit distinct banks
+  __shared__ uint32_t stk[kArmStack * kArmBlock];
+  const int lane = threadIdx.x;
+  const int64_t qi = (int64_t)blockIdx.x * kArmBlock + lane;
+  if (qi >= nq) return;
+  const int64_t arm = queue[qi];
+  const int k = A.k;
+  uint32_t F[kMaxLabel];
+  arm_facet(A, arm, F);
   uint32_t result = kInvalid;
   uint64_t local_steps = 0;
-  if (k == 1) result = F[0];  // order one: the arm is a point, its own minimum
   for (int step = 0; step < 4096 && result == kInvalid; ++step) {
-    const int64_t hit = lookup(F, k, A.table_keys, A.table_vals, A.mask, A.labels);
-    if (hit >= 0) { result = (uint32_t)hit; break; }
+    if (step > 0) {
+      const int64_t hit = lookup(F, k, A.table_keys, A.table_vals, A.mask, A.labels);
+      if (hit >= 0) { result = (uint32_t)hit; break; }
+    }
     // miniball of the facet
     V3 pts[kMaxLabel];
     for (int i = 0; i < k; ++i) pts[i] = P3(A.X, A.Y, A.Z, F[i]);
@@ -439,8 +467,20 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   A.nodes = c.nodes.p; A.LX = c.LX.p; A.LY = c.LY.p; A.LZ = c.LZ.p; A.leaf_id = c.leaf_id.p; A.n = c.n;
   A.carrier = carrier.p;
   A.steps = (unsigned long long *)steps.p;
+  DevBuf<uint32_t> queue;
+  DevBuf<uint64_t> qcount;
+  queue.ensure(arms);
+  qcount.ensure(1);
+  MHGP_HIP(hipMemsetAsync(qcount.p, 0, 8, s));
   c.arm_clock.start(s);
-  hipLaunchKernelGGL(k_resolve_arms, dim3(blocks_for(arms, kArmBlock)), dim3(kArmBlock), 0, s, A);
+  hipLaunchKernelGGL(k_arm_lookup, dim3(blocks_for(arms, 256)), dim3(256), 0, s, A, queue.p,
+                     (unsigned long long *)qcount.p);
+  uint64_t nq = 0;
+  MHGP_HIP(hipMemcpyAsync(&nq, qcount.p, 8, hipMemcpyDeviceToHost, s));
+  MHGP_HIP(hipStreamSynchronize(s));
+  if (nq)
+    hipLaunchKernelGGL(k_resolve_arms, dim3(blocks_for(nq, kArmBlock)), dim3(kArmBlock), 0, s, A, queue.p,
+                       (uint32_t)nq);
   c.arm_clock.stop(s);
   MHGP_LAUNCH_CHECK();
   c.stats.arm_ms += c.arm_clock.ms();
