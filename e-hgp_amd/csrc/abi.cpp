@@ -44,6 +44,8 @@ int guarded(mhgp::Context *c, F &&f) {
     return fail(c, MHGP_E_STATE, e.what(), false);
   } catch (const mhgp::UnsupportedError &e) {
     return fail(c, MHGP_E_UNSUPPORTED, e.what(), false);
+  } catch (const mhgp::CapacityError &e) {
+    return fail(c, MHGP_E_CAPACITY, e.what(), false);
   } catch (const mhgp::HipError &e) {
     return fail(c, MHGP_E_DEVICE, e.what(), true);
   } catch (const std::exception &e) {

@@ -17,6 +17,10 @@ struct NonFiniteError : std::invalid_argument {
 struct StateError : std::logic_error {
   using std::logic_error::logic_error;
 };
+// Work beyond a documented device capacity.
+struct CapacityError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
 // A well-defined input for which the requested object does not exist
 // (e.g. the direct tower of a cloud with extra-shell diagnostics).
 struct UnsupportedError : std::runtime_error {

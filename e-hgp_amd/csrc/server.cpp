@@ -124,6 +124,8 @@ void Server::worker_main() {
       status = MHGP_E_ARGUMENT;
     } catch (const UnsupportedError &e) {
       status = MHGP_E_UNSUPPORTED;
+    } catch (const CapacityError &e) {
+      status = MHGP_E_CAPACITY;
     } catch (const HipError &e) {
       status = MHGP_E_DEVICE;
       std::lock_guard<std::mutex> g(mu);

@@ -53,7 +53,12 @@ namespace {
 
 constexpr int kBlock = 256;
 
-enum : uint32_t { ST_PENDING = 0, ST_DONE = 1, ST_WIDEN = 2, ST_BIG = 3 };
+enum : uint32_t { ST_PENDING = 0, ST_DONE = 1, ST_WIDEN = 2, ST_BIG = 3, ST_HUGE = 4 };
+
+// Regions of the global-memory path are enumerated face by face in
+// O(region^3) work; beyond this size one anchor would keep a workgroup busy
+// for many seconds, and the catalogue reports a capacity error instead.
+constexpr uint32_t kBigRegionMax = 2048;
 
 struct Params {
   const double *X, *Y, *Z;       // canonical coordinates
@@ -346,20 +351,21 @@ __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uin
     }
     return;
   }
-  uint32_t cand = 0, pack[3] = {0u, 0u, 0u};
+  uint32_t cand = 0;
 #pragma unroll
   for (int j = 0; j < SweepList::kCap; ++j) {
     const double terr = (double)top.err[j], h = top.hi[j];
     // inside points behind: counted while their key exceeds z's
     const bool ok = top.slot[j] != kInvalid && h - 2.0 * terr <= thr && cin_side + beh.count_above(h) < K;
     cand |= ok ? (1u << j) : 0u;
-    pack[j / 3] |= (ok ? top.slot[j] : 0u) << (10 * (j % 3));
   }
   while (cand) {
     const int j = __ffs(cand) - 1;
     cand &= cand - 1;
-    const uint32_t word = j < 3 ? pack[0] : (j < 6 ? pack[1] : pack[2]);
-    const uint32_t zi = w.ridx[(word >> (10 * (j % 3))) & 1023u];
+    uint32_t slot = 0;  // selected by static index, the list stays in registers
+#pragma unroll
+    for (int q = 0; q < SweepList::kCap; ++q) slot = q == j ? top.slot[q] : slot;
+    const uint32_t zi = w.ridx[slot];
     if (w.gid[zi] <= wid) continue;
     process_tet(P, w, u, M, f, vi, wi, zi, region_slot);
   }
@@ -641,6 +647,10 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   const uint32_t M = sM;
   if (M > P.rcap) {
     if (t == 0) P.status[u] = ST_BIG;
+    return;
+  }
+  if (P.big && M > kBigRegionMax) {
+    if (t == 0) P.status[u] = ST_HUGE;
     return;
   }
   for (uint32_t s = t; s < M; s += blockDim.x)
@@ -1006,6 +1016,8 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
           uint32_t bn = std::min(chunk, bc - b0);
           MHGP_HIP(hipMemcpyAsync(B.anchors.p, big.data() + b0, bn * 4, hipMemcpyHostToDevice, s));
           Params Q = P;
+          // the gather CSR may have been reallocated by a regather below
+          Q.gcount = B.gcount.p; Q.goff = B.goff.p; Q.glist = B.glist.p; Q.radius = B.radius.p;
           Q.anchors = B.anchors.p; Q.anchor_count = bn; Q.big = 1;
           Q.scratch = B.scratch.p; Q.slab_doubles = slab; Q.gcap = gmax; Q.rcap = gmax;
           hipLaunchKernelGGL(k_anchor_big, dim3(bn), dim3(kBlock), 0, s, Q);
@@ -1019,6 +1031,9 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
             rad[u] = std::max(need[u] * 1.02, rad[u] * 1.05);
             again.push_back(u);
             widened++;
+          } else if (st[u] == ST_HUGE) {
+            throw CapacityError("an anchor's candidate region exceeds " + std::to_string(kBigRegionMax) +
+                                " points (dense anisotropic cluster); the device catalogue does not enumerate it");
           } else if (st[u] != ST_DONE) {
             throw std::runtime_error("a big anchor did not complete");
           }
