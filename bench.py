@@ -139,6 +139,44 @@ def served_stream(k_values=(4, 10), count=48, rate_hz=None, slots=4, seed=7):
     }
 
 
+def batched(count=256, slots=4, plain_subset=32):
+    """256 clouds (sizes log-uniform in [1 000, 50 000], each a jittered
+    lattice or a 5-20 blob anisotropic mixture) submitted at once to the
+    server; the plain path runs the first clouds one after another."""
+    sizes = workloads.batch_sizes(count)
+    clouds = [workloads.batch_cloud(int(m), seed=2000 + i) for i, m in enumerate(sizes)]
+    codes = {}
+    ctx = mhgp.Context(0)
+    t0 = time.perf_counter()
+    for pts in clouds[:plain_subset]:
+        try:
+            run_step(ctx, pts, 10)
+        except mhgp.MhgpError:
+            pass
+    plain_s = time.perf_counter() - t0
+    ctx.close()
+    srv = mhgp.Server(0, slots=slots)
+    t0 = time.perf_counter()
+    tickets = [srv.submit(p, 10) for p in clouds]
+    lat = []
+    for i, t in enumerate(tickets):
+        r = srv.wait(t)
+        codes[int(r.code)] = codes.get(int(r.code), 0) + 1
+        if (i + 1) % 32 == 0:
+            print(f"batched: {i + 1}/{count} done", file=sys.stderr, flush=True)
+        lat.append(r.latency_ms)
+        srv.release(t)
+    served_s = time.perf_counter() - t0
+    srv.close()
+    return {
+        "clouds": count, "points": int(sizes.sum()), "slots": slots,
+        "served_clouds_per_s": round(count / served_s, 3),
+        "plain_clouds_per_s_first_%d" % plain_subset: round(plain_subset / plain_s, 3),
+        "result_codes": {str(k): v for k, v in sorted(codes.items())},
+        "served_total_s": round(served_s, 2),
+    }
+
+
 def traffic_from_profile():
     path = os.path.join(ROOT, "profiles", "anchor_kernel_c2_traffic.json")
     if not os.path.exists(path):
@@ -156,6 +194,7 @@ def main() -> None:
     ap.add_argument("--kmax", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--served", action="store_true", help="add the served-stream leg")
+    ap.add_argument("--batched", action="store_true", help="add the 256-cloud batch leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain kernel of known traffic (counter scaling)")
     args = ap.parse_args()
@@ -227,6 +266,8 @@ def main() -> None:
     ctx.close()
     if args.served:
         line["config"]["served"] = served_stream()
+    if args.batched:
+        line["config"]["batched"] = batched()
     print(json.dumps(line))
 
 
