@@ -96,3 +96,24 @@ def test_catalog_euler_characteristic_lattice_5k():
     assert cat.info.diagnostic_count == 0
     for k in range(1, 9):  # complete windows: ranks k..k+3 <= s_max = 11
         assert _euler(cat, len(pts), k) == 1, k
+
+
+def test_capacity_error_is_clean_and_context_survives():
+    """Anchors whose candidate region exceeds the global-memory path's bound
+    report MHGP_E_CAPACITY (no fault, no poisoning); the same context then
+    serves an ordinary cloud."""
+    pts = workloads.gaussian_mixture(50_000, seed=0)
+    print("capacity case: snapped mixture n=50000 k=10", flush=True)
+    with mhgp.Context(0) as ctx:
+        ctx.upload(pts)
+        try:
+            ctx.supports(10, fetch=False)
+        except mhgp.MhgpError as err:
+            assert err.code == mhgp.E_CAPACITY, err
+        small = workloads.uniform_cloud(400, seed=9)
+        ctx.upload(small)
+        cat = ctx.supports(4)
+    with mhgp.Context(0) as fresh:
+        fresh.upload(small)
+        ref = fresh.supports(4)
+    assert np.array_equal(cat.ids, ref.ids) and np.array_equal(cat.level_class, ref.level_class)
