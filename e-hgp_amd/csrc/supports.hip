@@ -53,6 +53,13 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// LDS written by some lanes of a wave, then read by other lanes of it
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 enum : uint32_t { ST_PENDING = 0, ST_DONE = 1, ST_WIDEN = 2, ST_BIG = 3, ST_HUGE = 4 };
 
 // Regions of the global-memory path are enumerated face by face in
@@ -292,9 +299,26 @@ struct Capsule {
   }
 };
 
+// Per-wave work queues of the face phase (LDS).  The first pass of every
+// face runs one face per lane; the per-side passes and the tetrahedra it
+// leads to are queued and then taken one per lane, so the lanes of a wave
+// run passes of equal length instead of waiting for the face with the most
+// apexes.  A full queue falls back to doing the work in place.
+constexpr uint32_t kSideQ = 128;  // >= 2 items for each of a wave's 64 faces
+constexpr uint32_t kTetQ = 256;
+struct WaveQueues {
+  uint32_t *side;    // vi | wi << 10 | side << 20 | cin << 21 (gathered indices, < 1024)
+  double *side_r;    // the face's rmax: the side pass rebuilds exactly the face pass's capsule
+  uint32_t *tet;     // vi | wi << 10 | zi << 20
+  uint32_t *nside, *ntet;
+};
+
+__device__ void process_tet(const Params &P, const Work &w, uint32_t u, uint32_t M, const TriSphere &f,
+                            uint32_t vi, uint32_t wi, uint32_t zi, uint32_t *region_slot);
+
 __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uint32_t M, const TriSphere &f,
                                  const Capsule &cap, uint32_t vi, uint32_t wi, uint32_t wid, int side,
-                                 int cin_side, int K, uint32_t *region_slot) {
+                                 int cin_side, int K, uint32_t *region_slot, WaveQueues *q) {
   const int need = K - cin_side;  // slots left for points swept before z
   if (need <= 0) return;
   SweepList top;
@@ -367,16 +391,41 @@ __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uin
     for (int q = 0; q < SweepList::kCap; ++q) slot = q == j ? top.slot[q] : slot;
     const uint32_t zi = w.ridx[slot];
     if (w.gid[zi] <= wid) continue;
+    if (q) {
+      const uint32_t at = atomicAdd(q->ntet, 1u);
+      if (at < kTetQ) {
+        q->tet[at] = vi | (wi << 10) | (zi << 20);
+        continue;
+      }
+    }
     process_tet(P, w, u, M, f, vi, wi, zi, region_slot);
   }
+}
+
+// The pencil capsule of face (u, v, w) for critical radii up to rmax.
+__device__ inline Capsule face_capsule(const TriSphere &f, double rmax) {
+  Capsule cap;
+  cap.c3 = {0.0, 0.0, 0.0};
+  cap.bound2 = INFINITY;
+  if (f.D > 1e-6 * f.DA) {
+    const double s2 = 0.5 / f.D;
+    cap.c3 = {(f.al * f.a.x + f.be * f.b.x) * s2, (f.al * f.a.y + f.be * f.b.y) * s2,
+              (f.al * f.a.z + f.be * f.b.z) * s2};
+    const double r3 = sqrt(dot(cap.c3, cap.c3));
+    const double T = sqrt(fmax(0.0, rmax * rmax - r3 * r3));
+    const double Rb = fmax(rmax, r3);
+    cap.bound2 = (T + Rb) * (T + Rb) * (1.0 + 1e-6);
+  }
+  return cap;
 }
 
 // Face (u, v, w): its triangle event, then the apexes of each side.  The
 // first pass certifies every side and insideness, builds the triangle's
 // interior and counts the inside points ahead of each side; each side then
 // takes one pass of its own.
+template <bool QUEUED>
 __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_t M, uint32_t vi,
-                             uint32_t wi, double rmax, uint32_t *region_slot) {
+                             uint32_t wi, double rmax, uint32_t *region_slot, WaveQueues *q) {
   // the sphere is built from its points in PointId order, so its FP64 level
   // (and every later tetrahedron's) does not depend on the enumeration order
   uint32_t ids[2] = {w.gid[vi], w.gid[wi]};
@@ -395,18 +444,7 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
   // pencil capsule: a critical sphere through u, v, w has its direction in a
   // cell shared by v and w, so its radius is at most rmax; its centre is
   // c3 + t n with r3^2 + t^2 <= rmax^2.  Only well-conditioned faces use it.
-  Capsule cap;
-  cap.c3 = {0.0, 0.0, 0.0};
-  cap.bound2 = INFINITY;
-  if (f.D > 1e-6 * f.DA) {
-    const double s2 = 0.5 / f.D;
-    cap.c3 = {(f.al * f.a.x + f.be * f.b.x) * s2, (f.al * f.a.y + f.be * f.b.y) * s2,
-              (f.al * f.a.z + f.be * f.b.z) * s2};
-    const double r3 = sqrt(dot(cap.c3, cap.c3));
-    const double T = sqrt(fmax(0.0, rmax * rmax - r3 * r3));
-    const double Rb = fmax(rmax, r3);
-    cap.bound2 = (T + Rb) * (T + Rb) * (1.0 + 1e-6);
-  }
+  const Capsule cap = face_capsule(f, rmax);
   int cin0 = 0, cin1 = 0;  // inside the triangle's sphere, ahead of side 0 / 1
   int total_in = 0;
   uint32_t tri_int[kMaxInterior];
@@ -443,8 +481,41 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
     return;
   }
   if (K <= 0) return;
-  face_side_apexes(P, w, u, M, f, cap, vi, wi, wid, 0, cin0, K, region_slot);
-  face_side_apexes(P, w, u, M, f, cap, vi, wi, wid, 1, cin1, K, region_slot);
+  for (int side = 0; side < 2; ++side) {
+    const int cin_side = side == 0 ? cin0 : cin1;
+    if (cin_side >= K) continue;
+    if (QUEUED) {
+      // at most 2 items for each of the wave's 64 faces: the queue cannot fill
+      const uint32_t at = atomicAdd(q->nside, 1u);
+      if (at < kSideQ) {
+        q->side[at] = vi | (wi << 10) | ((uint32_t)side << 20) | ((uint32_t)cin_side << 21);
+        q->side_r[at] = rmax;
+      } else {
+        atomicAdd(&P.counters[4], 1ull);  // reported by the host as an internal error
+      }
+    } else {
+      face_side_apexes(P, w, u, M, f, cap, vi, wi, wid, side, cin_side, K, region_slot, nullptr);
+    }
+  }
+}
+
+// A queued side of a face: the face's sphere rebuilt (same arithmetic, same
+// values), then the side's pass.
+__device__ void side_item(const Params &P, const Work &w, uint32_t u, uint32_t M, uint32_t lo, double rmax,
+                          uint32_t *region_slot, WaveQueues *q) {
+  const uint32_t vi = lo & 1023u, wi = (lo >> 10) & 1023u, side = (lo >> 20) & 1u, cin_side = lo >> 21;
+  TriSphere f;
+  f.init(gpt(w, vi), gpt(w, wi));
+  const Capsule cap = face_capsule(f, rmax);
+  face_side_apexes(P, w, u, M, f, cap, vi, wi, w.gid[wi], (int)side, (int)cin_side, P.s_max - 3, region_slot, q);
+}
+
+__device__ void tet_item(const Params &P, const Work &w, uint32_t u, uint32_t M, uint32_t item,
+                         uint32_t *region_slot) {
+  const uint32_t vi = item & 1023u, wi = (item >> 10) & 1023u, zi = item >> 20;
+  TriSphere f;
+  f.init(gpt(w, vi), gpt(w, wi));
+  process_tet(P, w, u, M, f, vi, wi, zi, region_slot);
 }
 
 // Cell direction and angular radius in FP32, rounded so that the cell only
@@ -488,6 +559,9 @@ __device__ bool member_mask(const Params &P, const Work &w, uint32_t i, uint64_t
 
 constexpr int kCoverCap = 10;  // s_max - 1 <= 10
 
+// BIG: arrays in global slabs, regions beyond LDS; the face phase works in
+// place instead of through the wave queues.
+template <bool BIG>
 __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   __shared__ uint32_t sM, sK, sflag, sregion;
   __shared__ double smax_rho;
@@ -659,37 +733,64 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   const uint32_t Kc = sK;
   // 3. pairs (u, v), v > u
   for (uint32_t k = t; k < Kc; k += blockDim.x) process_pair(P, w, u, M, w.ridx[w.kidx[k]], &sregion);
-  // 4. faces (u, v, w), u < v < w, whose members share a cell
+  // 4. faces (u, v, w), u < v < w, whose members share a cell.  Each wave
+  // takes 64 candidate faces at a time: first passes one per lane, then the
+  // queued sides and tetrahedra one per lane (LDS path; the global-memory
+  // path works in place).
+  __shared__ uint32_t q_side[kBlock / 64][kSideQ];
+  __shared__ double q_side_r[kBlock / 64][kSideQ];
+  __shared__ uint32_t q_tet[kBlock / 64][kTetQ];
+  __shared__ uint32_t q_n[kBlock / 64][2];
+  const int wave = t >> 6, lane = t & 63, nwaves = (int)(blockDim.x >> 6);
+  WaveQueues wq{q_side[wave], q_side_r[wave], q_tet[wave], &q_n[wave][0], &q_n[wave][1]};
+  WaveQueues *q = BIG ? nullptr : &wq;
+  if (lane == 0) { q_n[wave][0] = 0; q_n[wave][1] = 0; }
+  wave_sync();
   const uint64_t F = (uint64_t)Kc * (Kc - 1) / 2;
-  for (uint64_t fidx = t; fidx < F; fidx += blockDim.x) {
-    // decode the pair index (i < j) of the strictly lower triangle
-    uint64_t j = (uint64_t)((1.0 + sqrt(1.0 + 8.0 * (double)fidx)) * 0.5);
-    while (j * (j - 1) / 2 > fidx) j--;
-    while ((j + 1) * j / 2 <= fidx) j++;
-    uint64_t i = fidx - j * (j - 1) / 2;
-    uint32_t si = w.kidx[i], sj = w.kidx[j];
-    bool share = false;
-    for (int k = 0; k < 6; ++k) share |= (w.rmask[6 * si + k] & w.rmask[6 * sj + k]) != 0;
-    if (!share) continue;
-    // a sphere through u, v, w has radius >= r3; skip faces beyond every shared cell
-    uint32_t vi = w.ridx[si], wi = w.ridx[sj];
-    V3 a = gpt(w, vi), b = gpt(w, wi);
-    double aa = dot(a, a), bb = dot(b, b), ab = dot(a, b);
-    double D = aa * bb - ab * ab;
-    V3 amb = sub(a, b);
-    double r2 = D > 0 ? aa * bb * dot(amb, amb) / (4.0 * D) : INFINITY;
-    double rmax = 0;
-    for (int k = 0; k < 6; ++k) {
-      uint64_t mm = w.rmask[6 * si + k] & w.rmask[6 * sj + k];
-      while (mm) {
-        int bit = __ffsll((long long)mm) - 1;
-        mm &= mm - 1;
-        rmax = fmax(rmax, w.rho[64 * k + bit]);
+  for (uint64_t base = (uint64_t)wave * 64; base < F; base += (uint64_t)nwaves * 64) {
+    const uint64_t fidx = base + lane;
+    if (fidx < F) {
+      // decode the pair index (i < j) of the strictly lower triangle
+      uint64_t j = (uint64_t)((1.0 + sqrt(1.0 + 8.0 * (double)fidx)) * 0.5);
+      while (j * (j - 1) / 2 > fidx) j--;
+      while ((j + 1) * j / 2 <= fidx) j++;
+      const uint64_t i = fidx - j * (j - 1) / 2;
+      const uint32_t si = w.kidx[i], sj = w.kidx[j];
+      bool share = false;
+      for (int k = 0; k < 6; ++k) share |= (w.rmask[6 * si + k] & w.rmask[6 * sj + k]) != 0;
+      if (share) {
+        // a sphere through u, v, w has radius >= r3; skip faces beyond every shared cell
+        const uint32_t vi = w.ridx[si], wi = w.ridx[sj];
+        const V3 a = gpt(w, vi), b = gpt(w, wi);
+        const double aa = dot(a, a), bb = dot(b, b), ab = dot(a, b);
+        const double D = aa * bb - ab * ab;
+        const V3 amb = sub(a, b);
+        const double r2 = D > 0 ? aa * bb * dot(amb, amb) / (4.0 * D) : INFINITY;
+        double rmax = 0;
+        for (int k = 0; k < 6; ++k) {
+          uint64_t mm = w.rmask[6 * si + k] & w.rmask[6 * sj + k];
+          while (mm) {
+            const int bit = __ffsll((long long)mm) - 1;
+            mm &= mm - 1;
+            rmax = fmax(rmax, w.rho[64 * k + bit]);
+          }
+        }
+        // prune only well-conditioned faces (FP64 r2 is then accurate to ~1e-10)
+        if (!(D > 1e-6 * (aa * bb + fabs(ab) * fabs(ab)) && r2 > rmax * rmax * (1.0 + 1e-6)))
+          process_face<!BIG>(P, w, u, M, vi, wi, rmax, &sregion, q);
       }
     }
-    // prune only well-conditioned faces (FP64 r2 is then accurate to ~1e-10)
-    if (D > 1e-6 * (aa * bb + fabs(ab) * fabs(ab)) && r2 > rmax * rmax * (1.0 + 1e-6)) continue;
-    process_face(P, w, u, M, vi, wi, rmax, &sregion);
+    if (BIG) continue;
+    wave_sync();
+    const uint32_t ns = min(q_n[wave][0], kSideQ);
+    for (uint32_t it = lane; it < ns; it += 64)
+      side_item(P, w, u, M, q_side[wave][it], q_side_r[wave][it], &sregion, q);
+    wave_sync();
+    const uint32_t nt = min(q_n[wave][1], kTetQ);
+    for (uint32_t it = lane; it < nt; it += 64) tet_item(P, w, u, M, q_tet[wave][it], &sregion);
+    wave_sync();
+    if (lane == 0) { q_n[wave][0] = 0; q_n[wave][1] = 0; }
+    wave_sync();
   }
   __syncthreads();
   if (t == 0) P.status[u] = ST_DONE;
@@ -713,7 +814,7 @@ __global__ void __launch_bounds__(kBlock) k_anchor_lds(Params P) {
     return;
   }
   Work w{gx, gy, gz, gid, rho, ridx, rmask, kidx, reinterpret_cast<float4 *>(rmask)};
-  anchor_work(P, w, u);
+  anchor_work<false>(P, w, u);
 }
 
 // Offset (in doubles, even) of the FP32 offsets in a global slab of gather
@@ -739,7 +840,7 @@ __global__ void __launch_bounds__(kBlock) k_anchor_big(Params P) {
   w.ridx = w.gid + G;
   w.kidx = w.ridx + G;
   w.fc = reinterpret_cast<float4 *>(slab + big_fc_offset(G));
-  anchor_work(P, w, u);
+  anchor_work<true>(P, w, u);
 }
 
 // Ball count / emit around every anchor of the list.
@@ -1071,6 +1172,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
   if (ovf) throw std::runtime_error("LBVH traversal stack overflow in the gather");
   if (ctr[0] > event_cap || ctr[1] > fb_cap || ctr[2] > reg_cap)
     throw std::runtime_error("catalogue queue capacity exceeded");
+  if (ctr[4]) throw std::runtime_error("face side queue overflow");
   // records stay on the device; only the host's share of the work moves
   (void)events;
   B.count = (int64_t)ctr[0];
