@@ -15,6 +15,9 @@
 //    with no prior reduced root creates a reduced birth (order k >= 2), one
 //    root grows silently, two or more roots create a multifusion.
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <exception>
 #include <thread>
@@ -247,6 +250,8 @@ struct Dsu {
 // Level-ordered contraction of one order: carrier DSU over the minima, each
 // class owning at most one reduced root.  Node ids are local to the order.
 void contract_order(int k, int64_t n, TowerOrderData &D, TowerState &F) {
+  const bool timing = std::getenv("MHGP_TIMING") != nullptr;
+  const auto t0 = std::chrono::steady_clock::now();
   // carrier DSU; each class may own a reduced root
   const int64_t M = k == 1 ? n : (int64_t)D.minima;
   std::vector<int32_t> owner(M, -1);  // forest node of a component root (-1: none)
@@ -336,6 +341,10 @@ void contract_order(int k, int64_t n, TowerOrderData &D, TowerState &F) {
     }
     i = j;
   }
+  if (timing)
+    std::fprintf(stderr, "[mhgp] contract order %2d: %u saddles, %u arms, %u minima, %zu nodes, %.1f ms\n", k,
+                 D.saddles, D.arms, D.minima, F.order.size(),
+                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   D = TowerOrderData{};  // release the order's arrays
 }
 

@@ -400,6 +400,58 @@ __global__ void k_saddle_labels(const EventRec *__restrict__ ev, const uint32_t 
   for (int q = 0; q < 4; ++q) sup[4 * i + q] = e.ids[q];
 }
 
+// Saddles that cannot change the contraction: every arm meets one minimum,
+// and an earlier saddle already touched that minimum (its component then has
+// a reduced root, and the saddle adds no connection).  first[m] is the least
+// saddle with an arm at minimum m.
+__global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_first_touch(const uint32_t *__restrict__ arm_saddle, const uint32_t *__restrict__ carrier,
+                              int64_t arms, int64_t minima, uint32_t *first, uint32_t *invalid) {
+  int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= arms) return;
+  const uint32_t c = carrier[a];
+  if (c == kInvalid || c >= minima) {
+    *invalid = 1u;
+    return;
+  }
+  atomicMin(&first[c], arm_saddle[a]);
+}
+
+__global__ void k_keep(const uint32_t *__restrict__ base, const uint32_t *__restrict__ incl,
+                       const uint32_t *__restrict__ carrier, const uint32_t *__restrict__ first, int64_t S,
+                       uint32_t *keep, uint32_t *kept_arms) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const uint32_t b = base[s], e = incl[s];
+  const uint32_t c0 = carrier[b];
+  bool single = true;
+  for (uint32_t a = b + 1; a < e; ++a) single &= carrier[a] == c0;
+  const bool k = !single || first[c0] == (uint32_t)s;
+  keep[s] = k ? 1u : 0u;
+  kept_arms[s] = k ? e - b : 0u;
+}
+
+__global__ void k_compact_saddles(int64_t S, const uint32_t *__restrict__ kpos, const uint32_t *__restrict__ kinc,
+                                  const uint32_t *__restrict__ base, const uint32_t *__restrict__ incl,
+                                  const uint32_t *__restrict__ carrier, const uint32_t *__restrict__ scls,
+                                  const double *__restrict__ slev, uint32_t *o_cls, double *o_lev, uint32_t *o_base,
+                                  uint32_t *o_car) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const uint32_t prev = s ? kpos[s - 1] : 0u;
+  if (kpos[s] == prev) return;  // dropped
+  const uint32_t b = base[s], e = incl[s];
+  const uint32_t p = prev, nb = kinc[s] - (e - b);
+  o_cls[p] = scls[s];
+  o_lev[p] = slev[s];
+  o_base[p] = nb;
+  for (uint32_t a = b; a < e; ++a) o_car[nb + (a - b)] = carrier[a];
+}
+
 }  // namespace
 
 void tower_order_device(Context &c, int k, TowerOrderData &out) {
@@ -492,28 +544,68 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   slev.ensure(S);
   hipLaunchKernelGGL(k_saddle_meta, dim3(blocks_for(S, 256)), dim3(256), 0, s, B.events.p, B.order.p, B.cls.p,
                      saddle_pos.p, (int64_t)S, scls.p, slev.p);
+  // drop the saddles that cannot change the contraction (unless the host
+  // must descend some arms: then every saddle goes down as it is)
+  const int64_t minima_n = k == 1 ? c.n : (int64_t)M;
+  DevBuf<uint32_t> first, keep, kept, flag1;
+  first.ensure(minima_n);
+  keep.ensure(S);
+  kept.ensure(S);
+  flag1.ensure(1);
+  MHGP_HIP(hipMemsetAsync(flag1.p, 0, 4, s));
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(minima_n, 256)), dim3(256), 0, s, first.p, minima_n, kInvalid);
+  hipLaunchKernelGGL(k_first_touch, dim3(blocks_for(arms, 256)), dim3(256), 0, s, arm_saddle.p, carrier.p,
+                     (int64_t)arms, minima_n, first.p, flag1.p);
+  uint32_t any_invalid = 0;
+  copy_wait(&any_invalid, flag1.p, 4, hipMemcpyDeviceToHost, s);
+  uint32_t S2 = S, arms2 = arms;
+  const uint32_t *d_cls = scls.p, *d_base = base.p, *d_car = carrier.p;
+  const double *d_lev = slev.p;
+  DevBuf<uint32_t> c_cls, c_base, c_car;
+  DevBuf<double> c_lev;
+  if (!any_invalid) {
+    hipLaunchKernelGGL(k_keep, dim3(blocks_for(S, 256)), dim3(256), 0, s, base.p, sizes.p, carrier.p, first.p,
+                       (int64_t)S, keep.p, kept.p);
+    c.scanner.inclusive(keep.p, S, s);
+    c.scanner.inclusive(kept.p, S, s);
+    MHGP_HIP(hipMemcpyAsync(&S2, keep.p + S - 1, 4, hipMemcpyDeviceToHost, s));
+    MHGP_HIP(hipMemcpyAsync(&arms2, kept.p + S - 1, 4, hipMemcpyDeviceToHost, s));
+    MHGP_HIP(hipStreamSynchronize(s));
+    c_cls.ensure(S2);
+    c_lev.ensure(S2);
+    c_base.ensure(S2);
+    c_car.ensure(arms2);
+    hipLaunchKernelGGL(k_compact_saddles, dim3(blocks_for(S, 256)), dim3(256), 0, s, (int64_t)S, keep.p, kept.p,
+                       base.p, sizes.p, carrier.p, scls.p, slev.p, c_cls.p, c_lev.p, c_base.p, c_car.p);
+    d_cls = c_cls.p;
+    d_lev = c_lev.p;
+    d_base = c_base.p;
+    d_car = c_car.p;
+  }
   // page-locked results: cls | arm_base | carrier | level
-  const size_t o_base = (size_t)S * 4, o_car = o_base + ((size_t)S + 1) * 4;
-  const size_t o_lev = (o_car + (size_t)arms * 4 + 7) & ~(size_t)7;
-  char *h = (char *)c.tower_pinned[k].ensure(o_lev + (size_t)S * 8);
+  const size_t o_base = (size_t)S2 * 4, o_car = o_base + ((size_t)S2 + 1) * 4;
+  const size_t o_lev = (o_car + (size_t)arms2 * 4 + 7) & ~(size_t)7;
+  char *h = (char *)c.tower_pinned[k].ensure(o_lev + (size_t)S2 * 8 + 8);
   uint32_t *h_cls = (uint32_t *)h, *h_base = (uint32_t *)(h + o_base), *h_car = (uint32_t *)(h + o_car);
   double *h_lev = (double *)(h + o_lev);
-  MHGP_HIP(hipMemcpyAsync(h_cls, scls.p, (size_t)S * 4, hipMemcpyDeviceToHost, s));
-  MHGP_HIP(hipMemcpyAsync(h_base, base.p, (size_t)S * 4, hipMemcpyDeviceToHost, s));
-  MHGP_HIP(hipMemcpyAsync(h_car, carrier.p, (size_t)arms * 4, hipMemcpyDeviceToHost, s));
-  MHGP_HIP(hipMemcpyAsync(h_lev, slev.p, (size_t)S * 8, hipMemcpyDeviceToHost, s));
+  if (S2) {
+    MHGP_HIP(hipMemcpyAsync(h_cls, d_cls, (size_t)S2 * 4, hipMemcpyDeviceToHost, s));
+    MHGP_HIP(hipMemcpyAsync(h_base, d_base, (size_t)S2 * 4, hipMemcpyDeviceToHost, s));
+    MHGP_HIP(hipMemcpyAsync(h_car, d_car, (size_t)arms2 * 4, hipMemcpyDeviceToHost, s));
+    MHGP_HIP(hipMemcpyAsync(h_lev, d_lev, (size_t)S2 * 8, hipMemcpyDeviceToHost, s));
+  }
   uint64_t st = 0;
   MHGP_HIP(hipMemcpyAsync(&st, steps.p, 8, hipMemcpyDeviceToHost, s));
   MHGP_HIP(hipStreamSynchronize(s));
-  h_base[S] = arms;
+  h_base[S2] = arms2;
+  out.saddles = S2;
   out.cls = h_cls;
   out.arm_base = h_base;
   out.carrier = h_car;
   out.level = h_lev;
-  out.arms = arms;
+  out.arms = arms2;
   out.steps = (int64_t)st;
-  bool need_labels = false;
-  for (uint32_t a = 0; a < arms; ++a) need_labels |= h_car[a] == kInvalid;
+  const bool need_labels = any_invalid != 0;  // then the saddles were not filtered
   if (need_labels) {
     // the host descends unresolved arms; it needs the saddle labels, their
     // supports and the minima labels
