@@ -153,6 +153,14 @@ int mhgp_supports(mhgp_ctx *ctx, int k_max, mhgp_catalog_info *info) {
   });
 }
 
+int mhgp_closed_ball(mhgp_ctx *ctx, const uint32_t *support_ids, const uint8_t *support_sizes, int64_t queries,
+                     uint32_t cap, uint32_t *interior_ids, uint32_t *interior_count, uint32_t *shell_count) {
+  return guarded(ctx ? &ctx->c : nullptr, [&] {
+    mhgp::closed_ball(ctx->c, support_ids, support_sizes, queries, cap, interior_ids, interior_count, shell_count);
+    return MHGP_OK;
+  });
+}
+
 int mhgp_supports_fetch(mhgp_ctx *ctx, uint32_t *ids, uint32_t *meta, uint32_t *interior,
                         double *level, int64_t *level_class) {
   return guarded(ctx ? &ctx->c : nullptr, [&] {

@@ -33,6 +33,10 @@ void upload_device(Context &c, const double *d_xyz, int64_t n);
 void require_cloud(const Context &c);
 void canonical_points(Context &c, double *xyz_out, uint32_t *source_index_out);
 
+// ball.hip
+void closed_ball(Context &c, const uint32_t *ids, const uint8_t *sizes, int64_t Q, uint32_t cap,
+                 uint32_t *interior, uint32_t *nin, uint32_t *nshell);
+
 // knn.hip
 void knn_all(Context &c, int k, uint32_t *d_out, uint8_t *d_flags);
 // k nearest neighbours to host memory, near ties settled exactly on the host

@@ -80,6 +80,7 @@ def lib() -> ctypes.CDLL:
         L.mhgp_tower.argtypes = [p, i32, ctypes.POINTER(TowerInfo)]
         L.mhgp_tower_fetch.argtypes = [p, p, p, p, p, p, p]
         L.mhgp_ctx_stats.argtypes = [p, ctypes.POINTER(Stats), i32]
+        L.mhgp_closed_ball.argtypes = [p, p, p, i64, ctypes.c_uint32, p, p, p]
         L.mhgp_server_create.argtypes = [i32, i32, i64, ctypes.POINTER(p)]
         L.mhgp_server_submit.argtypes = [p, p, i64, i32, ctypes.POINTER(i64)]
         L.mhgp_server_wait.argtypes = [p, i64, ctypes.POINTER(RequestResult)]
@@ -96,7 +97,7 @@ EXPORTED = [
     "mhgp_cloud_upload", "mhgp_cloud_upload_device", "mhgp_point_count",
     "mhgp_canonical_points", "mhgp_lbvh_snapshot", "mhgp_knn", "mhgp_emst", "mhgp_supports",
     "mhgp_supports_fetch", "mhgp_tower", "mhgp_tower_fetch", "mhgp_ctx_stats",
-    "mhgp_server_create", "mhgp_server_submit", "mhgp_server_wait", "mhgp_server_fetch",
+    "mhgp_closed_ball", "mhgp_server_create", "mhgp_server_submit", "mhgp_server_wait", "mhgp_server_fetch",
     "mhgp_server_release", "mhgp_server_destroy",
 ]
 
@@ -190,6 +191,22 @@ class Context:
         out = np.zeros((max(self.n - 1, 0), 2), dtype=np.uint32)
         _check(lib().mhgp_emst(self._h, _ptr(out)))
         return out
+
+    def closed_ball(self, supports, cap: int = 16):
+        """Interior ids (complete when the count <= cap), interior counts and
+        shell counts of the circumspheres of the given supports (lists of 1..4
+        canonical PointIds)."""
+        q = len(supports)
+        ids = np.full((max(q, 1), 4), 0xFFFFFFFF, dtype=np.uint32)
+        sizes = np.zeros(max(q, 1), dtype=np.uint8)
+        for i, sup in enumerate(supports):
+            ids[i, : len(sup)] = sup
+            sizes[i] = len(sup)
+        inter = np.zeros((max(q, 1), max(cap, 1)), dtype=np.uint32)
+        nin = np.zeros(max(q, 1), dtype=np.uint32)
+        nsh = np.zeros(max(q, 1), dtype=np.uint32)
+        _check(lib().mhgp_closed_ball(self._h, _ptr(ids), _ptr(sizes), q, cap, _ptr(inter), _ptr(nin), _ptr(nsh)))
+        return inter[:q, :cap], nin[:q], nsh[:q]
 
     def supports(self, k_max: int, fetch: bool = True):
         info = CatalogInfo()

@@ -86,6 +86,20 @@ int mhgp_knn(mhgp_ctx *ctx, int k, uint32_t *ids_out);
  * Borůvka contraction (morsehgp3d/src/cpu/hierarchy/boruvka.cpp:893). */
 int mhgp_emst(mhgp_ctx *ctx, uint32_t *edges_out);
 
+/* Closed-ball partition of the cloud against the circumspheres of `queries`
+ * supports (reference lbvh_closed_ball, morsehgp3d/src/cpu/spatial/
+ * lbvh.cpp:1210, and the sparse classification of the support streams,
+ * pair_support_stream.cpp:1781-1952).  support_ids: 4 canonical PointIds per
+ * query, the first support_sizes[q] (1..4, affinely independent) used; the
+ * sphere is their circumsphere with centre in their affine hull.
+ * interior_count[q]: points strictly inside; shell_count[q]: points on the
+ * sphere, the support included; exterior = n - interior - shell.
+ * interior_ids: cap per query, ascending, UINT32_MAX padded; complete when
+ * interior_count[q] <= cap.  Exact: undecided points are settled on the host. */
+int mhgp_closed_ball(mhgp_ctx *ctx, const uint32_t *support_ids, const uint8_t *support_sizes,
+                     int64_t queries, uint32_t cap, uint32_t *interior_ids, uint32_t *interior_count,
+                     uint32_t *shell_count);
+
 /* Critical-sphere catalogue for orders 1..k_max (s_max = min(min(k_max,n)+1, n)):
  * every support of 2..4 points whose circumcentre lies strictly inside its
  * convex hull, whose sphere holds no further point and whose closed rank is
