@@ -236,49 +236,53 @@ class Descender {
   Grid grid_;
 };
 
-struct Dsu {
-  std::vector<uint32_t> p;
-  explicit Dsu(int64_t n) : p(n) { std::iota(p.begin(), p.end(), 0u); }
-  uint32_t find(uint32_t x) {
-    while (p[x] != x) { p[x] = p[p[x]]; x = p[x]; }
-    return x;
-  }
-};
 
 }  // namespace
 
 // Level-ordered contraction of one order: carrier DSU over the minima, each
 // class owning at most one reduced root.  Node ids are local to the order.
+// Union-find cell of one minimum: parent and the forest node owned by the
+// component when the cell is a root (-1: none), in one 8-byte word so that a
+// find and the owner lookup after it share the cache line.
+struct Cell {
+  uint32_t parent;
+  int32_t owner;
+};
+
+inline uint32_t cell_find(std::vector<Cell> &c, uint32_t x) {
+  while (c[x].parent != x) {
+    c[x].parent = c[c[x].parent].parent;  // path halving
+    x = c[x].parent;
+  }
+  return x;
+}
+
 void contract_order(int k, int64_t n, TowerOrderData &D, TowerState &F) {
   const bool timing = std::getenv("MHGP_TIMING") != nullptr;
   const auto t0 = std::chrono::steady_clock::now();
-  // carrier DSU; each class may own a reduced root
+  // carrier union-find; each class may own a reduced root
   const int64_t M = k == 1 ? n : (int64_t)D.minima;
-  std::vector<int32_t> owner(M, -1);  // forest node of a component root (-1: none)
+  std::vector<Cell> cell(M);
+  for (int64_t x = 0; x < M; ++x) cell[x] = {(uint32_t)x, -1};
   F.reserve(k == 1 ? (size_t)n + D.saddles / 8 : D.saddles / 8);
   if (k == 1)
-    for (int64_t p = 0; p < n; ++p) owner[p] = F.add(1, 0, -1, 0.0);
-  Dsu dsu(M);
-  std::vector<int64_t> touched, bp_parent;
+    for (int64_t p = 0; p < n; ++p) cell[p].owner = (int32_t)F.add(1, 0, -1, 0.0);
   std::unordered_map<int64_t, int64_t> bp;
   for (uint32_t i = 0; i < D.saddles;) {
     uint32_t j = i;
     while (j < D.saddles && D.cls[j] == D.cls[i]) j++;
-    // the DSU walk is latency bound: touch the carriers a few saddles ahead
+    // the walk is latency bound: touch the carriers a few saddles ahead
     if (i + 8 < D.saddles)
-      for (uint32_t a = D.arm_base[i + 8]; a < D.arm_base[i + 9]; ++a) {
-        __builtin_prefetch(&dsu.p[D.carrier[a]]);
-        __builtin_prefetch(&owner[D.carrier[a]]);
-      }
+      for (uint32_t a = D.arm_base[i + 8]; a < D.arm_base[i + 9]; ++a) __builtin_prefetch(&cell[D.carrier[a]]);
     if (j == i + 1) {
       // single saddle at this level: distinct carrier roots are the group
-      int64_t members[4];
+      uint32_t members[4];
       int nm = 0;
       uint32_t last = kInvalid;
       for (uint32_t a = D.arm_base[i]; a < D.arm_base[i + 1]; ++a) {
         if (D.carrier[a] == last) continue;  // same carrier, same root
         last = D.carrier[a];
-        int64_t r = dsu.find(last);
+        const uint32_t r = cell_find(cell, last);
         bool seen = false;
         for (int q = 0; q < nm; ++q) seen |= members[q] == r;
         if (!seen) members[nm++] = r;
@@ -287,18 +291,18 @@ void contract_order(int k, int64_t n, TowerOrderData &D, TowerState &F) {
         int64_t roots[4];
         int nr = 0;
         for (int q = 0; q < nm; ++q)
-          if (owner[members[q]] >= 0) roots[nr++] = owner[members[q]];
+          if (cell[members[q]].owner >= 0) roots[nr++] = cell[members[q]].owner;
         std::sort(roots, roots + nr);
         nr = (int)(std::unique(roots, roots + nr) - roots);
         int64_t node;
         if (nr == 0) node = F.add(k, 1, D.cls[i], D.level[i]);
         else if (nr == 1) node = roots[0];
         else node = F.add(k, 2, D.cls[i], D.level[i], roots, nr);
-        for (int q = 1; q < nm; ++q) dsu.p[members[q]] = members[0];
-        owner[members[0]] = node;
-      } else if (owner[members[0]] < 0) {
+        for (int q = 1; q < nm; ++q) cell[members[q]].parent = members[0];
+        cell[members[0]].owner = (int32_t)node;
+      } else if (cell[members[0]].owner < 0) {
         // all arms in one latent minimum cannot happen in general position
-        owner[members[0]] = F.add(k, 1, D.cls[i], D.level[i]);
+        cell[members[0]].owner = (int32_t)F.add(k, 1, D.cls[i], D.level[i]);
       }
       i = j;
       continue;
@@ -312,7 +316,7 @@ void contract_order(int k, int64_t n, TowerOrderData &D, TowerState &F) {
     for (uint32_t e = i; e < j; ++e) {
       int64_t first = -1;
       for (uint32_t a = D.arm_base[e]; a < D.arm_base[e + 1]; ++a) {
-        int64_t r = dsu.find(D.carrier[a]);
+        const int64_t r = cell_find(cell, D.carrier[a]);
         if (!bp.count(r)) bp[r] = r;
         if (first < 0) first = r;
         else { int64_t x = bfind(first), y = bfind(r); if (x != y) bp[y] = x; }
@@ -329,15 +333,16 @@ void contract_order(int k, int64_t n, TowerOrderData &D, TowerState &F) {
     for (auto &members : glist) {
       std::vector<int64_t> roots;
       for (int64_t r : members)
-        if (owner[r] >= 0) roots.push_back(owner[r]);
+        if (cell[r].owner >= 0) roots.push_back(cell[r].owner);
       std::sort(roots.begin(), roots.end());
       roots.erase(std::unique(roots.begin(), roots.end()), roots.end());
       int64_t node;
       if (roots.empty()) node = F.add(k, 1, D.cls[i], D.level[i]);
       else if (roots.size() == 1) node = roots[0];
       else node = F.add(k, 2, D.cls[i], D.level[i], roots);
-      for (size_t q = 1; q < members.size(); ++q) dsu.p[dsu.find(members[q])] = dsu.find(members[0]);
-      owner[dsu.find(members[0])] = node;
+      for (size_t q = 1; q < members.size(); ++q)
+        cell[cell_find(cell, (uint32_t)members[q])].parent = cell_find(cell, (uint32_t)members[0]);
+      cell[cell_find(cell, (uint32_t)members[0])].owner = (int32_t)node;
     }
     i = j;
   }
