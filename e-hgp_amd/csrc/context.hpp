@@ -15,6 +15,7 @@ struct CatalogBuffers;   // supports.hpp
 struct EmstBuffers;      // emst.hpp
 struct HostMirror;       // pipeline.cpp
 struct TowerState;       // tower.cpp
+struct TowerWork;        // tower_gpu.hpp
 
 struct Context {
   int device = 0;
@@ -71,6 +72,7 @@ struct Context {
   std::unique_ptr<EmstBuffers> emst;
   std::unique_ptr<HostMirror> host;
   std::unique_ptr<TowerState> tower;
+  std::unique_ptr<TowerWork> tower_work;
   PinnedBuf tower_pinned[11];  // per tower order: saddle results on the host
 
   Context();

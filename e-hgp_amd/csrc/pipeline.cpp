@@ -7,6 +7,7 @@
 
 #include "stages.hpp"
 #include "supports.hpp"
+#include "tower_gpu.hpp"
 #include "tower_state.hpp"
 
 namespace mhgp {
@@ -20,6 +21,7 @@ int64_t device_bytes(const Context &c) {
               c.LX.bytes() * 3 + c.path.bytes() + c.nodes.bytes() + c.parent_int.bytes() +
               c.parent_leaf.bytes() + c.wit.bytes() + c.counter.bytes();
   if (c.catalog) b += catalog_device_bytes(*c.catalog);
+  if (c.tower_work) b += c.tower_work->bytes();
   return b;
 }
 

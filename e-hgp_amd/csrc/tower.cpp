@@ -268,6 +268,7 @@ void contract_order(int k, int64_t n, TowerOrderData &D, TowerState &F) {
   if (k == 1)
     for (int64_t p = 0; p < n; ++p) cell[p].owner = (int32_t)F.add(1, 0, -1, 0.0);
   std::unordered_map<int64_t, int64_t> bp;
+  std::unordered_map<int64_t, uint32_t> efirst;
   for (uint32_t i = 0; i < D.saddles;) {
     uint32_t j = i;
     while (j < D.saddles && D.cls[j] == D.cls[i]) j++;
@@ -307,8 +308,10 @@ void contract_order(int k, int64_t n, TowerOrderData &D, TowerState &F) {
       i = j;
       continue;
     }
-    // several saddles at one exact level: contract them together
+    // several saddles at one exact level: contract them together; the groups
+    // are emitted in the order of their first saddle
     bp.clear();
+    efirst.clear();
     auto bfind = [&](int64_t x) {
       while (bp[x] != x) { bp[x] = bp[bp[x]]; x = bp[x]; }
       return x;
@@ -317,20 +320,25 @@ void contract_order(int k, int64_t n, TowerOrderData &D, TowerState &F) {
       int64_t first = -1;
       for (uint32_t a = D.arm_base[e]; a < D.arm_base[e + 1]; ++a) {
         const int64_t r = cell_find(cell, D.carrier[a]);
-        if (!bp.count(r)) bp[r] = r;
+        if (!bp.count(r)) { bp[r] = r; efirst[r] = e; }
         if (first < 0) first = r;
         else { int64_t x = bfind(first), y = bfind(r); if (x != y) bp[y] = x; }
       }
     }
-    std::unordered_map<int64_t, std::vector<int64_t>> groups;
-    for (auto &kv : bp) groups[bfind(kv.first)].push_back(kv.first);
-    std::vector<std::vector<int64_t>> glist;
-    for (auto &kv : groups) {
-      std::sort(kv.second.begin(), kv.second.end());
-      glist.push_back(kv.second);
+    std::unordered_map<int64_t, std::pair<uint32_t, std::vector<int64_t>>> groups;
+    for (auto &kv : bp) {
+      auto &g = groups.try_emplace(bfind(kv.first), UINT32_MAX, std::vector<int64_t>{}).first->second;
+      g.first = std::min(g.first, efirst[kv.first]);
+      g.second.push_back(kv.first);
     }
-    std::sort(glist.begin(), glist.end());
-    for (auto &members : glist) {
+    std::vector<std::pair<uint32_t, std::vector<int64_t>>> glist;
+    for (auto &kv : groups) {
+      std::sort(kv.second.second.begin(), kv.second.second.end());
+      glist.push_back(std::move(kv.second));
+    }
+    std::sort(glist.begin(), glist.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+    for (auto &entry : glist) {
+      const std::vector<int64_t> &members = entry.second;
       std::vector<int64_t> roots;
       for (int64_t r : members)
         if (cell[r].owner >= 0) roots.push_back(cell[r].owner);

@@ -400,13 +400,32 @@ __global__ void k_saddle_labels(const EventRec *__restrict__ ev, const uint32_t 
   for (int q = 0; q < 4; ++q) sup[4 * i + q] = e.ids[q];
 }
 
-// Saddles that cannot change the contraction: every arm meets one minimum,
-// and an earlier saddle already touched that minimum (its component then has
-// a reduced root, and the saddle adds no connection).  first[m] is the least
-// saddle with an arm at minimum m.
+// Reduction of an order's saddles before the level-ordered contraction.
+//
+// Every saddle s contributes star edges (its first arm's carrier to each other
+// arm's carrier), keyed by arm index, so edge order is level order.  A
+// Boruvka minimum spanning forest over these edges tells, for each saddle, how
+// many distinct components its carriers join: the saddle's forest edges.
+// first[m] is the least saddle with an arm at minimum m; a carrier with
+// first[m] == s is fresh (an untouched one-minimum component), any other
+// carrier lies in a component that already has a reduced root.  For a saddle
+// whose exact level no other saddle shares:
+//  * all carriers fresh: a reduced birth (kept);
+//  * fresh carriers f and owned carriers spanning D = edges - f + 1 distinct
+//    components: D >= 2 is a multifusion (kept); D == 1 changes no root, the
+//    fresh minima join the owned component silently (the saddle is dropped
+//    and the fresh minima are absorbed into an owned carrier).
+// Saddles of a shared level are kept as they are (their group is contracted
+// together on the host).  Order one has no fresh minima: every point is born
+// at level zero.
 __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+__global__ void k_identity(uint32_t *p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (uint32_t)i;
 }
 
 __global__ void k_first_touch(const uint32_t *__restrict__ arm_saddle, const uint32_t *__restrict__ carrier,
@@ -421,24 +440,179 @@ __global__ void k_first_touch(const uint32_t *__restrict__ arm_saddle, const uin
   atomicMin(&first[c], arm_saddle[a]);
 }
 
-__global__ void k_keep(const uint32_t *__restrict__ base, const uint32_t *__restrict__ incl,
-                       const uint32_t *__restrict__ carrier, const uint32_t *__restrict__ first, int64_t S,
-                       uint32_t *keep, uint32_t *kept_arms) {
+__global__ void k_edge_flags(const uint32_t *__restrict__ arm_saddle, const uint32_t *__restrict__ base,
+                             const uint32_t *__restrict__ carrier, int64_t arms, uint32_t *flag) {
+  int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= arms) return;
+  const uint32_t b = base[arm_saddle[a]], c = carrier[a];
+  bool dup = false;  // the hub, or a carrier an earlier arm already joins
+  for (uint32_t x = b; x < (uint32_t)a; ++x) dup |= carrier[x] == c;
+  flag[a] = ((uint32_t)a != b && !dup) ? 1u : 0u;
+}
+
+__global__ void k_edge_list(const uint32_t *__restrict__ arm_saddle, const uint32_t *__restrict__ base,
+                            const uint32_t *__restrict__ carrier, const uint32_t *__restrict__ scan, int64_t arms,
+                            uint32_t *eu, uint32_t *ev, uint32_t *es) {
+  int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= arms) return;
+  const uint32_t prev = a ? scan[a - 1] : 0u;
+  if (scan[a] == prev) return;
+  const uint32_t s = arm_saddle[a];
+  eu[prev] = carrier[base[s]];
+  ev[prev] = carrier[a];
+  es[prev] = s;
+}
+
+// Boruvka round, part 1: least edge (by index) leaving each component.  The
+// edges still joining two components are appended to the next round's list
+// (li == nullptr: the list is the full edge list, li[i] = i).
+__global__ void k_msf_best(int64_t L, const uint32_t *__restrict__ lu, const uint32_t *__restrict__ lv,
+                           const uint32_t *__restrict__ li, const uint32_t *__restrict__ comp, uint32_t *best,
+                           uint32_t *ou, uint32_t *ov, uint32_t *oi, uint32_t *count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t u = 0, v = 0, idx = 0, cu = 0, cv = 0;
+  if (i < L) {
+    u = lu[i];
+    v = lv[i];
+    idx = li ? li[i] : (uint32_t)i;
+    cu = comp[u];
+    cv = comp[v];
+  }
+  const bool live = i < L && cu != cv;
+  if (live) {
+    atomicMin(&best[cu], idx);
+    atomicMin(&best[cv], idx);
+  }
+  // wave-aggregated append
+  const uint64_t mask = __ballot(live);
+  if (!mask) return;
+  const int lane = (int)(threadIdx.x & 63u);
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t at = 0;
+  if (lane == leader) at = atomicAdd(count, (uint32_t)__popcll(mask));
+  at = __shfl(at, leader);
+  if (live) {
+    const uint32_t pos = at + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    ou[pos] = u;
+    ov[pos] = v;
+    oi[pos] = idx;
+  }
+}
+
+// Part 2: every component root hooks along its least edge, which is a forest
+// edge; of a pair choosing the same edge the smaller root stays a root.
+// comp is only read here; the hooks are applied by the next kernel.
+__global__ void k_msf_hook(int64_t M, const uint32_t *__restrict__ eu, const uint32_t *__restrict__ ev,
+                           const uint32_t *__restrict__ comp, const uint32_t *__restrict__ best, uint32_t *hook,
+                           uint32_t *msf, uint32_t *changed) {
+  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M || comp[m] != (uint32_t)m) return;
+  const uint32_t i = best[m];
+  if (i == kInvalid) return;
+  const uint32_t cu = comp[eu[i]], cv = comp[ev[i]];
+  const uint32_t other = cu == (uint32_t)m ? cv : cu;
+  msf[i] = 1u;
+  if (best[other] == i && (uint32_t)m < other) return;
+  hook[m] = other;
+  *changed = 1u;
+}
+
+__global__ void k_msf_apply(int64_t M, uint32_t *comp, uint32_t *best, uint32_t *hook) {
+  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M || comp[m] != (uint32_t)m) return;
+  if (hook[m] != kInvalid) comp[m] = hook[m];
+  hook[m] = kInvalid;
+  best[m] = kInvalid;
+}
+
+// Root of m in a forest of parent pointers (a root points to itself, or
+// holds kInvalid when nil_root); path halving.  Concurrent threads only ever
+// store an ancestor of a node into it, so the forest keeps its roots; the
+// roots themselves go to another array.  The walk is bounded by the node
+// count: a longer one (a cycle, which a correct forest cannot hold) raises
+// *err and returns kInvalid.
+__device__ inline uint32_t forest_root(uint32_t *parent, uint32_t m, bool nil_root, int64_t nodes, uint32_t *err) {
+  uint32_t x = m;
+  for (int64_t step = 0;; ++step) {
+    if (step > nodes) {
+      *err = 1u;
+      return kInvalid;
+    }
+    const uint32_t p = parent[x];
+    if (nil_root ? p == kInvalid : p == x) break;
+    const uint32_t g = parent[p];
+    if (nil_root ? g == kInvalid : g == p) { x = p; break; }
+    parent[x] = g;
+    x = g;
+  }
+  return x;
+}
+
+// Roots of the hook forest into the next round's component array.
+__global__ void k_msf_flatten(int64_t M, uint32_t *comp, uint32_t *next, uint32_t *err) {
+  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  next[m] = forest_root(comp, (uint32_t)m, false, M, err);
+}
+
+__global__ void k_msf_count(int64_t E, const uint32_t *__restrict__ msf, const uint32_t *__restrict__ es,
+                            uint32_t *mcount) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < E && msf[i]) atomicAdd(&mcount[es[i]], 1u);
+}
+
+__global__ void k_reduce_saddles(int64_t S, const uint32_t *__restrict__ base, const uint32_t *__restrict__ incl,
+                                 const uint32_t *__restrict__ carrier, const uint32_t *__restrict__ cls,
+                                 const uint32_t *__restrict__ first, const uint32_t *__restrict__ mcount,
+                                 int all_owned, uint32_t *keep, uint32_t *kept_arms, uint32_t *absorb) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
   const uint32_t b = base[s], e = incl[s];
-  const uint32_t c0 = carrier[b];
-  bool single = true;
-  for (uint32_t a = b + 1; a < e; ++a) single &= carrier[a] == c0;
-  const bool k = !single || first[c0] == (uint32_t)s;
+  const uint32_t c = cls[s];
+  const bool shared = (s > 0 && cls[s - 1] == c) || (s + 1 < S && cls[s + 1] == c);
+  bool k = true;
+  if (!shared) {
+    uint32_t seen[4];
+    int ns = 0, fresh = 0, owned_n = 0;
+    uint32_t owned = kInvalid;
+    for (uint32_t a = b; a < e && ns < 4; ++a) {
+      const uint32_t x = carrier[a];
+      bool dup = false;
+      for (int q = 0; q < ns; ++q) dup |= seen[q] == x;
+      if (dup) continue;
+      seen[ns++] = x;
+      if (!all_owned && first[x] == (uint32_t)s) fresh++;
+      else { owned_n++; if (owned == kInvalid) owned = x; }
+    }
+    if (owned_n > 0 && (int)mcount[s] + 1 - fresh < 2) {
+      k = false;
+      for (int q = 0; q < ns; ++q)
+        if (seen[q] != owned && !all_owned && first[seen[q]] == (uint32_t)s) absorb[seen[q]] = owned;
+    }
+  }
   keep[s] = k ? 1u : 0u;
   kept_arms[s] = k ? e - b : 0u;
+}
+
+__global__ void k_absorb_roots(int64_t M, uint32_t *absorb, uint32_t *rep, uint32_t *err) {
+  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < M) rep[m] = forest_root(absorb, (uint32_t)m, true, M, err);
+}
+
+// Minima met by a kept saddle get dense ids (flag, then an inclusive scan).
+__global__ void k_flag_used(int64_t S, const uint32_t *__restrict__ keep, const uint32_t *__restrict__ base,
+                            const uint32_t *__restrict__ incl, const uint32_t *__restrict__ carrier,
+                            const uint32_t *__restrict__ rep, uint32_t *used) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S || !keep[s]) return;
+  for (uint32_t a = base[s]; a < incl[s]; ++a) used[rep[carrier[a]]] = 1u;
 }
 
 __global__ void k_compact_saddles(int64_t S, const uint32_t *__restrict__ kpos, const uint32_t *__restrict__ kinc,
                                   const uint32_t *__restrict__ base, const uint32_t *__restrict__ incl,
                                   const uint32_t *__restrict__ carrier, const uint32_t *__restrict__ scls,
-                                  const double *__restrict__ slev, uint32_t *o_cls, double *o_lev, uint32_t *o_base,
+                                  const double *__restrict__ slev, const uint32_t *__restrict__ rep,
+                                  const uint32_t *__restrict__ dense, uint32_t *o_cls, double *o_lev, uint32_t *o_base,
                                   uint32_t *o_car) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
@@ -449,7 +623,7 @@ __global__ void k_compact_saddles(int64_t S, const uint32_t *__restrict__ kpos, 
   o_cls[p] = scls[s];
   o_lev[p] = slev[s];
   o_base[p] = nb;
-  for (uint32_t a = b; a < e; ++a) o_car[nb + (a - b)] = carrier[a];
+  for (uint32_t a = b; a < e; ++a) o_car[nb + (a - b)] = rep ? dense[rep[carrier[a]]] - 1u : carrier[a];
 }
 
 }  // namespace
@@ -459,7 +633,9 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   hipStream_t s = c.stream;
   const int64_t n = B.count;
   out = TowerOrderData{};
-  DevBuf<uint32_t> flag, minima_pos, saddle_pos;
+  if (!c.tower_work) c.tower_work = std::make_unique<TowerWork>();
+  TowerWork &W = *c.tower_work;
+  auto &flag = W.flag, &minima_pos = W.minima_pos, &saddle_pos = W.saddle_pos;
   flag.ensure(n);
   // minima of order k (k >= 2)
   hipLaunchKernelGGL(k_flag_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.events.p, B.order.p, n, k, flag.p);
@@ -471,8 +647,8 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   out.minima = M;
   uint64_t cap = 1;
   while (cap < 2ull * M + 16) cap <<= 1;
-  DevBuf<uint64_t> tkeys;
-  DevBuf<uint32_t> tvals, labels;
+  auto &tkeys = W.tkeys;
+  auto &tvals = W.tvals, &labels = W.labels;
   tkeys.ensure(cap);
   tvals.ensure(cap);
   labels.ensure((size_t)M * k + 1);
@@ -490,9 +666,9 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   hipLaunchKernelGGL(k_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.order.p, flag.p, n, saddle_pos.p);
   out.saddles = S;
   if (S == 0) return;
-  DevBuf<uint32_t> sizes, carrier, scls;
-  DevBuf<double> slev;
-  DevBuf<uint64_t> steps;
+  auto &sizes = W.sizes, &carrier = W.carrier, &scls = W.scls;
+  auto &slev = W.slev;
+  auto &steps = W.steps;
   sizes.ensure(S);
   hipLaunchKernelGGL(k_arm_sizes, dim3(blocks_for(S, 256)), dim3(256), 0, s, B.events.p, B.order.p,
                      saddle_pos.p, (int64_t)S, sizes.p);
@@ -500,12 +676,12 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   uint32_t arms = 0;
   copy_wait(&arms, sizes.p + S - 1, 4, hipMemcpyDeviceToHost, s);
   // exclusive bases: shift the inclusive scan by one
-  DevBuf<uint32_t> base;
+  auto &base = W.base;
   base.ensure(S);
   MHGP_HIP(hipMemsetAsync(base.p, 0, 4, s));
   if (S > 1) MHGP_HIP(hipMemcpyAsync(base.p + 1, sizes.p, (S - 1) * 4, hipMemcpyDeviceToDevice, s));
   carrier.ensure(arms);
-  DevBuf<uint32_t> arm_saddle;
+  auto &arm_saddle = W.arm_saddle;
   arm_saddle.ensure(arms);
   hipLaunchKernelGGL(k_arm_owner, dim3(blocks_for(S, 256)), dim3(256), 0, s, base.p, sizes.p, (int64_t)S,
                      arm_saddle.p);
@@ -519,8 +695,8 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   A.nodes = c.nodes.p; A.LX = c.LX.p; A.LY = c.LY.p; A.LZ = c.LZ.p; A.leaf_id = c.leaf_id.p; A.n = c.n;
   A.carrier = carrier.p;
   A.steps = (unsigned long long *)steps.p;
-  DevBuf<uint32_t> queue;
-  DevBuf<uint64_t> qcount;
+  auto &queue = W.queue;
+  auto &qcount = W.qcount;
   queue.ensure(arms);
   qcount.ensure(1);
   MHGP_HIP(hipMemsetAsync(qcount.p, 0, 8, s));
@@ -547,7 +723,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   // drop the saddles that cannot change the contraction (unless the host
   // must descend some arms: then every saddle goes down as it is)
   const int64_t minima_n = k == 1 ? c.n : (int64_t)M;
-  DevBuf<uint32_t> first, keep, kept, flag1;
+  auto &first = W.first, &keep = W.keep, &kept = W.kept, &flag1 = W.flag1;
   first.ensure(minima_n);
   keep.ensure(S);
   kept.ensure(S);
@@ -561,11 +737,97 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   uint32_t S2 = S, arms2 = arms;
   const uint32_t *d_cls = scls.p, *d_base = base.p, *d_car = carrier.p;
   const double *d_lev = slev.p;
-  DevBuf<uint32_t> c_cls, c_base, c_car;
-  DevBuf<double> c_lev;
+  auto &c_cls = W.c_cls, &c_base = W.c_base, &c_car = W.c_car;
+  auto &c_lev = W.c_lev;
   if (!any_invalid) {
-    hipLaunchKernelGGL(k_keep, dim3(blocks_for(S, 256)), dim3(256), 0, s, base.p, sizes.p, carrier.p, first.p,
-                       (int64_t)S, keep.p, kept.p);
+    const dim3 tb(256);
+    // star edges of the saddles, in arm (= level) order
+    auto &eflag = W.eflag, &eu = W.eu, &ev = W.ev, &es = W.es;
+    eflag.ensure(arms);
+    hipLaunchKernelGGL(k_edge_flags, dim3(blocks_for(arms, 256)), tb, 0, s, arm_saddle.p, base.p, carrier.p,
+                       (int64_t)arms, eflag.p);
+    c.scanner.inclusive(eflag.p, arms, s);
+    uint32_t E = 0;
+    copy_wait(&E, eflag.p + arms - 1, 4, hipMemcpyDeviceToHost, s);
+    eu.ensure(E);
+    ev.ensure(E);
+    es.ensure(E);
+    hipLaunchKernelGGL(k_edge_list, dim3(blocks_for(arms, 256)), tb, 0, s, arm_saddle.p, base.p, carrier.p, eflag.p,
+                       (int64_t)arms, eu.p, ev.p, es.p);
+    // Boruvka spanning forest of the minima over the star edges
+    auto &comp = W.comp, &comp2 = W.comp2, &best = W.best, &hook = W.hook, &msf = W.msf;
+    auto &mcount = W.mcount, &changed = W.changed, &err = W.err, &live_n = W.live_n;
+    auto &Lu = W.Lu, &Lv = W.Lv, &Li = W.Li;
+    for (int q = 0; q < 2; ++q) {
+      Lu[q].ensure(E);
+      Lv[q].ensure(E);
+      Li[q].ensure(E);
+    }
+    live_n.ensure(1);
+    comp.ensure(minima_n);
+    comp2.ensure(minima_n);
+    err.ensure(1);
+    best.ensure(minima_n);
+    hook.ensure(minima_n);
+    msf.ensure(E + 1);
+    mcount.ensure(S);
+    changed.ensure(1);
+    const dim3 gm(blocks_for(minima_n, 256)), ge(blocks_for(E + 1, 256));
+    hipLaunchKernelGGL(k_identity, gm, tb, 0, s, comp.p, minima_n);
+    hipLaunchKernelGGL(k_fill_u32, gm, tb, 0, s, best.p, minima_n, kInvalid);
+    hipLaunchKernelGGL(k_fill_u32, gm, tb, 0, s, hook.p, minima_n, kInvalid);
+    hipLaunchKernelGGL(k_fill_u32, ge, tb, 0, s, msf.p, (int64_t)E, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(S, 256)), tb, 0, s, mcount.p, (int64_t)S, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), tb, 0, s, err.p, (int64_t)1, 0u);
+    const uint32_t *lu = eu.p, *lv = ev.p, *li = nullptr;
+    uint32_t live = E;
+    for (int round = 0; live > 0; ++round) {
+      if (round > 64) throw std::runtime_error("spanning forest of the tower did not converge");
+      const int q = round & 1;
+      hipLaunchKernelGGL(k_fill_u32, dim3(1), tb, 0, s, changed.p, (int64_t)1, 0u);
+      hipLaunchKernelGGL(k_fill_u32, dim3(1), tb, 0, s, live_n.p, (int64_t)1, 0u);
+      hipLaunchKernelGGL(k_msf_best, dim3(blocks_for(live, 256)), tb, 0, s, (int64_t)live, lu, lv, li, comp.p,
+                         best.p, Lu[q].p, Lv[q].p, Li[q].p, live_n.p);
+      hipLaunchKernelGGL(k_msf_hook, gm, tb, 0, s, minima_n, eu.p, ev.p, comp.p, best.p, hook.p, msf.p, changed.p);
+      hipLaunchKernelGGL(k_msf_apply, gm, tb, 0, s, minima_n, comp.p, best.p, hook.p);
+      hipLaunchKernelGGL(k_msf_flatten, gm, tb, 0, s, minima_n, comp.p, comp2.p, err.p);
+      std::swap(comp.p, comp2.p);
+      std::swap(comp.cap, comp2.cap);
+      uint32_t flags[3] = {0, 0, 0};
+      MHGP_HIP(hipMemcpyAsync(&flags[0], changed.p, 4, hipMemcpyDeviceToHost, s));
+      MHGP_HIP(hipMemcpyAsync(&flags[2], live_n.p, 4, hipMemcpyDeviceToHost, s));
+      copy_wait(&flags[1], err.p, 4, hipMemcpyDeviceToHost, s);
+      if (flags[1]) throw std::runtime_error("spanning forest of the tower: cyclic hook forest");
+      if (!flags[0]) break;
+      live = flags[2];
+      lu = Lu[q].p;
+      lv = Lv[q].p;
+      li = Li[q].p;
+    }
+    if (E) hipLaunchKernelGGL(k_msf_count, ge, tb, 0, s, (int64_t)E, msf.p, es.p, mcount.p);
+    // keep births, multifusions and shared levels; absorb silent growth
+    auto &absorb = W.absorb, &rep = W.rep, &dense = W.dense;
+    absorb.ensure(minima_n);
+    rep.ensure(minima_n);
+    hipLaunchKernelGGL(k_fill_u32, gm, tb, 0, s, absorb.p, minima_n, kInvalid);
+    hipLaunchKernelGGL(k_reduce_saddles, dim3(blocks_for(S, 256)), tb, 0, s, (int64_t)S, base.p, sizes.p,
+                       carrier.p, scls.p, first.p, mcount.p, k == 1 ? 1 : 0, keep.p, kept.p, absorb.p);
+    const uint32_t *d_rep = nullptr, *d_dense = nullptr;
+    uint32_t Mdense = (uint32_t)minima_n;
+    if (k >= 2) {
+      hipLaunchKernelGGL(k_absorb_roots, gm, tb, 0, s, minima_n, absorb.p, rep.p, err.p);
+      uint32_t bad = 0;
+      copy_wait(&bad, err.p, 4, hipMemcpyDeviceToHost, s);
+      if (bad) throw std::runtime_error("tower absorption forest is cyclic");
+      dense.ensure(minima_n);
+      hipLaunchKernelGGL(k_fill_u32, gm, tb, 0, s, dense.p, minima_n, 0u);
+      hipLaunchKernelGGL(k_flag_used, dim3(blocks_for(S, 256)), tb, 0, s, (int64_t)S, keep.p, base.p, sizes.p,
+                         carrier.p, rep.p, dense.p);
+      c.scanner.inclusive(dense.p, minima_n, s);
+      copy_wait(&Mdense, dense.p + minima_n - 1, 4, hipMemcpyDeviceToHost, s);
+      d_rep = rep.p;
+      d_dense = dense.p;
+    }
     c.scanner.inclusive(keep.p, S, s);
     c.scanner.inclusive(kept.p, S, s);
     MHGP_HIP(hipMemcpyAsync(&S2, keep.p + S - 1, 4, hipMemcpyDeviceToHost, s));
@@ -575,12 +837,13 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
     c_lev.ensure(S2);
     c_base.ensure(S2);
     c_car.ensure(arms2);
-    hipLaunchKernelGGL(k_compact_saddles, dim3(blocks_for(S, 256)), dim3(256), 0, s, (int64_t)S, keep.p, kept.p,
-                       base.p, sizes.p, carrier.p, scls.p, slev.p, c_cls.p, c_lev.p, c_base.p, c_car.p);
+    hipLaunchKernelGGL(k_compact_saddles, dim3(blocks_for(S, 256)), tb, 0, s, (int64_t)S, keep.p, kept.p, base.p,
+                       sizes.p, carrier.p, scls.p, slev.p, d_rep, d_dense, c_cls.p, c_lev.p, c_base.p, c_car.p);
     d_cls = c_cls.p;
     d_lev = c_lev.p;
     d_base = c_base.p;
     d_car = c_car.p;
+    if (k >= 2) out.minima = Mdense;
   }
   // page-locked results: cls | arm_base | carrier | level
   const size_t o_base = (size_t)S2 * 4, o_car = o_base + ((size_t)S2 + 1) * 4;
@@ -611,7 +874,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
     // supports and the minima labels
     out.minima_labels.resize((size_t)M * k);
     if (M) copy_wait(out.minima_labels.data(), labels.p, (size_t)M * k * 4, hipMemcpyDeviceToHost, s);
-    DevBuf<uint32_t> slab, ssup;
+    auto &slab = W.slab, &ssup = W.ssup;
     slab.ensure((size_t)S * kMaxLabel);
     ssup.ensure(4 * (size_t)S);
     hipLaunchKernelGGL(k_saddle_labels, dim3(blocks_for(S, 256)), dim3(256), 0, s, B.events.p, B.order.p,

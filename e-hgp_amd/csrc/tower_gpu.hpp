@@ -1,6 +1,7 @@
 #pragma once
 
 #include <cstdint>
+#include <initializer_list>
 #include <vector>
 
 #include "context.hpp"
@@ -22,6 +23,30 @@ struct TowerOrderData {
   std::vector<uint32_t> minima_labels;  // only when the host must descend
   std::vector<uint32_t> saddle_labels;  // 11 per saddle, only when the host must descend
   int64_t steps = 0;
+};
+
+// Device scratch of the per-order tower work, kept by the context; a buffer
+// is reallocated only when an order needs more than it holds.
+struct TowerWork {
+  DevBuf<uint32_t> flag, minima_pos, saddle_pos, tvals, labels;
+  DevBuf<uint64_t> tkeys, steps, qcount;
+  DevBuf<uint32_t> sizes, carrier, scls, base, arm_saddle, queue;
+  DevBuf<double> slev, c_lev;
+  DevBuf<uint32_t> first, keep, kept, flag1, c_cls, c_base, c_car;
+  DevBuf<uint32_t> eflag, eu, ev, es;
+  DevBuf<uint32_t> comp, comp2, best, hook, msf, mcount, changed, err, live_n;
+  DevBuf<uint32_t> Lu[2], Lv[2], Li[2];
+  DevBuf<uint32_t> absorb, rep, dense, slab, ssup;
+  int64_t bytes() const {
+    int64_t b = 0;
+    for (const DevBuf<uint32_t> *x : {&flag, &minima_pos, &saddle_pos, &tvals, &labels, &sizes, &carrier, &scls,
+                                      &base, &arm_saddle, &queue, &first, &keep, &kept, &flag1, &c_cls, &c_base,
+                                      &c_car, &eflag, &eu, &ev, &es, &comp, &comp2, &best, &hook, &msf, &mcount,
+                                      &changed, &err, &live_n, &Lu[0], &Lu[1], &Lv[0], &Lv[1], &Li[0], &Li[1],
+                                      &absorb, &rep, &dense, &slab, &ssup})
+      b += (int64_t)x->bytes();
+    return b + (int64_t)(tkeys.bytes() + steps.bytes() + qcount.bytes() + slev.bytes() + c_lev.bytes());
+  }
 };
 
 void tower_order_device(Context &c, int k, TowerOrderData &out);

@@ -83,3 +83,17 @@ def test_tower_contract_overlap_case():
     for name in ("overlap-k2", "k1-emst"):
         case = cases[name]
         _check(np.asarray(case["points"], dtype=float), case["k_max"], name, gamma=True)
+
+
+def test_tower_shared_levels_translated_copies():
+    """Three exact translates of one cluster: every sphere has twins of the
+    same exact level, so saddles share levels and are contracted as groups."""
+    rng = np.random.default_rng(5)
+    a = np.round(rng.random((16, 3)) * 2**20) / 2**20  # translation by 4 stays exact
+    pts = np.concatenate([a, a + np.array([4.0, 0, 0]), a + np.array([0, 4.0, 0])])
+    _check(pts, 8, "translated-copies")
+
+
+def test_tower_lattice90():
+    """Enough minima per order that most saddles only grow a component."""
+    _check(workloads.jittered_lattice(90, side=5, seed=44), 5, "lattice90")
