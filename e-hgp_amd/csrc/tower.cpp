@@ -390,6 +390,7 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
         if (t.joinable()) t.join();
     }
   } joiner{workers};
+  tower_rank_index(c);
   // highest order first: its contraction is the longest and overlaps the
   // device work of all lower orders
   for (int k = K; k >= 1; --k) {

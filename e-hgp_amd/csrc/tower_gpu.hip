@@ -11,6 +11,9 @@
 //    least point strictly inside (the miniball strictly shrinks);
 //  * anything the certified filters cannot decide is flagged for the host's
 //    exact descent.
+#include <cstdio>
+#include <cstdlib>
+
 #include "pipeline.hpp"
 #include "predicates.hpp"
 #include "scan.hpp"
@@ -50,22 +53,23 @@ __device__ inline int event_label(const EventRec &e, uint32_t *out) {
 
 __device__ inline int rank_of(const EventRec &e) { return (e.meta >> 8) & 0xff; }
 
-// Select records of one closed rank in canonical order (positions flagged).
-__global__ void k_flag_rank(const EventRec *__restrict__ ev, const uint32_t *__restrict__ order, int64_t n,
-                            int rank, uint32_t *flag) {
+// Records grouped by closed rank, canonical order kept inside each rank:
+// key = rank (0xff for a degenerate record), value = sorted position.
+__global__ void k_rank_keys(const EventRec *__restrict__ ev, const uint32_t *__restrict__ order, int64_t n,
+                            uint64_t *keys, uint32_t *vals) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const EventRec &e = ev[order[i]];
-  flag[i] = (rank_of(e) == rank && !((e.meta >> 24) & 1)) ? 1u : 0u;
+  keys[i] = ((e.meta >> 24) & 1) ? 0xffull : (uint64_t)rank_of(e);
+  vals[i] = (uint32_t)i;
 }
 
-__global__ void k_compact(const uint32_t *__restrict__ order, const uint32_t *__restrict__ scan, int64_t n,
-                          uint32_t *out) {
+__global__ void k_rank_bounds(const uint64_t *__restrict__ keys, int64_t n, uint32_t *lo, uint32_t *hi) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint32_t s = scan[i];
-  uint32_t prev = i ? scan[i - 1] : 0;
-  if (s != prev) out[s - 1] = (uint32_t)i;  // sorted position
+  const uint64_t key = keys[i];
+  if (i == 0 || keys[i - 1] != key) lo[key] = (uint32_t)i;
+  if (i == n - 1 || keys[i + 1] != key) hi[key] = (uint32_t)(i + 1);
 }
 
 __global__ void k_insert_minima(const EventRec *__restrict__ ev, const uint32_t *__restrict__ order,
@@ -402,22 +406,23 @@ __global__ void k_saddle_labels(const EventRec *__restrict__ ev, const uint32_t 
 
 // Reduction of an order's saddles before the level-ordered contraction.
 //
-// Every saddle s contributes star edges (its first arm's carrier to each other
-// arm's carrier), keyed by arm index, so edge order is level order.  A
-// Boruvka minimum spanning forest over these edges tells, for each saddle, how
-// many distinct components its carriers join: the saddle's forest edges.
-// first[m] is the least saddle with an arm at minimum m; a carrier with
-// first[m] == s is fresh (an untouched one-minimum component), any other
-// carrier lies in a component that already has a reduced root.  For a saddle
-// whose exact level no other saddle shares:
-//  * all carriers fresh: a reduced birth (kept);
-//  * fresh carriers f and owned carriers spanning D = edges - f + 1 distinct
-//    components: D >= 2 is a multifusion (kept); D == 1 changes no root, the
-//    fresh minima join the owned component silently (the saddle is dropped
-//    and the fresh minima are absorbed into an owned carrier).
-// Saddles of a shared level are kept as they are (their group is contracted
-// together on the host).  Order one has no fresh minima: every point is born
-// at level zero.
+// first[m] is the least saddle with an arm at minimum m.  A carrier m of
+// saddle s is fresh when first[m] == s (an untouched one-minimum component),
+// owned otherwise (its component already has a reduced root; order one has
+// no fresh minima, every point is born at level zero).
+//  * Attachment: every fresh carrier joins, from its first saddle on, the
+//    component of that saddle's first owned carrier, or of its first fresh
+//    carrier when it has none.  The attachment pointers form a forest (a
+//    pointer goes to a minimum first touched strictly earlier, or to a root)
+//    and its roots are the representatives.
+//  * Owned carriers of one saddle on distinct representatives give star
+//    edges, keyed by arm index (level order).  A Boruvka spanning forest over
+//    them counts, per saddle, the components its owned carriers join beyond
+//    the first.
+//  * A saddle at an unshared level with owned carriers and no such forest edge
+//    changes no root (silent growth or no new connection) and is dropped; the
+//    others (births, multifusions, shared levels) go to the host with their
+//    carriers replaced by representatives.
 __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -440,26 +445,72 @@ __global__ void k_first_touch(const uint32_t *__restrict__ arm_saddle, const uin
   atomicMin(&first[c], arm_saddle[a]);
 }
 
+__device__ inline bool carrier_owned(const uint32_t *first, uint32_t c, uint32_t s, int all_owned) {
+  return all_owned || first[c] < s;
+}
+
+__global__ void k_attach(int64_t S, const uint32_t *__restrict__ base, const uint32_t *__restrict__ incl,
+                         const uint32_t *__restrict__ carrier, const uint32_t *__restrict__ first, int all_owned,
+                         uint32_t *attach) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const uint32_t b = base[s], e = incl[s];
+  uint32_t target = kInvalid, fresh0 = kInvalid;
+  for (uint32_t a = b; a < e; ++a) {
+    const uint32_t c = carrier[a];
+    if (carrier_owned(first, c, (uint32_t)s, all_owned)) { target = c; break; }
+    if (fresh0 == kInvalid) fresh0 = c;
+  }
+  if (target == kInvalid) target = fresh0;
+  for (uint32_t a = b; a < e; ++a) {
+    const uint32_t c = carrier[a];
+    if (c != target && !carrier_owned(first, c, (uint32_t)s, all_owned)) attach[c] = target;
+  }
+}
+
+// Star edge of arm a: an owned carrier on a representative that neither the
+// saddle's first owned carrier nor an earlier owned arm has.
+__device__ inline bool arm_edge(const uint32_t *base, const uint32_t *incl, const uint32_t *carrier,
+                                const uint32_t *first, const uint32_t *rep, uint32_t s, uint32_t a, int all_owned,
+                                uint32_t &r0, uint32_t &r) {
+  const uint32_t c = carrier[a];
+  if (!carrier_owned(first, c, s, all_owned)) return false;
+  r = rep[c];
+  uint32_t x = base[s];
+  while (!carrier_owned(first, carrier[x], s, all_owned)) ++x;  // a itself is owned
+  if (x == a) return false;
+  r0 = rep[carrier[x]];
+  if (r == r0) return false;
+  for (++x; x < a; ++x)
+    if (carrier_owned(first, carrier[x], s, all_owned) && rep[carrier[x]] == r) return false;
+  (void)incl;
+  return true;
+}
+
 __global__ void k_edge_flags(const uint32_t *__restrict__ arm_saddle, const uint32_t *__restrict__ base,
-                             const uint32_t *__restrict__ carrier, int64_t arms, uint32_t *flag) {
+                             const uint32_t *__restrict__ incl, const uint32_t *__restrict__ carrier,
+                             const uint32_t *__restrict__ first, const uint32_t *__restrict__ rep, int all_owned,
+                             int64_t arms, uint32_t *flag) {
   int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= arms) return;
-  const uint32_t b = base[arm_saddle[a]], c = carrier[a];
-  bool dup = false;  // the hub, or a carrier an earlier arm already joins
-  for (uint32_t x = b; x < (uint32_t)a; ++x) dup |= carrier[x] == c;
-  flag[a] = ((uint32_t)a != b && !dup) ? 1u : 0u;
+  uint32_t r0, r;
+  flag[a] = arm_edge(base, incl, carrier, first, rep, arm_saddle[a], (uint32_t)a, all_owned, r0, r) ? 1u : 0u;
 }
 
 __global__ void k_edge_list(const uint32_t *__restrict__ arm_saddle, const uint32_t *__restrict__ base,
-                            const uint32_t *__restrict__ carrier, const uint32_t *__restrict__ scan, int64_t arms,
-                            uint32_t *eu, uint32_t *ev, uint32_t *es) {
+                            const uint32_t *__restrict__ incl, const uint32_t *__restrict__ carrier,
+                            const uint32_t *__restrict__ first, const uint32_t *__restrict__ rep, int all_owned,
+                            const uint32_t *__restrict__ scan, int64_t arms, uint32_t *eu, uint32_t *ev,
+                            uint32_t *es) {
   int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= arms) return;
   const uint32_t prev = a ? scan[a - 1] : 0u;
   if (scan[a] == prev) return;
   const uint32_t s = arm_saddle[a];
-  eu[prev] = carrier[base[s]];
-  ev[prev] = carrier[a];
+  uint32_t r0 = 0, r = 0;
+  arm_edge(base, incl, carrier, first, rep, s, (uint32_t)a, all_owned, r0, r);
+  eu[prev] = r0;
+  ev[prev] = r;
   es[prev] = s;
 }
 
@@ -480,8 +531,10 @@ __global__ void k_msf_best(int64_t L, const uint32_t *__restrict__ lu, const uin
   }
   const bool live = i < L && cu != cv;
   if (live) {
-    atomicMin(&best[cu], idx);
-    atomicMin(&best[cv], idx);
+    // the list runs in increasing index order, so a plain read first lets most
+    // edges of a large component skip an atomic that cannot win
+    if (idx < best[cu]) atomicMin(&best[cu], idx);
+    if (idx < best[cv]) atomicMin(&best[cv], idx);
   }
   // wave-aggregated append
   const uint64_t mask = __ballot(live);
@@ -564,39 +617,25 @@ __global__ void k_msf_count(int64_t E, const uint32_t *__restrict__ msf, const u
 __global__ void k_reduce_saddles(int64_t S, const uint32_t *__restrict__ base, const uint32_t *__restrict__ incl,
                                  const uint32_t *__restrict__ carrier, const uint32_t *__restrict__ cls,
                                  const uint32_t *__restrict__ first, const uint32_t *__restrict__ mcount,
-                                 int all_owned, uint32_t *keep, uint32_t *kept_arms, uint32_t *absorb) {
+                                 int all_owned, uint32_t *keep, uint32_t *kept_arms) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
   const uint32_t b = base[s], e = incl[s];
   const uint32_t c = cls[s];
   const bool shared = (s > 0 && cls[s - 1] == c) || (s + 1 < S && cls[s + 1] == c);
   bool k = true;
-  if (!shared) {
-    uint32_t seen[4];
-    int ns = 0, fresh = 0, owned_n = 0;
-    uint32_t owned = kInvalid;
-    for (uint32_t a = b; a < e && ns < 4; ++a) {
-      const uint32_t x = carrier[a];
-      bool dup = false;
-      for (int q = 0; q < ns; ++q) dup |= seen[q] == x;
-      if (dup) continue;
-      seen[ns++] = x;
-      if (!all_owned && first[x] == (uint32_t)s) fresh++;
-      else { owned_n++; if (owned == kInvalid) owned = x; }
-    }
-    if (owned_n > 0 && (int)mcount[s] + 1 - fresh < 2) {
-      k = false;
-      for (int q = 0; q < ns; ++q)
-        if (seen[q] != owned && !all_owned && first[seen[q]] == (uint32_t)s) absorb[seen[q]] = owned;
-    }
+  if (!shared && mcount[s] == 0) {
+    bool any_owned = false;
+    for (uint32_t a = b; a < e; ++a) any_owned |= carrier_owned(first, carrier[a], (uint32_t)s, all_owned);
+    k = !any_owned;  // no owned carrier: a birth
   }
   keep[s] = k ? 1u : 0u;
   kept_arms[s] = k ? e - b : 0u;
 }
 
-__global__ void k_absorb_roots(int64_t M, uint32_t *absorb, uint32_t *rep, uint32_t *err) {
+__global__ void k_attach_roots(int64_t M, uint32_t *attach, uint32_t *rep, uint32_t *err) {
   int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m < M) rep[m] = forest_root(absorb, (uint32_t)m, true, M, err);
+  if (m < M) rep[m] = forest_root(attach, (uint32_t)m, true, M, err);
 }
 
 // Minima met by a kept saddle get dense ids (flag, then an inclusive scan).
@@ -628,22 +667,39 @@ __global__ void k_compact_saddles(int64_t S, const uint32_t *__restrict__ kpos, 
 
 }  // namespace
 
-void tower_order_device(Context &c, int k, TowerOrderData &out) {
+void tower_rank_index(Context &c) {
   CatalogBuffers &B = *c.catalog;
   hipStream_t s = c.stream;
   const int64_t n = B.count;
-  out = TowerOrderData{};
   if (!c.tower_work) c.tower_work = std::make_unique<TowerWork>();
   TowerWork &W = *c.tower_work;
-  auto &flag = W.flag, &minima_pos = W.minima_pos, &saddle_pos = W.saddle_pos;
-  flag.ensure(n);
-  // minima of order k (k >= 2)
-  hipLaunchKernelGGL(k_flag_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.events.p, B.order.p, n, k, flag.p);
-  c.scanner.inclusive(flag.p, n, s);
-  uint32_t M = 0;
-  if (n) copy_wait(&M, flag.p + n - 1, 4, hipMemcpyDeviceToHost, s);
-  minima_pos.ensure(M);
-  hipLaunchKernelGGL(k_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.order.p, flag.p, n, minima_pos.p);
+  std::fill(W.rank_lo, W.rank_lo + 256, 0u);
+  std::fill(W.rank_hi, W.rank_hi + 256, 0u);
+  if (n == 0) return;
+  W.rank_keys.ensure(n);
+  W.rank_vals.ensure(n);
+  W.rank_bounds.ensure(512);
+  hipLaunchKernelGGL(k_rank_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.events.p, B.order.p, n,
+                     W.rank_keys.p, W.rank_vals.p);
+  c.sorter.sort(W.rank_keys.p, W.rank_vals.p, n, 0, 8, s);
+  hipLaunchKernelGGL(k_fill_u32, dim3(2), dim3(256), 0, s, W.rank_bounds.p, (int64_t)512, 0u);
+  hipLaunchKernelGGL(k_rank_bounds, dim3(blocks_for(n, 256)), dim3(256), 0, s, W.rank_keys.p, n, W.rank_bounds.p,
+                     W.rank_bounds.p + 256);
+  uint32_t h[512];
+  copy_wait(h, W.rank_bounds.p, sizeof h, hipMemcpyDeviceToHost, s);
+  std::copy(h, h + 256, W.rank_lo);
+  std::copy(h + 256, h + 512, W.rank_hi);
+}
+
+void tower_order_device(Context &c, int k, TowerOrderData &out) {
+  CatalogBuffers &B = *c.catalog;
+  hipStream_t s = c.stream;
+  out = TowerOrderData{};
+  TowerWork &W = *c.tower_work;
+  // minima of order k (k >= 2) and saddles of order k: records of closed
+  // rank k and k + 1, as runs of the rank index
+  const uint32_t M = W.rank_hi[k] - W.rank_lo[k], S = W.rank_hi[k + 1] - W.rank_lo[k + 1];
+  const uint32_t *minima_pos = W.rank_vals.p + W.rank_lo[k], *saddle_pos = W.rank_vals.p + W.rank_lo[k + 1];
   out.minima = M;
   uint64_t cap = 1;
   while (cap < 2ull * M + 16) cap <<= 1;
@@ -655,15 +711,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   MHGP_HIP(hipMemsetAsync(tkeys.p, 0, cap * 8, s));
   if (M)
     hipLaunchKernelGGL(k_insert_minima, dim3(blocks_for(M, 256)), dim3(256), 0, s, B.events.p, B.order.p,
-                       minima_pos.p, (int64_t)M, k, labels.p, tkeys.p, tvals.p, cap - 1);
-  // saddles of order k
-  hipLaunchKernelGGL(k_flag_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.events.p, B.order.p, n, k + 1,
-                     flag.p);
-  c.scanner.inclusive(flag.p, n, s);
-  uint32_t S = 0;
-  if (n) copy_wait(&S, flag.p + n - 1, 4, hipMemcpyDeviceToHost, s);
-  saddle_pos.ensure(S);
-  hipLaunchKernelGGL(k_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.order.p, flag.p, n, saddle_pos.p);
+                       minima_pos, (int64_t)M, k, labels.p, tkeys.p, tvals.p, cap - 1);
   out.saddles = S;
   if (S == 0) return;
   auto &sizes = W.sizes, &carrier = W.carrier, &scls = W.scls;
@@ -671,7 +719,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   auto &steps = W.steps;
   sizes.ensure(S);
   hipLaunchKernelGGL(k_arm_sizes, dim3(blocks_for(S, 256)), dim3(256), 0, s, B.events.p, B.order.p,
-                     saddle_pos.p, (int64_t)S, sizes.p);
+                     saddle_pos, (int64_t)S, sizes.p);
   c.scanner.inclusive(sizes.p, S, s);
   uint32_t arms = 0;
   copy_wait(&arms, sizes.p + S - 1, 4, hipMemcpyDeviceToHost, s);
@@ -688,7 +736,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   steps.ensure(1);
   MHGP_HIP(hipMemsetAsync(steps.p, 0, 8, s));
   ArmParams A{};
-  A.ev = B.events.p; A.order = B.order.p; A.saddle_pos = saddle_pos.p; A.arm_base = base.p;
+  A.ev = B.events.p; A.order = B.order.p; A.saddle_pos = saddle_pos; A.arm_base = base.p;
   A.arm_saddle = arm_saddle.p; A.arms = arms; A.k = k;
   A.table_keys = tkeys.p; A.table_vals = tvals.p; A.mask = cap - 1; A.labels = labels.p;
   A.X = c.X.p; A.Y = c.Y.p; A.Z = c.Z.p;
@@ -719,7 +767,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   scls.ensure(S);
   slev.ensure(S);
   hipLaunchKernelGGL(k_saddle_meta, dim3(blocks_for(S, 256)), dim3(256), 0, s, B.events.p, B.order.p, B.cls.p,
-                     saddle_pos.p, (int64_t)S, scls.p, slev.p);
+                     saddle_pos, (int64_t)S, scls.p, slev.p);
   // drop the saddles that cannot change the contraction (unless the host
   // must descend some arms: then every saddle goes down as it is)
   const int64_t minima_n = k == 1 ? c.n : (int64_t)M;
@@ -739,24 +787,39 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   const double *d_lev = slev.p;
   auto &c_cls = W.c_cls, &c_base = W.c_base, &c_car = W.c_car;
   auto &c_lev = W.c_lev;
+  uint32_t msf_edges = 0;
+  int msf_rounds = 0;
   if (!any_invalid) {
     const dim3 tb(256);
-    // star edges of the saddles, in arm (= level) order
+    const int all_owned = k == 1 ? 1 : 0;
+    const dim3 gm(blocks_for(minima_n, 256)), gs(blocks_for(S, 256)), ga(blocks_for(arms, 256));
+    // attachment forest of the fresh minima and its representatives
+    auto &attach = W.attach, &rep = W.rep, &dense = W.dense, &err = W.err;
+    attach.ensure(minima_n);
+    rep.ensure(minima_n);
+    err.ensure(1);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), tb, 0, s, err.p, (int64_t)1, 0u);
+    hipLaunchKernelGGL(k_fill_u32, gm, tb, 0, s, attach.p, minima_n, kInvalid);
+    hipLaunchKernelGGL(k_attach, gs, tb, 0, s, (int64_t)S, base.p, sizes.p, carrier.p, first.p, all_owned, attach.p);
+    hipLaunchKernelGGL(k_attach_roots, gm, tb, 0, s, minima_n, attach.p, rep.p, err.p);
+    // star edges between the representatives of owned carriers, in arm (= level) order
     auto &eflag = W.eflag, &eu = W.eu, &ev = W.ev, &es = W.es;
     eflag.ensure(arms);
-    hipLaunchKernelGGL(k_edge_flags, dim3(blocks_for(arms, 256)), tb, 0, s, arm_saddle.p, base.p, carrier.p,
-                       (int64_t)arms, eflag.p);
+    hipLaunchKernelGGL(k_edge_flags, ga, tb, 0, s, arm_saddle.p, base.p, sizes.p, carrier.p, first.p, rep.p,
+                       all_owned, (int64_t)arms, eflag.p);
     c.scanner.inclusive(eflag.p, arms, s);
-    uint32_t E = 0;
+    uint32_t E = 0, bad = 0;
+    MHGP_HIP(hipMemcpyAsync(&bad, err.p, 4, hipMemcpyDeviceToHost, s));
     copy_wait(&E, eflag.p + arms - 1, 4, hipMemcpyDeviceToHost, s);
+    if (bad) throw std::runtime_error("tower attachment forest is cyclic");
     eu.ensure(E);
     ev.ensure(E);
     es.ensure(E);
-    hipLaunchKernelGGL(k_edge_list, dim3(blocks_for(arms, 256)), tb, 0, s, arm_saddle.p, base.p, carrier.p, eflag.p,
-                       (int64_t)arms, eu.p, ev.p, es.p);
+    hipLaunchKernelGGL(k_edge_list, ga, tb, 0, s, arm_saddle.p, base.p, sizes.p, carrier.p, first.p, rep.p,
+                       all_owned, eflag.p, (int64_t)arms, eu.p, ev.p, es.p);
     // Boruvka spanning forest of the minima over the star edges
     auto &comp = W.comp, &comp2 = W.comp2, &best = W.best, &hook = W.hook, &msf = W.msf;
-    auto &mcount = W.mcount, &changed = W.changed, &err = W.err, &live_n = W.live_n;
+    auto &mcount = W.mcount, &changed = W.changed, &live_n = W.live_n;
     auto &Lu = W.Lu, &Lv = W.Lv, &Li = W.Li;
     for (int q = 0; q < 2; ++q) {
       Lu[q].ensure(E);
@@ -766,23 +829,23 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
     live_n.ensure(1);
     comp.ensure(minima_n);
     comp2.ensure(minima_n);
-    err.ensure(1);
     best.ensure(minima_n);
     hook.ensure(minima_n);
     msf.ensure(E + 1);
     mcount.ensure(S);
     changed.ensure(1);
-    const dim3 gm(blocks_for(minima_n, 256)), ge(blocks_for(E + 1, 256));
+    const dim3 ge(blocks_for(E + 1, 256));
     hipLaunchKernelGGL(k_identity, gm, tb, 0, s, comp.p, minima_n);
     hipLaunchKernelGGL(k_fill_u32, gm, tb, 0, s, best.p, minima_n, kInvalid);
     hipLaunchKernelGGL(k_fill_u32, gm, tb, 0, s, hook.p, minima_n, kInvalid);
     hipLaunchKernelGGL(k_fill_u32, ge, tb, 0, s, msf.p, (int64_t)E, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(S, 256)), tb, 0, s, mcount.p, (int64_t)S, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), tb, 0, s, err.p, (int64_t)1, 0u);
     const uint32_t *lu = eu.p, *lv = ev.p, *li = nullptr;
     uint32_t live = E;
+    msf_edges = E;
     for (int round = 0; live > 0; ++round) {
       if (round > 64) throw std::runtime_error("spanning forest of the tower did not converge");
+      msf_rounds = round + 1;
       const int q = round & 1;
       hipLaunchKernelGGL(k_fill_u32, dim3(1), tb, 0, s, changed.p, (int64_t)1, 0u);
       hipLaunchKernelGGL(k_fill_u32, dim3(1), tb, 0, s, live_n.p, (int64_t)1, 0u);
@@ -805,24 +868,15 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
       li = Li[q].p;
     }
     if (E) hipLaunchKernelGGL(k_msf_count, ge, tb, 0, s, (int64_t)E, msf.p, es.p, mcount.p);
-    // keep births, multifusions and shared levels; absorb silent growth
-    auto &absorb = W.absorb, &rep = W.rep, &dense = W.dense;
-    absorb.ensure(minima_n);
-    rep.ensure(minima_n);
-    hipLaunchKernelGGL(k_fill_u32, gm, tb, 0, s, absorb.p, minima_n, kInvalid);
-    hipLaunchKernelGGL(k_reduce_saddles, dim3(blocks_for(S, 256)), tb, 0, s, (int64_t)S, base.p, sizes.p,
-                       carrier.p, scls.p, first.p, mcount.p, k == 1 ? 1 : 0, keep.p, kept.p, absorb.p);
+    // keep births, multifusions and shared levels
+    hipLaunchKernelGGL(k_reduce_saddles, gs, tb, 0, s, (int64_t)S, base.p, sizes.p, carrier.p, scls.p, first.p,
+                       mcount.p, all_owned, keep.p, kept.p);
     const uint32_t *d_rep = nullptr, *d_dense = nullptr;
     uint32_t Mdense = (uint32_t)minima_n;
-    if (k >= 2) {
-      hipLaunchKernelGGL(k_absorb_roots, gm, tb, 0, s, minima_n, absorb.p, rep.p, err.p);
-      uint32_t bad = 0;
-      copy_wait(&bad, err.p, 4, hipMemcpyDeviceToHost, s);
-      if (bad) throw std::runtime_error("tower absorption forest is cyclic");
+    if (k >= 2) {  // order one keeps point ids: every point is born
       dense.ensure(minima_n);
       hipLaunchKernelGGL(k_fill_u32, gm, tb, 0, s, dense.p, minima_n, 0u);
-      hipLaunchKernelGGL(k_flag_used, dim3(blocks_for(S, 256)), tb, 0, s, (int64_t)S, keep.p, base.p, sizes.p,
-                         carrier.p, rep.p, dense.p);
+      hipLaunchKernelGGL(k_flag_used, gs, tb, 0, s, (int64_t)S, keep.p, base.p, sizes.p, carrier.p, rep.p, dense.p);
       c.scanner.inclusive(dense.p, minima_n, s);
       copy_wait(&Mdense, dense.p + minima_n - 1, 4, hipMemcpyDeviceToHost, s);
       d_rep = rep.p;
@@ -868,6 +922,11 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   out.level = h_lev;
   out.arms = arms2;
   out.steps = (int64_t)st;
+  if (std::getenv("MHGP_TIMING"))
+    std::fprintf(stderr,
+                 "[mhgp] arms order %2d: %u saddles, %u arms, %llu descending, %llu steps, %u forest edges in %d "
+                 "rounds, %u kept\n",
+                 k, S, arms, (unsigned long long)nq, (unsigned long long)st, msf_edges, msf_rounds, S2);
   const bool need_labels = any_invalid != 0;  // then the saddles were not filtered
   if (need_labels) {
     // the host descends unresolved arms; it needs the saddle labels, their
@@ -878,7 +937,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
     slab.ensure((size_t)S * kMaxLabel);
     ssup.ensure(4 * (size_t)S);
     hipLaunchKernelGGL(k_saddle_labels, dim3(blocks_for(S, 256)), dim3(256), 0, s, B.events.p, B.order.p,
-                       saddle_pos.p, (int64_t)S, slab.p, ssup.p);
+                       saddle_pos, (int64_t)S, slab.p, ssup.p);
     out.saddle_labels.resize((size_t)S * kMaxLabel);
     out.sup.resize(4 * (size_t)S);
     MHGP_HIP(hipMemcpyAsync(out.saddle_labels.data(), slab.p, (size_t)S * kMaxLabel * 4, hipMemcpyDeviceToHost, s));

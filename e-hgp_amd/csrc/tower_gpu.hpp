@@ -28,7 +28,11 @@ struct TowerOrderData {
 // Device scratch of the per-order tower work, kept by the context; a buffer
 // is reallocated only when an order needs more than it holds.
 struct TowerWork {
-  DevBuf<uint32_t> flag, minima_pos, saddle_pos, tvals, labels;
+  // records grouped by closed rank: rank r occupies rank_vals[rank_lo[r], rank_hi[r])
+  DevBuf<uint64_t> rank_keys;
+  DevBuf<uint32_t> rank_vals, rank_bounds;
+  uint32_t rank_lo[256] = {}, rank_hi[256] = {};
+  DevBuf<uint32_t> tvals, labels;
   DevBuf<uint64_t> tkeys, steps, qcount;
   DevBuf<uint32_t> sizes, carrier, scls, base, arm_saddle, queue;
   DevBuf<double> slev, c_lev;
@@ -36,19 +40,22 @@ struct TowerWork {
   DevBuf<uint32_t> eflag, eu, ev, es;
   DevBuf<uint32_t> comp, comp2, best, hook, msf, mcount, changed, err, live_n;
   DevBuf<uint32_t> Lu[2], Lv[2], Li[2];
-  DevBuf<uint32_t> absorb, rep, dense, slab, ssup;
+  DevBuf<uint32_t> attach, rep, dense, slab, ssup;
   int64_t bytes() const {
     int64_t b = 0;
-    for (const DevBuf<uint32_t> *x : {&flag, &minima_pos, &saddle_pos, &tvals, &labels, &sizes, &carrier, &scls,
+    for (const DevBuf<uint32_t> *x : {&rank_vals, &rank_bounds, &tvals, &labels, &sizes, &carrier, &scls,
                                       &base, &arm_saddle, &queue, &first, &keep, &kept, &flag1, &c_cls, &c_base,
                                       &c_car, &eflag, &eu, &ev, &es, &comp, &comp2, &best, &hook, &msf, &mcount,
                                       &changed, &err, &live_n, &Lu[0], &Lu[1], &Lv[0], &Lv[1], &Li[0], &Li[1],
-                                      &absorb, &rep, &dense, &slab, &ssup})
+                                      &attach, &rep, &dense, &slab, &ssup})
       b += (int64_t)x->bytes();
-    return b + (int64_t)(tkeys.bytes() + steps.bytes() + qcount.bytes() + slev.bytes() + c_lev.bytes());
+    return b + (int64_t)(rank_keys.bytes() + tkeys.bytes() + steps.bytes() + qcount.bytes() + slev.bytes() + c_lev.bytes());
   }
 };
 
+// Groups the catalogue's records by closed rank (once per tower call).
+void tower_rank_index(Context &c);
+// Resolves the arms of order k and reduces its saddles (after tower_rank_index).
 void tower_order_device(Context &c, int k, TowerOrderData &out);
 
 }  // namespace mhgp
