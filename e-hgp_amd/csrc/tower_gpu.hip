@@ -237,6 +237,7 @@ struct ArmParams {
   const Node *nodes;
   const double *LX, *LY, *LZ;
   const uint32_t *leaf_id;
+  const uint32_t *leaf_pos;
   int64_t n;
   uint32_t *carrier;  // per arm: minimum index, or kInvalid (host)
   unsigned long long *steps;
@@ -260,22 +261,35 @@ __device__ inline void arm_facet(const ArmParams &A, int64_t arm, uint32_t *F) {
 // One thread per strict arm: the facet is a minimum (hash hit) or the arm
 // is queued for descents.  Splitting the two keeps the descending lanes of a
 // wave together.
-__global__ void k_arm_lookup(ArmParams A, uint32_t *queue, unsigned long long *qcount) {
+// Queued arms carry the Morton position of their facet's least point, so the
+// descents can run in spatial order.
+__global__ void k_arm_lookup(ArmParams A, uint32_t *queue, uint64_t *qkey, unsigned long long *qcount) {
   const int64_t arm = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (arm >= A.arms) return;
+  bool descend = false;
   uint32_t F[kMaxLabel];
-  arm_facet(A, arm, F);
-  if (A.k == 1) {  // order one: the arm is a point, its own minimum
-    A.carrier[arm] = F[0];
-    return;
+  if (arm < A.arms) {
+    arm_facet(A, arm, F);
+    if (A.k == 1) {  // order one: the arm is a point, its own minimum
+      A.carrier[arm] = F[0];
+    } else {
+      const int64_t hit = lookup(F, A.k, A.table_keys, A.table_vals, A.mask, A.labels);
+      A.carrier[arm] = hit >= 0 ? (uint32_t)hit : kInvalid;
+      descend = hit < 0;
+    }
   }
-  const int64_t hit = lookup(F, A.k, A.table_keys, A.table_vals, A.mask, A.labels);
-  if (hit >= 0) {
-    A.carrier[arm] = (uint32_t)hit;
-    return;
+  // wave-aggregated append to the descent queue
+  const uint64_t mask = __ballot(descend);
+  if (!mask) return;
+  const int lane = (int)(threadIdx.x & 63u);
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  unsigned long long at = 0;
+  if (lane == leader) at = atomicAdd(qcount, (unsigned long long)__popcll(mask));
+  at = __shfl(at, leader);
+  if (descend) {
+    const unsigned long long pos = at + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
+    queue[pos] = (uint32_t)arm;
+    qkey[pos] = A.leaf_pos[F[0]];
   }
-  A.carrier[arm] = kInvalid;
-  queue[atomicAdd(qcount, 1ull)] = (uint32_t)arm;
 }
 
 // One thread per queued arm: certified descents until a minimum's label is
@@ -740,7 +754,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   A.arm_saddle = arm_saddle.p; A.arms = arms; A.k = k;
   A.table_keys = tkeys.p; A.table_vals = tvals.p; A.mask = cap - 1; A.labels = labels.p;
   A.X = c.X.p; A.Y = c.Y.p; A.Z = c.Z.p;
-  A.nodes = c.nodes.p; A.LX = c.LX.p; A.LY = c.LY.p; A.LZ = c.LZ.p; A.leaf_id = c.leaf_id.p; A.n = c.n;
+  A.nodes = c.nodes.p; A.LX = c.LX.p; A.LY = c.LY.p; A.LZ = c.LZ.p; A.leaf_id = c.leaf_id.p; A.leaf_pos = c.leaf_pos.p; A.n = c.n;
   A.carrier = carrier.p;
   A.steps = (unsigned long long *)steps.p;
   auto &queue = W.queue;
@@ -749,11 +763,18 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   qcount.ensure(1);
   MHGP_HIP(hipMemsetAsync(qcount.p, 0, 8, s));
   c.arm_clock.start(s);
-  hipLaunchKernelGGL(k_arm_lookup, dim3(blocks_for(arms, 256)), dim3(256), 0, s, A, queue.p,
+  auto &qkey = W.qkey;
+  qkey.ensure(arms);
+  hipLaunchKernelGGL(k_arm_lookup, dim3(blocks_for(arms, 256)), dim3(256), 0, s, A, queue.p, qkey.p,
                      (unsigned long long *)qcount.p);
   uint64_t nq = 0;
   MHGP_HIP(hipMemcpyAsync(&nq, qcount.p, 8, hipMemcpyDeviceToHost, s));
   MHGP_HIP(hipStreamSynchronize(s));
+  if (nq > 1) {  // spatial order: neighbouring lanes walk neighbouring nodes
+    int bits = 8;
+    while (bits < 32 && (1ll << bits) < c.n) bits += 8;
+    c.sorter.sort(qkey.p, queue.p, (int64_t)nq, 0, bits, s);
+  }
   if (nq)
     hipLaunchKernelGGL(k_resolve_arms, dim3(blocks_for(nq, kArmBlock)), dim3(kArmBlock), 0, s, A, queue.p,
                        (uint32_t)nq);

@@ -33,7 +33,7 @@ struct TowerWork {
   DevBuf<uint32_t> rank_vals, rank_bounds;
   uint32_t rank_lo[256] = {}, rank_hi[256] = {};
   DevBuf<uint32_t> tvals, labels;
-  DevBuf<uint64_t> tkeys, steps, qcount;
+  DevBuf<uint64_t> tkeys, steps, qcount, qkey;
   DevBuf<uint32_t> sizes, carrier, scls, base, arm_saddle, queue;
   DevBuf<double> slev, c_lev;
   DevBuf<uint32_t> first, keep, kept, flag1, c_cls, c_base, c_car;
@@ -49,7 +49,7 @@ struct TowerWork {
                                       &changed, &err, &live_n, &Lu[0], &Lu[1], &Lv[0], &Lv[1], &Li[0], &Li[1],
                                       &attach, &rep, &dense, &slab, &ssup})
       b += (int64_t)x->bytes();
-    return b + (int64_t)(rank_keys.bytes() + tkeys.bytes() + steps.bytes() + qcount.bytes() + slev.bytes() + c_lev.bytes());
+    return b + (int64_t)(rank_keys.bytes() + qkey.bytes() + tkeys.bytes() + steps.bytes() + qcount.bytes() + slev.bytes() + c_lev.bytes());
   }
 };
 
