@@ -53,6 +53,34 @@ __device__ inline int event_label(const EventRec &e, uint32_t *out) {
 
 __device__ inline int rank_of(const EventRec &e) { return (e.meta >> 8) & 0xff; }
 
+// FP32 copy of the LBVH for ball queries: each internal node holds both
+// children's boxes rounded outward (a leaf's box is its point).
+__global__ void k_wide_nodes(int64_t m, const Node *__restrict__ nodes, const double *__restrict__ LX,
+                             const double *__restrict__ LY, const double *__restrict__ LZ, WideNode *out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const Node &N = nodes[i];
+  WideNode w;
+  for (int c = 0; c < 2; ++c) {
+    const uint32_t x = c ? N.right : N.left;
+    double lo[3], hi[3];
+    if (x & kLeafBit) {
+      const uint32_t lp = x & ~kLeafBit;
+      lo[0] = hi[0] = LX[lp];
+      lo[1] = hi[1] = LY[lp];
+      lo[2] = hi[2] = LZ[lp];
+    } else {
+      for (int a = 0; a < 3; ++a) { lo[a] = nodes[x].lo[a]; hi[a] = nodes[x].hi[a]; }
+    }
+    for (int a = 0; a < 3; ++a) {
+      w.lo[c][a] = __double2float_rd(lo[a]);
+      w.hi[c][a] = __double2float_ru(hi[a]);
+    }
+    w.child[c] = x;
+  }
+  out[i] = w;
+}
+
 // Records grouped by closed rank, canonical order kept inside each rank:
 // key = rank (0xff for a degenerate record), value = sorted position.
 __global__ void k_rank_keys(const EventRec *__restrict__ ev, const uint32_t *__restrict__ order, int64_t n,
@@ -234,7 +262,7 @@ struct ArmParams {
   uint64_t mask;
   const uint32_t *labels;
   const double *X, *Y, *Z;
-  const Node *nodes;
+  const WideNode *wide;
   const double *LX, *LY, *LZ;
   const uint32_t *leaf_id;
   const uint32_t *leaf_pos;
@@ -247,7 +275,9 @@ constexpr int kArmBlock = 64;
 constexpr int kArmStack = 40;  // LBVH depth is 18 at 50k points, 32 at 200k; overflow goes to the host
 
 // Facet of an arm: the saddle's closed label without the arm's shell point.
-__device__ inline void arm_facet(const ArmParams &A, int64_t arm, uint32_t *F) {
+// hint (optional) receives the saddle's other shell points, kInvalid padded:
+// the likely support of the facet's miniball.
+__device__ inline void arm_facet(const ArmParams &A, int64_t arm, uint32_t *F, uint32_t *hint = nullptr) {
   const uint32_t s = A.arm_saddle[arm];
   const int j = (int)(arm - A.arm_base[s]);
   const EventRec &e = A.ev[A.order[A.saddle_pos[s]]];
@@ -256,13 +286,19 @@ __device__ inline void arm_facet(const ArmParams &A, int64_t arm, uint32_t *F) {
   int f = 0;
   for (int i = 0; i < len; ++i)
     if (L[i] != e.ids[j]) F[f++] = L[i];
+  if (hint) {
+    const int m = e.meta & 0xff;
+    int h = 0;
+    for (int q = 0; q < m; ++q)
+      if (q != j) hint[h++] = e.ids[q];
+    while (h < 4) hint[h++] = kInvalid;
+  }
 }
 
 // One thread per strict arm: the facet is a minimum (hash hit) or the arm
 // is queued for descents.  Splitting the two keeps the descending lanes of a
-// wave together.
-// Queued arms carry the Morton position of their facet's least point, so the
-// descents can run in spatial order.
+// wave together.  Queued arms carry the Morton position of their facet's
+// least point, so the descents can run in spatial order.
 __global__ void k_arm_lookup(ArmParams A, uint32_t *queue, uint64_t *qkey, unsigned long long *qcount) {
   const int64_t arm = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool descend = false;
@@ -302,8 +338,8 @@ __global__ void __launch_bounds__(kArmBlock) k_resolve_arms(ArmParams A, const u
   if (qi >= nq) return;
   const int64_t arm = queue[qi];
   const int k = A.k;
-  uint32_t F[kMaxLabel];
-  arm_facet(A, arm, F);
+  uint32_t F[kMaxLabel], hint[4];
+  arm_facet(A, arm, F, hint);
   uint32_t result = kInvalid;
   uint64_t local_steps = 0;
   for (int step = 0; step < 4096 && result == kInvalid; ++step) {
@@ -311,13 +347,29 @@ __global__ void __launch_bounds__(kArmBlock) k_resolve_arms(ArmParams A, const u
       const int64_t hit = lookup(F, k, A.table_keys, A.table_vals, A.mask, A.labels);
       if (hit >= 0) { result = (uint32_t)hit; break; }
     }
-    // miniball of the facet
+    // miniball of the facet; the likely support goes first, so that Welzl's
+    // outer loop mostly meets points already inside
+    uint32_t ord[kMaxLabel];
+    int no = 0;
+    for (int h = 0; h < 4; ++h) {
+      bool in = hint[h] != kInvalid;
+      if (in) {
+        in = false;
+        for (int i = 0; i < k; ++i) in |= F[i] == hint[h];
+      }
+      if (in) ord[no++] = hint[h];
+    }
+    for (int i = 0; i < k; ++i) {
+      bool hinted = false;
+      for (int h = 0; h < 4; ++h) hinted |= hint[h] == F[i];
+      if (!hinted) ord[no++] = F[i];
+    }
     V3 pts[kMaxLabel];
-    for (int i = 0; i < k; ++i) pts[i] = P3(A.X, A.Y, A.Z, F[i]);
+    for (int i = 0; i < k; ++i) pts[i] = P3(A.X, A.Y, A.Z, ord[i]);
     Ball b;
     if (!welzl(pts, 1, k, b)) break;
     uint32_t sid[4];
-    for (int i = 0; i < b.m; ++i) sid[i] = F[b.sup[i]];
+    for (int i = 0; i < b.m; ++i) sid[i] = ord[b.sup[i]];
     SupSphere S;
     if (S.init(A.X, A.Y, A.Z, sid, b.m) <= 0) break;
     // every facet point must be certainly inside, or a support point
@@ -333,14 +385,27 @@ __global__ void __launch_bounds__(kArmBlock) k_resolve_arms(ArmParams A, const u
     bool undecided = false;
     const double q[3] = {b.c.x, b.c.y, b.c.z};
     const double R2 = b.r2 * (1.0 + 1e-9) + 1e-300;
+    // FP32 pruning against outward-rounded child boxes: the radius is
+    // inflated by the rounding of the query point and of the FP32 arithmetic
+    // (a non-finite bound prunes nothing)
+    const float qf[3] = {(float)q[0], (float)q[1], (float)q[2]};
+    const double qmax = fmax(fmax(fabs(q[0]), fabs(q[1])), fabs(q[2]));
+    const double rb = sqrt(R2) * (1.0 + 1e-6) + qmax * 1e-6 + 1e-30;
+    float Tf = (float)(rb * rb * (1.0 + 1e-5));
+    if (!(Tf < 3.0e38f)) Tf = INFINITY;
     int sp = 0;
     stk[lane] = 0;
     sp = 1;
     while (sp && !undecided) {
-      const Node &N = A.nodes[stk[(--sp) * kArmBlock + lane]];
-      const uint32_t ch[2] = {N.left, N.right};
+      const WideNode &N = A.wide[stk[(--sp) * kArmBlock + lane]];
       for (int c = 0; c < 2; ++c) {
-        const uint32_t x = ch[c];
+        float d2 = 0.f;
+        for (int a = 0; a < 3; ++a) {
+          const float d = fmaxf(fmaxf(N.lo[c][a] - qf[a], qf[a] - N.hi[c][a]), 0.f);
+          d2 += d * d;
+        }
+        if (d2 > Tf) continue;
+        const uint32_t x = N.child[c];
         if (x & kLeafBit) {
           const uint32_t lp = x & ~kLeafBit;
           const double dx = A.LX[lp] - q[0], dy = A.LY[lp] - q[1], dz = A.LZ[lp] - q[2];
@@ -352,9 +417,10 @@ __global__ void __launch_bounds__(kArmBlock) k_resolve_arms(ArmParams A, const u
           const int sd = S.side(A.X, A.Y, A.Z, id);
           if (sd < 0) xin = min(xin, id);
           else if (sd == 0) undecided = true;
-        } else if (box_d2(q, A.nodes[x].lo, A.nodes[x].hi) <= R2) {
-          if (sp < kArmStack) stk[(sp++) * kArmBlock + lane] = x;
-          else undecided = true;
+        } else if (sp < kArmStack) {
+          stk[(sp++) * kArmBlock + lane] = x;
+        } else {
+          undecided = true;
         }
       }
     }
@@ -372,6 +438,8 @@ __global__ void __launch_bounds__(kArmBlock) k_resolve_arms(ArmParams A, const u
     }
     if (!placed) G[w++] = xin;
     for (int i = 0; i < k; ++i) F[i] = G[i];
+    // the next miniball's likely support: this one's, y traded for xin
+    for (int h = 0; h < 4; ++h) hint[h] = h < b.m ? (sid[h] == y ? xin : sid[h]) : kInvalid;
     local_steps++;
   }
   A.carrier[arm] = result;
@@ -689,6 +757,11 @@ void tower_rank_index(Context &c) {
   TowerWork &W = *c.tower_work;
   std::fill(W.rank_lo, W.rank_lo + 256, 0u);
   std::fill(W.rank_hi, W.rank_hi + 256, 0u);
+  const int64_t internal = std::max<int64_t>(c.n - 1, 0);
+  W.wide.ensure(internal);
+  if (internal)
+    hipLaunchKernelGGL(k_wide_nodes, dim3(blocks_for(internal, 256)), dim3(256), 0, s, internal, c.nodes.p, c.LX.p,
+                       c.LY.p, c.LZ.p, W.wide.p);
   if (n == 0) return;
   W.rank_keys.ensure(n);
   W.rank_vals.ensure(n);
@@ -754,7 +827,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   A.arm_saddle = arm_saddle.p; A.arms = arms; A.k = k;
   A.table_keys = tkeys.p; A.table_vals = tvals.p; A.mask = cap - 1; A.labels = labels.p;
   A.X = c.X.p; A.Y = c.Y.p; A.Z = c.Z.p;
-  A.nodes = c.nodes.p; A.LX = c.LX.p; A.LY = c.LY.p; A.LZ = c.LZ.p; A.leaf_id = c.leaf_id.p; A.leaf_pos = c.leaf_pos.p; A.n = c.n;
+  A.wide = W.wide.p; A.LX = c.LX.p; A.LY = c.LY.p; A.LZ = c.LZ.p; A.leaf_id = c.leaf_id.p; A.leaf_pos = c.leaf_pos.p; A.n = c.n;
   A.carrier = carrier.p;
   A.steps = (unsigned long long *)steps.p;
   auto &queue = W.queue;

@@ -25,6 +25,13 @@ struct TowerOrderData {
   int64_t steps = 0;
 };
 
+// LBVH internal node for FP32 ball queries: both children's boxes, rounded
+// outward, and the children as in Node::left/right.
+struct alignas(64) WideNode {
+  float lo[2][3], hi[2][3];
+  uint32_t child[2];
+};
+
 // Device scratch of the per-order tower work, kept by the context; a buffer
 // is reallocated only when an order needs more than it holds.
 struct TowerWork {
@@ -32,6 +39,7 @@ struct TowerWork {
   DevBuf<uint64_t> rank_keys;
   DevBuf<uint32_t> rank_vals, rank_bounds;
   uint32_t rank_lo[256] = {}, rank_hi[256] = {};
+  DevBuf<WideNode> wide;
   DevBuf<uint32_t> tvals, labels;
   DevBuf<uint64_t> tkeys, steps, qcount, qkey;
   DevBuf<uint32_t> sizes, carrier, scls, base, arm_saddle, queue;
@@ -49,7 +57,7 @@ struct TowerWork {
                                       &changed, &err, &live_n, &Lu[0], &Lu[1], &Lv[0], &Lv[1], &Li[0], &Li[1],
                                       &attach, &rep, &dense, &slab, &ssup})
       b += (int64_t)x->bytes();
-    return b + (int64_t)(rank_keys.bytes() + qkey.bytes() + tkeys.bytes() + steps.bytes() + qcount.bytes() + slev.bytes() + c_lev.bytes());
+    return b + (int64_t)(wide.bytes() + rank_keys.bytes() + qkey.bytes() + tkeys.bytes() + steps.bytes() + qcount.bytes() + slev.bytes() + c_lev.bytes());
   }
 };
 
