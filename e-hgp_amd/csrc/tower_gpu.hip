@@ -53,30 +53,46 @@ __device__ inline int event_label(const EventRec &e, uint32_t *out) {
 
 __device__ inline int rank_of(const EventRec &e) { return (e.meta >> 8) & 0xff; }
 
-// FP32 copy of the LBVH for ball queries: each internal node holds both
-// children's boxes rounded outward (a leaf's box is its point).
+// FP32 copy of the LBVH for ball queries: each internal node lists its
+// grandchildren (a leaf child stands for itself) with boxes rounded outward
+// (a leaf's box is its point).
+__device__ inline void wide_entry(const Node *nodes, const double *LX, const double *LY, const double *LZ,
+                                  uint32_t x, WideNode &w, int slot) {
+  double lo[3], hi[3];
+  if (x & kLeafBit) {
+    const uint32_t lp = x & ~kLeafBit;
+    lo[0] = hi[0] = LX[lp];
+    lo[1] = hi[1] = LY[lp];
+    lo[2] = hi[2] = LZ[lp];
+  } else {
+    for (int a = 0; a < 3; ++a) { lo[a] = nodes[x].lo[a]; hi[a] = nodes[x].hi[a]; }
+  }
+  for (int a = 0; a < 3; ++a) {
+    w.lo[slot][a] = __double2float_rd(lo[a]);
+    w.hi[slot][a] = __double2float_ru(hi[a]);
+  }
+  w.child[slot] = x;
+}
+
 __global__ void k_wide_nodes(int64_t m, const Node *__restrict__ nodes, const double *__restrict__ LX,
                              const double *__restrict__ LY, const double *__restrict__ LZ, WideNode *out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const Node &N = nodes[i];
   WideNode w;
+  int slot = 0;
   for (int c = 0; c < 2; ++c) {
     const uint32_t x = c ? N.right : N.left;
-    double lo[3], hi[3];
     if (x & kLeafBit) {
-      const uint32_t lp = x & ~kLeafBit;
-      lo[0] = hi[0] = LX[lp];
-      lo[1] = hi[1] = LY[lp];
-      lo[2] = hi[2] = LZ[lp];
+      wide_entry(nodes, LX, LY, LZ, x, w, slot++);
     } else {
-      for (int a = 0; a < 3; ++a) { lo[a] = nodes[x].lo[a]; hi[a] = nodes[x].hi[a]; }
+      wide_entry(nodes, LX, LY, LZ, nodes[x].left, w, slot++);
+      wide_entry(nodes, LX, LY, LZ, nodes[x].right, w, slot++);
     }
-    for (int a = 0; a < 3; ++a) {
-      w.lo[c][a] = __double2float_rd(lo[a]);
-      w.hi[c][a] = __double2float_ru(hi[a]);
-    }
-    w.child[c] = x;
+  }
+  for (; slot < 4; ++slot) {  // empty slot: a box no query meets
+    for (int a = 0; a < 3; ++a) { w.lo[slot][a] = INFINITY; w.hi[slot][a] = -INFINITY; }
+    w.child[slot] = kInvalid;
   }
   out[i] = w;
 }
@@ -398,14 +414,14 @@ __global__ void __launch_bounds__(kArmBlock) k_resolve_arms(ArmParams A, const u
     sp = 1;
     while (sp && !undecided) {
       const WideNode &N = A.wide[stk[(--sp) * kArmBlock + lane]];
-      for (int c = 0; c < 2; ++c) {
+      for (int c = 0; c < 4; ++c) {
         float d2 = 0.f;
         for (int a = 0; a < 3; ++a) {
           const float d = fmaxf(fmaxf(N.lo[c][a] - qf[a], qf[a] - N.hi[c][a]), 0.f);
           d2 += d * d;
         }
-        if (d2 > Tf) continue;
         const uint32_t x = N.child[c];
+        if (d2 > Tf || x == kInvalid) continue;
         if (x & kLeafBit) {
           const uint32_t lp = x & ~kLeafBit;
           const double dx = A.LX[lp] - q[0], dy = A.LY[lp] - q[1], dz = A.LZ[lp] - q[2];

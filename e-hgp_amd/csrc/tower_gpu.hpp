@@ -25,12 +25,14 @@ struct TowerOrderData {
   int64_t steps = 0;
 };
 
-// LBVH internal node for FP32 ball queries: both children's boxes, rounded
-// outward, and the children as in Node::left/right.
-struct alignas(64) WideNode {
-  float lo[2][3], hi[2][3];
-  uint32_t child[2];
+// LBVH internal node for FP32 ball queries, two levels collapsed into one
+// 128-byte line: the node's grandchildren (or a child that is a leaf), their
+// boxes rounded outward, children as in Node::left/right (kInvalid: none).
+struct alignas(128) WideNode {
+  float lo[4][3], hi[4][3];
+  uint32_t child[4];
 };
+static_assert(sizeof(WideNode) == 128, "wide node layout");
 
 // Device scratch of the per-order tower work, kept by the context; a buffer
 // is reallocated only when an order needs more than it holds.
