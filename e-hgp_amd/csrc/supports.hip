@@ -520,7 +520,13 @@ __device__ void tet_item(const Params &P, const Work &w, uint32_t u, uint32_t M,
 
 // Cell direction and angular radius in FP32, rounded so that the cell only
 // grows (cos delta down, sin delta up): every FP32 test below errs towards a
-// larger certificate radius and a larger region.
+// larger certificate radius and a larger region.  Reciprocals and square
+// roots use the hardware approximations (1 ulp); their error, like the
+// rounding of the products, stays far inside the slack of the tests (1e-5 on
+// the mask cosines, 1e-4 on the certificate radius; cancellation is bounded
+// by skipping covers with cos(theta + delta) < 0.01).  Lengths below kTinyLen
+// (where the approximations flush) take the conservative branch.
+constexpr float kTinyLen = 1e-30f;
 struct CellF {
   float dx, dy, dz, cosd, sind;
 };
@@ -536,18 +542,18 @@ __device__ inline CellF cell_f(const double *cd) {
 __device__ bool member_mask(const Params &P, const Work &w, uint32_t i, uint64_t mask[6]) {
   const float4 q = w.fc[i];
   for (int k = 0; k < 6; ++k) mask[k] = 0;
-  if (!(q.w > 0.f)) {  // distance lost to underflow: member everywhere
+  if (!(q.w > kTinyLen)) {  // distance lost to underflow: member everywhere
     for (int k = 0; k < 6; ++k) mask[k] = ~0ull;
     return true;
   }
-  const float inv = 1.0f / q.w;
+  const float inv = __builtin_amdgcn_rcpf(q.w);
   bool any = false;
   for (int c = 0; c < kCells; ++c) {
     const CellF cf = cell_f(P.cells + 8 * c);
     const float ct = (q.x * cf.dx + q.y * cf.dy + q.z * cf.dz) * inv;
     float cm;  // cos(max(theta - delta, 0))
     if (ct >= cf.cosd) cm = 1.0f;
-    else cm = ct * cf.cosd + sqrtf(fmaxf(0.0f, 1.0f - ct * ct)) * cf.sind;
+    else cm = ct * cf.cosd + __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - ct * ct)) * cf.sind;
     const float rho = (float)w.rho[c];
     if (q.w <= 2.0f * rho * (cm + 1e-5f) * (1.0f + 1e-5f)) {
       mask[c >> 6] |= 1ull << (c & 63);
@@ -632,13 +638,13 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     for (int j = 0; j < kCoverCap; ++j) best[j] = INFINITY;
     for (uint32_t i = 0; i < G; ++i) {
       const float4 q = w.fc[i];
-      if (!(q.w > 0.f)) continue;  // skipping a point only enlarges the radius
-      const float ct = (q.x * cf.dx + q.y * cf.dy + q.z * cf.dz) / q.w;
+      if (!(q.w > kTinyLen)) continue;  // skipping a point only enlarges the radius
+      const float ct = (q.x * cf.dx + q.y * cf.dy + q.z * cf.dz) * __builtin_amdgcn_rcpf(q.w);
       if (ct <= 0.0f) continue;
-      const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+      const float st = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
       const float cs = ct * cf.cosd - st * cf.sind;  // cos(theta + delta)
       if (cs < 0.01f) continue;  // ill-conditioned cover: skipped
-      const float r = q.w / (2.0f * cs);
+      const float r = q.w * __builtin_amdgcn_rcpf(2.0f * cs);
       if (!(r < best[kCoverCap - 1])) continue;
 #pragma unroll
       for (int j = kCoverCap - 1; j > 0; --j) best[j] = r < best[j - 1] ? best[j - 1] : (r < best[j] ? r : best[j]);
