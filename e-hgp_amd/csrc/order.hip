@@ -12,6 +12,7 @@
 #include <numeric>
 #include <thread>
 
+#include "wave.hpp"
 #include "pipeline.hpp"
 #include "scan.hpp"
 #include "level_dd.hpp"
@@ -44,12 +45,10 @@ __global__ void k_links(const EventRec *__restrict__ ev, const uint32_t *__restr
 // Start of every run of records whose mutual order the FP64 intervals leave open.
 __global__ void k_run_starts(const uint32_t *__restrict__ link, int64_t n, uint32_t *starts,
                              unsigned long long *count, uint64_t cap) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i + 1 >= n) return;
-  if (link[i] && !link[i + 1]) {
-    unsigned long long s = atomicAdd(count, 1ull);
-    if (s < cap) starts[s] = (uint32_t)i;
-  }
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool start = i + 1 < n && link[i] && !link[i + 1];
+  const unsigned long long s = wave_append(start, count);
+  if (start && s < cap) starts[s] = (uint32_t)i;
 }
 
 constexpr int kRunMax = 32;

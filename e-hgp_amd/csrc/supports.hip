@@ -35,6 +35,7 @@
 #include <cmath>
 #include <numeric>
 
+#include "wave.hpp"
 #include "pipeline.hpp"
 #include "predicates.hpp"
 #include "supports.hpp"
@@ -101,13 +102,14 @@ struct Work {
   uint64_t *rmask;  // 6 words per member
   uint32_t *kidx;
   float4 *fc;       // FP32 offset and distance per gathered point (certificate and masks)
+  uint32_t *cand;   // candidate supports classified (LDS, flushed once per anchor)
 };
 
 __device__ inline V3 gpt(const Work &w, uint32_t i) { return {w.gx[i], w.gy[i], w.gz[i]}; }
 
 __device__ void emit_event(const Params &P, uint32_t u, const uint32_t *sup, int m, int nint,
                            const uint32_t *interior, double level, double lerr) {
-  unsigned long long slot = atomicAdd(&P.counters[0], 1ull);
+  const unsigned long long slot = wave_append(true, &P.counters[0]);
   if (slot >= P.event_cap) return;
   EventRec &e = P.events[slot];
   e.ids[0] = u;
@@ -185,7 +187,7 @@ __device__ void process_pair(const Params &P, const Work &w, uint32_t u, uint32_
   int r = classify(P, w, M, sph, skip, 2, interior, nint);
   uint32_t vid = w.gid[vi];
   if (r == 0) return;
-  atomicAdd(&P.counters[3], 1ull);
+  atomicAdd(w.cand, 1u);
   if (r < 0) {
     emit_fallback(P, w, u, 2, &vid, 2, M, region_slot);
     return;
@@ -200,7 +202,7 @@ __device__ void process_tet(const Params &P, const Work &w, uint32_t u, uint32_t
   int wc = t.init(f.a, f.b, gpt(w, zi));
   uint32_t ids[3] = {w.gid[vi], w.gid[wi], w.gid[zi]};
   if (wc < 0) return;
-  atomicAdd(&P.counters[3], 1ull);
+  atomicAdd(w.cand, 1u);
   if (wc == 0) { emit_fallback(P, w, u, 4, ids, 4, M, region_slot); return; }
   uint32_t interior[kMaxInterior];
   int nint;
@@ -473,7 +475,7 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
   }
   // triangle event
   if (wc > 0 && total_in + 3 <= smax) {
-    atomicAdd(&P.counters[3], 1ull);
+    atomicAdd(w.cand, 1u);
     sort_small(tri_int, total_in);
     emit_event(P, u, ids, 3, total_in, tri_int, f.level(), f.level_err());
   } else if (wc == 0 && total_in + 3 <= smax) {
@@ -799,7 +801,10 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     wave_sync();
   }
   __syncthreads();
-  if (t == 0) P.status[u] = ST_DONE;
+  if (t == 0) {
+    if (*w.cand) atomicAdd(&P.counters[3], (unsigned long long)*w.cand);
+    P.status[u] = ST_DONE;
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_anchor_lds(Params P) {
@@ -819,7 +824,10 @@ __global__ void __launch_bounds__(kBlock) k_anchor_lds(Params P) {
     if (threadIdx.x == 0) P.status[u] = ST_BIG;
     return;
   }
-  Work w{gx, gy, gz, gid, rho, ridx, rmask, kidx, reinterpret_cast<float4 *>(rmask)};
+  __shared__ uint32_t cand;
+  if (threadIdx.x == 0) cand = 0;
+  __syncthreads();
+  Work w{gx, gy, gz, gid, rho, ridx, rmask, kidx, reinterpret_cast<float4 *>(rmask), &cand};
   anchor_work<false>(P, w, u);
 }
 
@@ -846,6 +854,10 @@ __global__ void __launch_bounds__(kBlock) k_anchor_big(Params P) {
   w.ridx = w.gid + G;
   w.kidx = w.ridx + G;
   w.fc = reinterpret_cast<float4 *>(slab + big_fc_offset(G));
+  __shared__ uint32_t cand;
+  if (threadIdx.x == 0) cand = 0;
+  __syncthreads();
+  w.cand = &cand;
   anchor_work<true>(P, w, u);
 }
 
