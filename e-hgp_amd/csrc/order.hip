@@ -12,7 +12,6 @@
 #include <numeric>
 #include <thread>
 
-#include "wave.hpp"
 #include "pipeline.hpp"
 #include "scan.hpp"
 #include "level_dd.hpp"
@@ -43,12 +42,26 @@ __global__ void k_links(const EventRec *__restrict__ ev, const uint32_t *__restr
 }
 
 // Start of every run of records whose mutual order the FP64 intervals leave open.
-__global__ void k_run_starts(const uint32_t *__restrict__ link, int64_t n, uint32_t *starts,
-                             unsigned long long *count, uint64_t cap) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool start = i + 1 < n && link[i] && !link[i + 1];
-  const unsigned long long s = wave_append(start, count);
-  if (start && s < cap) starts[s] = (uint32_t)i;
+// Each block takes kRunTile consecutive records and appends its starts with
+// one global atomic (runs are independent, their list order is free).
+constexpr int kRunTile = 16 * 256;
+__global__ void __launch_bounds__(256) k_run_starts(const uint32_t *__restrict__ link, int64_t n, uint32_t *starts,
+                                                    unsigned long long *count, uint64_t cap) {
+  __shared__ uint32_t local[kRunTile];
+  __shared__ uint32_t nlocal;
+  __shared__ unsigned long long base;
+  if (threadIdx.x == 0) nlocal = 0;
+  __syncthreads();
+  const int64_t tile = (int64_t)blockIdx.x * kRunTile;
+  for (int j = threadIdx.x; j < kRunTile; j += blockDim.x) {
+    const int64_t i = tile + j;
+    if (i + 1 < n && link[i] && !link[i + 1]) local[atomicAdd(&nlocal, 1u)] = (uint32_t)i;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) base = nlocal ? atomicAdd(count, (unsigned long long)nlocal) : 0ull;
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < nlocal; j += blockDim.x)
+    if (base + j < cap) starts[base + j] = local[j];
 }
 
 constexpr int kRunMax = 32;
@@ -191,7 +204,7 @@ void order_catalogue(Context &c, CatalogBuffers &B) {
   DevBuf<uint64_t> cnt;
   cnt.ensure(2);
   MHGP_HIP(hipMemsetAsync(cnt.p, 0, 16, s));
-  hipLaunchKernelGGL(k_run_starts, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.cls.p, n, starts,
+  hipLaunchKernelGGL(k_run_starts, dim3(blocks_for(n, kRunTile)), dim3(256), 0, s, B.cls.p, n, starts,
                      (unsigned long long *)cnt.p, cap);
   uint64_t runs = 0;
   copy_wait(&runs, cnt.p, 8, hipMemcpyDeviceToHost, s);

@@ -33,10 +33,7 @@ struct Request {
   int64_t predicted;
   Clock::time_point arrival, start, end;
   mhgp_request_result result{};
-  std::vector<int32_t> order, kind;
-  std::vector<int64_t> level_class;
-  std::vector<double> level;
-  std::vector<int64_t> child_offset, children;
+  TowerState tower;  // the request's forests, concatenated by fetch
   bool done = false;
 };
 
@@ -105,13 +102,8 @@ void Server::worker_main() {
       supports(c, r->k_max, &ci);
       mhgp_tower_info ti{};
       tower(c, r->k_max, &ti);
-      const TowerState &T = *c.tower;
-      r->order = T.order;
-      r->kind = T.kind;
-      r->level_class = T.level_class;
-      r->level = T.level;
-      r->child_offset = T.child_offset;
-      r->children = T.children;
+      r->tower = std::move(*c.tower);  // the context's tower is rebuilt by its next request
+      c.tower->valid = false;
       r->result.event_count = ci.event_count;
       r->result.node_count = ti.node_count;
       r->result.child_count = ti.child_count;
@@ -212,12 +204,7 @@ int mhgp_server_fetch(mhgp_server *srv, int64_t ticket, int32_t *order, int32_t 
   auto it = srv->s.all.find(ticket);
   if (it == srv->s.all.end() || !it->second->done) return MHGP_E_STATE;
   auto &r = *it->second;
-  std::copy(r.order.begin(), r.order.end(), order);
-  std::copy(r.kind.begin(), r.kind.end(), kind);
-  std::copy(r.level_class.begin(), r.level_class.end(), level_class);
-  std::copy(r.level.begin(), r.level.end(), level);
-  std::copy(r.child_offset.begin(), r.child_offset.end(), child_offset);
-  std::copy(r.children.begin(), r.children.end(), children);
+  r.tower.copy_out(order, kind, level_class, level, child_offset, children);
   return MHGP_OK;
 }
 

@@ -257,7 +257,7 @@ inline uint32_t cell_find(std::vector<Cell> &c, uint32_t x) {
   return x;
 }
 
-void contract_order(int k, int64_t n, TowerOrderData &D, TowerState &F) {
+void contract_order(int k, int64_t n, TowerOrderData &D, Forest &F) {
   const bool timing = std::getenv("MHGP_TIMING") != nullptr;
   const auto t0 = std::chrono::steady_clock::now();
   // carrier union-find; each class may own a reduced root
@@ -380,7 +380,7 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
   std::unique_ptr<Descender> desc;  // host exact descents, built on demand
   std::vector<double> xyz;
   std::vector<TowerOrderData> Ds(K);
-  std::vector<TowerState> forests(K);
+  std::vector<Forest> forests(K);
   std::vector<std::exception_ptr> errors(K);
   std::vector<std::thread> workers;
   struct Joiner {
@@ -438,24 +438,18 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
   for (auto &e : errors)
     if (e) std::rethrow_exception(e);
   tm.mark("tower forests (host)");
-  // concatenate the orders; child ids move by the nodes of the orders below
-  for (int k = 1; k <= K; ++k) {
-    const TowerState &F = forests[k - 1];
-    const int64_t base = (int64_t)T.order.size();
-    const int64_t cbase = (int64_t)T.children.size();
-    T.order.insert(T.order.end(), F.order.begin(), F.order.end());
-    T.kind.insert(T.kind.end(), F.kind.begin(), F.kind.end());
-    T.level_class.insert(T.level_class.end(), F.level_class.begin(), F.level_class.end());
-    T.level.insert(T.level.end(), F.level.begin(), F.level.end());
-    for (size_t i = 1; i < F.child_offset.size(); ++i) T.child_offset.push_back(cbase + F.child_offset[i]);
-    for (int64_t x : F.children) T.children.push_back(base + x);
+  // the orders stay separate; fetches concatenate them once
+  for (const Forest &F : forests) {
+    T.node_count += (int64_t)F.order.size();
+    T.child_count += (int64_t)F.children.size();
   }
+  T.forests = std::move(forests);
   T.descent_steps += desc ? desc->steps : 0;
   T.orders = K;
   T.status = 0;
   T.valid = true;
-  info->node_count = (int64_t)T.order.size();
-  info->child_count = (int64_t)T.children.size();
+  info->node_count = T.node_count;
+  info->child_count = T.child_count;
   info->descent_steps = T.descent_steps;
   info->orders = T.orders;
   info->status = T.status;
@@ -464,13 +458,7 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
 void tower_fetch(Context &c, int32_t *order, int32_t *kind, int64_t *level_class, double *level,
                  int64_t *child_offset, int64_t *children) {
   if (!c.tower || !c.tower->valid) throw StateError("no tower has been built");
-  const TowerState &T = *c.tower;
-  std::copy(T.order.begin(), T.order.end(), order);
-  std::copy(T.kind.begin(), T.kind.end(), kind);
-  std::copy(T.level_class.begin(), T.level_class.end(), level_class);
-  std::copy(T.level.begin(), T.level.end(), level);
-  std::copy(T.child_offset.begin(), T.child_offset.end(), child_offset);
-  std::copy(T.children.begin(), T.children.end(), children);
+  c.tower->copy_out(order, kind, level_class, level, child_offset, children);
 }
 
 }  // namespace mhgp

@@ -2,22 +2,20 @@
 #pragma once
 
 #include <cstdint>
+#include <thread>
 #include <vector>
 
 #include "stages.hpp"
 
 namespace mhgp {
 
-struct TowerState {
-  bool valid = false;
+// Merge forest of one order; node ids and children are local to the order.
+struct Forest {
   std::vector<int32_t> order, kind;
   std::vector<int64_t> level_class;
   std::vector<double> level;
   std::vector<int64_t> child_offset{0};
   std::vector<int64_t> children;
-  int64_t descent_steps = 0;
-  int32_t orders = 0;
-  int32_t status = 0;
 
   int64_t add(int32_t o, int32_t k, int64_t cls, double lev, const int64_t *kids = nullptr, size_t nk = 0) {
     order.push_back(o);
@@ -38,6 +36,39 @@ struct TowerState {
     level.reserve(nodes);
     child_offset.reserve(nodes + 1);
     children.reserve(2 * nodes);
+  }
+};
+
+struct TowerState {
+  bool valid = false;
+  std::vector<Forest> forests;  // order k at index k - 1
+  int64_t node_count = 0, child_count = 0;
+  int64_t descent_steps = 0;
+  int32_t orders = 0;
+  int32_t status = 0;
+
+  // The tower as one node list: orders ascending, node ids and child offsets
+  // shifted by the nodes and children of the orders below (one thread per
+  // order, each writes its own ranges).
+  void copy_out(int32_t *order, int32_t *kind, int64_t *level_class, double *level, int64_t *child_offset,
+                int64_t *children) const {
+    child_offset[0] = 0;
+    std::vector<std::thread> th;
+    int64_t nb = 0, cb = 0;
+    for (const Forest &F : forests) {
+      th.emplace_back([&F, nb, cb, order, kind, level_class, level, child_offset, children] {
+        const size_t m = F.order.size();
+        std::copy(F.order.begin(), F.order.end(), order + nb);
+        std::copy(F.kind.begin(), F.kind.end(), kind + nb);
+        std::copy(F.level_class.begin(), F.level_class.end(), level_class + nb);
+        std::copy(F.level.begin(), F.level.end(), level + nb);
+        for (size_t i = 1; i <= m; ++i) child_offset[nb + (int64_t)i] = cb + F.child_offset[i];
+        for (size_t j = 0; j < F.children.size(); ++j) children[cb + (int64_t)j] = nb + F.children[j];
+      });
+      nb += (int64_t)F.order.size();
+      cb += (int64_t)F.children.size();
+    }
+    for (auto &t : th) t.join();
   }
 };
 
