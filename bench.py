@@ -87,10 +87,12 @@ def copy_calibration(mib: int) -> None:
     print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4, "launches": 3}))
 
 
-def served_stream(k_values=(4, 10), count=48, rate_hz=None, slots=4, seed=7):
+def served_stream(k_values=(4, 10), count=48, rate_mult=4.0, slots=4, seed=7):
     """Poisson stream of clouds (sizes log-uniform in [2 000, 50 000]) through
     the continuous-batching server, against the same clouds run one after
-    another on one context (the plain path)."""
+    another on one context (the plain path).  The offered rate is rate_mult
+    times the plain path's rate; served clouds/s is the number of clouds over
+    the span from the first arrival to the last completion."""
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(np.log(2_000), np.log(50_000), size=count)).astype(int)
     ks = rng.choice(np.asarray(k_values), size=count)
@@ -107,7 +109,7 @@ def served_stream(k_values=(4, 10), count=48, rate_hz=None, slots=4, seed=7):
     plain_s = time.perf_counter() - t0
     ctx.close()
     plain_rate = count / plain_s
-    rate = rate_hz or 0.9 * plain_rate * min(slots, 2)
+    rate = rate_mult * plain_rate
     gaps = rng.exponential(1.0 / rate, size=count)
     srv = mhgp.Server(0, slots=slots)
     warm = srv.submit(clouds[0], int(ks[0]))

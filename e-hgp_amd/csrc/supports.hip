@@ -807,7 +807,7 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_anchor_lds(Params P) {
+__global__ void __launch_bounds__(kBlock, 3) k_anchor_lds(Params P) {
   __shared__ double gx[kGatherCap], gy[kGatherCap], gz[kGatherCap];
   __shared__ uint32_t gid[kGatherCap];
   __shared__ double rho[kCells];

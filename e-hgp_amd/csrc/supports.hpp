@@ -8,8 +8,8 @@
 namespace mhgp {
 
 constexpr int kCells = 384;      // 6 cube faces x 8 x 8 direction cells
-constexpr int kGatherCap = 1024; // gathered neighbours held in LDS
-constexpr int kRegionCap = 640;  // region members held in LDS
+constexpr int kGatherCap = 600;  // gathered neighbours held in LDS
+constexpr int kRegionCap = 384;  // region members held in LDS
 constexpr int kMaxInterior = 10; // s_max - 2 for s_max <= 11 (K_max <= 10)
 
 // One catalogue record: support ids (unused = kInvalid), meta
