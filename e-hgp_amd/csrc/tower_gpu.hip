@@ -20,6 +20,7 @@
 #include "supports.hpp"
 #include "traverse.hpp"
 #include "tower_gpu.hpp"
+#include "wave.hpp"
 
 namespace mhgp {
 
@@ -311,13 +312,53 @@ __device__ inline void arm_facet(const ArmParams &A, int64_t arm, uint32_t *F, u
   }
 }
 
+// Facets to descend, deduplicated: a slot holds (32 hash bits | 1) << 32 |
+// arm; an arm whose facet equals a stored arm's takes that arm's carrier
+// after the descents.  The probe length is bounded; an arm that finds no slot
+// simply descends itself.
+struct FacetSet {
+  unsigned long long *slots;
+  uint64_t mask;
+  uint32_t *alias_arm, *alias_rep;
+  unsigned long long *alias_count;
+};
+
+__device__ inline bool same_facet(const ArmParams &A, int64_t other, const uint32_t *F) {
+  uint32_t G[kMaxLabel];
+  arm_facet(A, other, G);
+  bool same = true;
+  for (int q = 0; q < A.k; ++q) same &= G[q] == F[q];
+  return same;
+}
+
+// Returns the arm already holding this facet, or -1 (this arm descends).
+__device__ inline int64_t facet_claim(const ArmParams &A, const FacetSet &D, int64_t arm, const uint32_t *F) {
+  const uint64_t h = label_hash(F, A.k);
+  const unsigned long long tag = ((h >> 32) | 1ull) << 32;
+  const unsigned long long mine = tag | (unsigned long long)(uint32_t)arm;
+  uint64_t slot = h & D.mask;
+  for (int probe = 0; probe < 64; ++probe) {
+    unsigned long long cur = atomicCAS(&D.slots[slot], 0ull, mine);
+    if (cur == 0ull) return -1;
+    if ((cur & 0xffffffff00000000ull) == tag) {
+      const int64_t other = (int64_t)(cur & 0xffffffffull);
+      if (same_facet(A, other, F)) return other;
+    }
+    slot = (slot + 1) & D.mask;
+  }
+  return -1;
+}
+
 // One thread per strict arm: the facet is a minimum (hash hit) or the arm
-// is queued for descents.  Splitting the two keeps the descending lanes of a
-// wave together.  Queued arms carry the Morton position of their facet's
-// least point, so the descents can run in spatial order.
-__global__ void k_arm_lookup(ArmParams A, uint32_t *queue, uint64_t *qkey, unsigned long long *qcount) {
+// is queued for descents (once per distinct facet).  Splitting the two keeps
+// the descending lanes of a wave together.  Queued arms carry the Morton
+// position of their facet's least point, so the descents can run in spatial
+// order.
+__global__ void k_arm_lookup(ArmParams A, FacetSet D, uint32_t *queue, uint64_t *qkey,
+                             unsigned long long *qcount) {
   const int64_t arm = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool descend = false;
+  bool descend = false, aliased = false;
+  int64_t rep = -1;
   uint32_t F[kMaxLabel];
   if (arm < A.arms) {
     arm_facet(A, arm, F);
@@ -326,8 +367,17 @@ __global__ void k_arm_lookup(ArmParams A, uint32_t *queue, uint64_t *qkey, unsig
     } else {
       const int64_t hit = lookup(F, A.k, A.table_keys, A.table_vals, A.mask, A.labels);
       A.carrier[arm] = hit >= 0 ? (uint32_t)hit : kInvalid;
-      descend = hit < 0;
+      if (hit < 0) {
+        rep = facet_claim(A, D, arm, F);
+        aliased = rep >= 0;
+        descend = !aliased;
+      }
     }
+  }
+  const unsigned long long ai = wave_append(aliased, D.alias_count);
+  if (aliased) {
+    D.alias_arm[ai] = (uint32_t)arm;
+    D.alias_rep[ai] = (uint32_t)rep;
   }
   // wave-aggregated append to the descent queue
   const uint64_t mask = __ballot(descend);
@@ -342,6 +392,12 @@ __global__ void k_arm_lookup(ArmParams A, uint32_t *queue, uint64_t *qkey, unsig
     queue[pos] = (uint32_t)arm;
     qkey[pos] = A.leaf_pos[F[0]];
   }
+}
+
+__global__ void k_alias_carriers(const uint32_t *__restrict__ alias_arm, const uint32_t *__restrict__ alias_rep,
+                                 uint64_t count, uint32_t *carrier) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) carrier[alias_arm[i]] = carrier[alias_rep[i]];
 }
 
 // One thread per queued arm: certified descents until a minimum's label is
@@ -522,6 +578,11 @@ __global__ void k_saddle_labels(const EventRec *__restrict__ ev, const uint32_t 
 //    others (births, multifusions, shared levels) go to the host with their
 //    carriers replaced by representatives.
 __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u64(uint64_t *p, int64_t n, uint64_t v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
@@ -854,10 +915,23 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   c.arm_clock.start(s);
   auto &qkey = W.qkey;
   qkey.ensure(arms);
-  hipLaunchKernelGGL(k_arm_lookup, dim3(blocks_for(arms, 256)), dim3(256), 0, s, A, queue.p, qkey.p,
+  // distinct facets of the arms to descend (load factor below 1/2)
+  uint64_t fcap = 1;
+  while (fcap < 2ull * arms + 16) fcap <<= 1;
+  W.facet_slots.ensure(fcap);
+  W.alias_arm.ensure(arms);
+  W.alias_rep.ensure(arms);
+  W.alias_count.ensure(1);
+  hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for((int64_t)fcap, 256)), dim3(256), 0, s, W.facet_slots.p,
+                     (int64_t)fcap, 0ull);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, s, W.alias_count.p, (int64_t)1, 0ull);
+  const FacetSet D{(unsigned long long *)W.facet_slots.p, fcap - 1, W.alias_arm.p, W.alias_rep.p,
+                   (unsigned long long *)W.alias_count.p};
+  hipLaunchKernelGGL(k_arm_lookup, dim3(blocks_for(arms, 256)), dim3(256), 0, s, A, D, queue.p, qkey.p,
                      (unsigned long long *)qcount.p);
-  uint64_t nq = 0;
+  uint64_t nq = 0, na = 0;
   MHGP_HIP(hipMemcpyAsync(&nq, qcount.p, 8, hipMemcpyDeviceToHost, s));
+  MHGP_HIP(hipMemcpyAsync(&na, W.alias_count.p, 8, hipMemcpyDeviceToHost, s));
   MHGP_HIP(hipStreamSynchronize(s));
   if (nq > 1) {  // spatial order: neighbouring lanes walk neighbouring nodes
     int bits = 8;
@@ -867,6 +941,9 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   if (nq)
     hipLaunchKernelGGL(k_resolve_arms, dim3(blocks_for(nq, kArmBlock)), dim3(kArmBlock), 0, s, A, queue.p,
                        (uint32_t)nq);
+  if (na)
+    hipLaunchKernelGGL(k_alias_carriers, dim3(blocks_for(na, 256)), dim3(256), 0, s, W.alias_arm.p, W.alias_rep.p,
+                       na, carrier.p);
   c.arm_clock.stop(s);
   MHGP_LAUNCH_CHECK();
   c.stats.arm_ms += c.arm_clock.ms();

@@ -43,7 +43,8 @@ struct TowerWork {
   uint32_t rank_lo[256] = {}, rank_hi[256] = {};
   DevBuf<WideNode> wide;
   DevBuf<uint32_t> tvals, labels;
-  DevBuf<uint64_t> tkeys, steps, qcount, qkey;
+  DevBuf<uint64_t> tkeys, steps, qcount, qkey, facet_slots, alias_count;
+  DevBuf<uint32_t> alias_arm, alias_rep;
   DevBuf<uint32_t> sizes, carrier, scls, base, arm_saddle, queue;
   DevBuf<double> slev, c_lev;
   DevBuf<uint32_t> first, keep, kept, flag1, c_cls, c_base, c_car;
@@ -57,9 +58,9 @@ struct TowerWork {
                                       &base, &arm_saddle, &queue, &first, &keep, &kept, &flag1, &c_cls, &c_base,
                                       &c_car, &eflag, &eu, &ev, &es, &comp, &comp2, &best, &hook, &msf, &mcount,
                                       &changed, &err, &live_n, &Lu[0], &Lu[1], &Lv[0], &Lv[1], &Li[0], &Li[1],
-                                      &attach, &rep, &dense, &slab, &ssup})
+                                      &attach, &rep, &dense, &slab, &ssup, &alias_arm, &alias_rep})
       b += (int64_t)x->bytes();
-    return b + (int64_t)(wide.bytes() + rank_keys.bytes() + qkey.bytes() + tkeys.bytes() + steps.bytes() + qcount.bytes() + slev.bytes() + c_lev.bytes());
+    return b + (int64_t)(wide.bytes() + rank_keys.bytes() + qkey.bytes() + facet_slots.bytes() + alias_count.bytes() + tkeys.bytes() + steps.bytes() + qcount.bytes() + slev.bytes() + c_lev.bytes());
   }
 };
 
