@@ -400,122 +400,153 @@ __global__ void k_alias_carriers(const uint32_t *__restrict__ alias_arm, const u
   if (i < count) carrier[alias_arm[i]] = carrier[alias_rep[i]];
 }
 
-// One thread per queued arm: certified descents until a minimum's label is
-// found (the facet itself is not one).
-__global__ void __launch_bounds__(kArmBlock) k_resolve_arms(ArmParams A, const uint32_t *queue, uint32_t nq) {
-  // traversal stacks, interleaved by lane so equal depths hit distinct banks
+// One descent step of facet F (sorted ids) with the likely support hint:
+// F := F - y + x, where x is the least point strictly inside the facet's
+// miniball and y its least support point.  false: the certified filters
+// cannot decide the step (the host descends the arm).  stk is the lane's
+// traversal stack in LDS (stride kArmBlock).
+__device__ bool descent_step(const ArmParams &A, uint32_t *F, uint32_t *hint, uint32_t *stk, int lane) {
+  const int k = A.k;
+  // miniball of the facet; the likely support goes first, so that Welzl's
+  // outer loop mostly meets points already inside
+  uint32_t ord[kMaxLabel];
+  int no = 0;
+  for (int h = 0; h < 4; ++h) {
+    bool in = hint[h] != kInvalid;
+    if (in) {
+      in = false;
+      for (int i = 0; i < k; ++i) in |= F[i] == hint[h];
+    }
+    if (in) ord[no++] = hint[h];
+  }
+  for (int i = 0; i < k; ++i) {
+    bool hinted = false;
+    for (int h = 0; h < 4; ++h) hinted |= hint[h] == F[i];
+    if (!hinted) ord[no++] = F[i];
+  }
+  V3 pts[kMaxLabel];
+  for (int i = 0; i < k; ++i) pts[i] = P3(A.X, A.Y, A.Z, ord[i]);
+  Ball b;
+  if (!welzl(pts, 1, k, b)) return false;
+  uint32_t sid[4];
+  for (int i = 0; i < b.m; ++i) sid[i] = ord[b.sup[i]];
+  SupSphere S;
+  if (S.init(A.X, A.Y, A.Z, sid, b.m) <= 0) return false;
+  // every facet point must be certainly inside, or a support point
+  bool ok = true;
+  for (int i = 0; i < k && ok; ++i) {
+    bool on_support = false;
+    for (int q = 0; q < b.m; ++q) on_support |= sid[q] == F[i];
+    if (!on_support) ok = S.side(A.X, A.Y, A.Z, F[i]) < 0;
+  }
+  if (!ok) return false;
+  // closed-ball points outside the facet
+  uint32_t xin = kInvalid;
+  bool undecided = false;
+  const double q[3] = {b.c.x, b.c.y, b.c.z};
+  const double R2 = b.r2 * (1.0 + 1e-9) + 1e-300;
+  // FP32 pruning against outward-rounded child boxes: the radius is
+  // inflated by the rounding of the query point and of the FP32 arithmetic
+  // (a non-finite bound prunes nothing)
+  const float qf[3] = {(float)q[0], (float)q[1], (float)q[2]};
+  const double qmax = fmax(fmax(fabs(q[0]), fabs(q[1])), fabs(q[2]));
+  const double rb = sqrt(R2) * (1.0 + 1e-6) + qmax * 1e-6 + 1e-30;
+  float Tf = (float)(rb * rb * (1.0 + 1e-5));
+  if (!(Tf < 3.0e38f)) Tf = INFINITY;
+  int sp = 0;
+  stk[lane] = 0;
+  sp = 1;
+  while (sp && !undecided) {
+    const WideNode &N = A.wide[stk[(--sp) * kArmBlock + lane]];
+    for (int c = 0; c < 4; ++c) {
+      float d2 = 0.f;
+      for (int a = 0; a < 3; ++a) {
+        const float d = fmaxf(fmaxf(N.lo[c][a] - qf[a], qf[a] - N.hi[c][a]), 0.f);
+        d2 += d * d;
+      }
+      const uint32_t x = N.child[c];
+      if (d2 > Tf || x == kInvalid) continue;
+      if (x & kLeafBit) {
+        const uint32_t lp = x & ~kLeafBit;
+        const double dx = A.LX[lp] - q[0], dy = A.LY[lp] - q[1], dz = A.LZ[lp] - q[2];
+        if (dx * dx + dy * dy + dz * dz > R2) continue;
+        const uint32_t id = A.leaf_id[lp];
+        bool inF = false;
+        for (int i = 0; i < k; ++i) inF |= F[i] == id;
+        if (inF) continue;
+        const int sd = S.side(A.X, A.Y, A.Z, id);
+        if (sd < 0) xin = min(xin, id);
+        else if (sd == 0) undecided = true;
+      } else if (sp < kArmStack) {
+        stk[(sp++) * kArmBlock + lane] = x;
+      } else {
+        undecided = true;
+      }
+    }
+  }
+  if (undecided || xin == kInvalid) return false;  // host decides (shell points or a missing minimum)
+  uint32_t y = sid[0];
+  for (int i = 1; i < b.m; ++i) y = min(y, sid[i]);
+  // F := F - y + xin, kept sorted
+  int w = 0;
+  uint32_t G[kMaxLabel];
+  bool placed = false;
+  for (int i = 0; i < k; ++i) {
+    if (F[i] == y) continue;
+    if (!placed && xin < F[i]) { G[w++] = xin; placed = true; }
+    G[w++] = F[i];
+  }
+  if (!placed) G[w++] = xin;
+  for (int i = 0; i < k; ++i) F[i] = G[i];
+  // the next miniball's likely support: this one's, y traded for xin
+  for (int h = 0; h < 4; ++h) hint[h] = h < b.m ? (sid[h] == y ? xin : sid[h]) : kInvalid;
+  return true;
+}
+
+// Descents advance one step per round for every arm still descending, so the
+// lanes of a wave do equal work.  Item state lives in SoA arrays of
+// capacity cap: facet ids F[q * cap + i], hint H[h * cap + i], origin arm.
+struct DescentItems {
+  uint32_t *F, *H, *origin;
+  uint64_t cap;
+};
+
+// Round 0 takes the queued arms (their facets are not minima); later rounds
+// first look their facet up among the minima.
+__global__ void __launch_bounds__(kArmBlock) k_descend_round(ArmParams A, int round, const uint32_t *queue,
+                                                             DescentItems in, uint32_t n_in, DescentItems out,
+                                                             unsigned long long *n_out) {
   __shared__ uint32_t stk[kArmStack * kArmBlock];
   const int lane = threadIdx.x;
-  const int64_t qi = (int64_t)blockIdx.x * kArmBlock + lane;
-  if (qi >= nq) return;
-  const int64_t arm = queue[qi];
+  const uint64_t i = (uint64_t)blockIdx.x * kArmBlock + lane;
   const int k = A.k;
-  uint32_t F[kMaxLabel], hint[4];
-  arm_facet(A, arm, F, hint);
-  uint32_t result = kInvalid;
-  uint64_t local_steps = 0;
-  for (int step = 0; step < 4096 && result == kInvalid; ++step) {
-    if (step > 0) {
-      const int64_t hit = lookup(F, k, A.table_keys, A.table_vals, A.mask, A.labels);
-      if (hit >= 0) { result = (uint32_t)hit; break; }
+  bool next = false;
+  uint32_t F[kMaxLabel], hint[4], origin = 0;
+  if (i < n_in) {
+    if (round == 0) {
+      origin = queue[i];
+      arm_facet(A, origin, F, hint);
+    } else {
+      origin = in.origin[i];
+      for (int q = 0; q < k; ++q) F[q] = in.F[q * in.cap + i];
+      for (int h = 0; h < 4; ++h) hint[h] = in.H[h * in.cap + i];
     }
-    // miniball of the facet; the likely support goes first, so that Welzl's
-    // outer loop mostly meets points already inside
-    uint32_t ord[kMaxLabel];
-    int no = 0;
-    for (int h = 0; h < 4; ++h) {
-      bool in = hint[h] != kInvalid;
-      if (in) {
-        in = false;
-        for (int i = 0; i < k; ++i) in |= F[i] == hint[h];
-      }
-      if (in) ord[no++] = hint[h];
+    int64_t hit = -1;
+    if (round > 0) hit = lookup(F, k, A.table_keys, A.table_vals, A.mask, A.labels);
+    if (hit >= 0) {
+      A.carrier[origin] = (uint32_t)hit;
+    } else if (round >= 4096 || !descent_step(A, F, hint, stk, lane)) {
+      A.carrier[origin] = kInvalid;  // the host descends this arm
+    } else {
+      next = true;
     }
-    for (int i = 0; i < k; ++i) {
-      bool hinted = false;
-      for (int h = 0; h < 4; ++h) hinted |= hint[h] == F[i];
-      if (!hinted) ord[no++] = F[i];
-    }
-    V3 pts[kMaxLabel];
-    for (int i = 0; i < k; ++i) pts[i] = P3(A.X, A.Y, A.Z, ord[i]);
-    Ball b;
-    if (!welzl(pts, 1, k, b)) break;
-    uint32_t sid[4];
-    for (int i = 0; i < b.m; ++i) sid[i] = ord[b.sup[i]];
-    SupSphere S;
-    if (S.init(A.X, A.Y, A.Z, sid, b.m) <= 0) break;
-    // every facet point must be certainly inside, or a support point
-    bool ok = true;
-    for (int i = 0; i < k && ok; ++i) {
-      bool on_support = false;
-      for (int q = 0; q < b.m; ++q) on_support |= sid[q] == F[i];
-      if (!on_support) ok = S.side(A.X, A.Y, A.Z, F[i]) < 0;
-    }
-    if (!ok) break;
-    // closed-ball points outside the facet
-    uint32_t xin = kInvalid;
-    bool undecided = false;
-    const double q[3] = {b.c.x, b.c.y, b.c.z};
-    const double R2 = b.r2 * (1.0 + 1e-9) + 1e-300;
-    // FP32 pruning against outward-rounded child boxes: the radius is
-    // inflated by the rounding of the query point and of the FP32 arithmetic
-    // (a non-finite bound prunes nothing)
-    const float qf[3] = {(float)q[0], (float)q[1], (float)q[2]};
-    const double qmax = fmax(fmax(fabs(q[0]), fabs(q[1])), fabs(q[2]));
-    const double rb = sqrt(R2) * (1.0 + 1e-6) + qmax * 1e-6 + 1e-30;
-    float Tf = (float)(rb * rb * (1.0 + 1e-5));
-    if (!(Tf < 3.0e38f)) Tf = INFINITY;
-    int sp = 0;
-    stk[lane] = 0;
-    sp = 1;
-    while (sp && !undecided) {
-      const WideNode &N = A.wide[stk[(--sp) * kArmBlock + lane]];
-      for (int c = 0; c < 4; ++c) {
-        float d2 = 0.f;
-        for (int a = 0; a < 3; ++a) {
-          const float d = fmaxf(fmaxf(N.lo[c][a] - qf[a], qf[a] - N.hi[c][a]), 0.f);
-          d2 += d * d;
-        }
-        const uint32_t x = N.child[c];
-        if (d2 > Tf || x == kInvalid) continue;
-        if (x & kLeafBit) {
-          const uint32_t lp = x & ~kLeafBit;
-          const double dx = A.LX[lp] - q[0], dy = A.LY[lp] - q[1], dz = A.LZ[lp] - q[2];
-          if (dx * dx + dy * dy + dz * dz > R2) continue;
-          const uint32_t id = A.leaf_id[lp];
-          bool inF = false;
-          for (int i = 0; i < k; ++i) inF |= F[i] == id;
-          if (inF) continue;
-          const int sd = S.side(A.X, A.Y, A.Z, id);
-          if (sd < 0) xin = min(xin, id);
-          else if (sd == 0) undecided = true;
-        } else if (sp < kArmStack) {
-          stk[(sp++) * kArmBlock + lane] = x;
-        } else {
-          undecided = true;
-        }
-      }
-    }
-    if (undecided || xin == kInvalid) break;  // host decides (shell points or a missing minimum)
-    uint32_t y = sid[0];
-    for (int i = 1; i < b.m; ++i) y = min(y, sid[i]);
-    // F := F - y + xin, kept sorted
-    int w = 0;
-    uint32_t G[kMaxLabel];
-    bool placed = false;
-    for (int i = 0; i < k; ++i) {
-      if (F[i] == y) continue;
-      if (!placed && xin < F[i]) { G[w++] = xin; placed = true; }
-      G[w++] = F[i];
-    }
-    if (!placed) G[w++] = xin;
-    for (int i = 0; i < k; ++i) F[i] = G[i];
-    // the next miniball's likely support: this one's, y traded for xin
-    for (int h = 0; h < 4; ++h) hint[h] = h < b.m ? (sid[h] == y ? xin : sid[h]) : kInvalid;
-    local_steps++;
   }
-  A.carrier[arm] = result;
-  if (local_steps) atomicAdd(A.steps, (unsigned long long)local_steps);
+  const unsigned long long at = wave_append(next, n_out);
+  if (next) {
+    for (int q = 0; q < k; ++q) out.F[q * out.cap + at] = F[q];
+    for (int h = 0; h < 4; ++h) out.H[h * out.cap + at] = hint[h];
+    out.origin[at] = origin;
+  }
 }
 
 __global__ void k_arm_owner(const uint32_t *__restrict__ base, const uint32_t *__restrict__ sizes_incl,
@@ -938,9 +969,28 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
     while (bits < 32 && (1ll << bits) < c.n) bits += 8;
     c.sorter.sort(qkey.p, queue.p, (int64_t)nq, 0, bits, s);
   }
-  if (nq)
-    hipLaunchKernelGGL(k_resolve_arms, dim3(blocks_for(nq, kArmBlock)), dim3(kArmBlock), 0, s, A, queue.p,
-                       (uint32_t)nq);
+  uint64_t steps_total = 0;
+  if (nq) {
+    DescentItems items[2];
+    for (int q = 0; q < 2; ++q) {
+      W.dF[q].ensure((size_t)kMaxLabel * nq);
+      W.dH[q].ensure(4 * nq);
+      W.dO[q].ensure(nq);
+      items[q] = DescentItems{W.dF[q].p, W.dH[q].p, W.dO[q].p, nq};
+    }
+    W.dcount.ensure(1);
+    uint32_t active = (uint32_t)nq;
+    for (int round = 0; active > 0; ++round) {
+      const int q = round & 1;
+      hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, W.dcount.p, (int64_t)1, 0ull);
+      hipLaunchKernelGGL(k_descend_round, dim3(blocks_for(active, kArmBlock)), dim3(kArmBlock), 0, s, A, round,
+                         queue.p, items[q ^ 1], active, items[q], (unsigned long long *)W.dcount.p);
+      uint64_t nxt = 0;
+      copy_wait(&nxt, W.dcount.p, 8, hipMemcpyDeviceToHost, s);
+      steps_total += nxt;
+      active = (uint32_t)nxt;
+    }
+  }
   if (na)
     hipLaunchKernelGGL(k_alias_carriers, dim3(blocks_for(na, 256)), dim3(256), 0, s, W.alias_arm.p, W.alias_rep.p,
                        na, carrier.p);
@@ -1098,8 +1148,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
     MHGP_HIP(hipMemcpyAsync(h_car, d_car, (size_t)arms2 * 4, hipMemcpyDeviceToHost, s));
     MHGP_HIP(hipMemcpyAsync(h_lev, d_lev, (size_t)S2 * 8, hipMemcpyDeviceToHost, s));
   }
-  uint64_t st = 0;
-  MHGP_HIP(hipMemcpyAsync(&st, steps.p, 8, hipMemcpyDeviceToHost, s));
+  const uint64_t st = steps_total;
   MHGP_HIP(hipStreamSynchronize(s));
   h_base[S2] = arms2;
   out.saddles = S2;

@@ -44,7 +44,8 @@ struct TowerWork {
   DevBuf<WideNode> wide;
   DevBuf<uint32_t> tvals, labels;
   DevBuf<uint64_t> tkeys, steps, qcount, qkey, facet_slots, alias_count;
-  DevBuf<uint32_t> alias_arm, alias_rep;
+  DevBuf<uint32_t> alias_arm, alias_rep, dF[2], dH[2], dO[2];
+  DevBuf<uint64_t> dcount;
   DevBuf<uint32_t> sizes, carrier, scls, base, arm_saddle, queue;
   DevBuf<double> slev, c_lev;
   DevBuf<uint32_t> first, keep, kept, flag1, c_cls, c_base, c_car;
@@ -58,9 +59,9 @@ struct TowerWork {
                                       &base, &arm_saddle, &queue, &first, &keep, &kept, &flag1, &c_cls, &c_base,
                                       &c_car, &eflag, &eu, &ev, &es, &comp, &comp2, &best, &hook, &msf, &mcount,
                                       &changed, &err, &live_n, &Lu[0], &Lu[1], &Lv[0], &Lv[1], &Li[0], &Li[1],
-                                      &attach, &rep, &dense, &slab, &ssup, &alias_arm, &alias_rep})
+                                      &attach, &rep, &dense, &slab, &ssup, &alias_arm, &alias_rep, &dF[0], &dF[1], &dH[0], &dH[1], &dO[0], &dO[1]})
       b += (int64_t)x->bytes();
-    return b + (int64_t)(wide.bytes() + rank_keys.bytes() + qkey.bytes() + facet_slots.bytes() + alias_count.bytes() + tkeys.bytes() + steps.bytes() + qcount.bytes() + slev.bytes() + c_lev.bytes());
+    return b + (int64_t)(wide.bytes() + rank_keys.bytes() + qkey.bytes() + facet_slots.bytes() + alias_count.bytes() + dcount.bytes() + tkeys.bytes() + steps.bytes() + qcount.bytes() + slev.bytes() + c_lev.bytes());
   }
 };
 
