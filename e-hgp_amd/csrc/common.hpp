@@ -89,6 +89,15 @@ static_assert(sizeof(Node) == 64, "node layout");
 
 constexpr uint32_t kLeafBit = 0x80000000u;
 
+// LBVH internal node for FP32 ball queries, two levels collapsed into one
+// 128-byte line: the node's grandchildren (or a child that is a leaf), their
+// boxes rounded outward, children as in Node::left/right (kInvalid: none).
+struct alignas(128) WideNode {
+  float lo[4][3], hi[4][3];
+  uint32_t child[4];
+};
+static_assert(sizeof(WideNode) == 128, "wide node layout");
+
 __host__ __device__ inline uint64_t order_key(uint64_t b) {
   if (b == 0x8000000000000000ull) b = 0;  // -0 -> +0
   return (b >> 63) ? ~b : (b ^ 0x8000000000000000ull);

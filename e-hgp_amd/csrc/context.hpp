@@ -47,6 +47,7 @@ struct Context {
   DevBuf<Node> nodes;         // n-1 internal nodes, root = 0
   DevBuf<uint32_t> parent_int, parent_leaf;
   DevBuf<uint32_t> wit;       // 6 witnesses per internal node (lo xyz, hi xyz)
+  DevBuf<WideNode> wide;      // FP32 two-level copy of the internal nodes (ball and k-NN queries)
   DevBuf<uint32_t> counter;
   RadixSorter sorter;
   Scanner scanner;

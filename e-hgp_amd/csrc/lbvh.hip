@@ -324,6 +324,50 @@ __global__ void k_snapshot(int64_t n, const Node *__restrict__ nodes, const uint
   o[9] = nd.end;
 }
 
+// FP32 copy of the LBVH for ball queries: each internal node lists its
+// grandchildren (a leaf child stands for itself) with boxes rounded outward
+// (a leaf's box is its point).
+__device__ inline void wide_entry(const Node *nodes, const double *LX, const double *LY, const double *LZ,
+                                  uint32_t x, WideNode &w, int slot) {
+  double lo[3], hi[3];
+  if (x & kLeafBit) {
+    const uint32_t lp = x & ~kLeafBit;
+    lo[0] = hi[0] = LX[lp];
+    lo[1] = hi[1] = LY[lp];
+    lo[2] = hi[2] = LZ[lp];
+  } else {
+    for (int a = 0; a < 3; ++a) { lo[a] = nodes[x].lo[a]; hi[a] = nodes[x].hi[a]; }
+  }
+  for (int a = 0; a < 3; ++a) {
+    w.lo[slot][a] = __double2float_rd(lo[a]);
+    w.hi[slot][a] = __double2float_ru(hi[a]);
+  }
+  w.child[slot] = x;
+}
+
+__global__ void k_wide_nodes(int64_t m, const Node *__restrict__ nodes, const double *__restrict__ LX,
+                             const double *__restrict__ LY, const double *__restrict__ LZ, WideNode *out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const Node &N = nodes[i];
+  WideNode w;
+  int slot = 0;
+  for (int c = 0; c < 2; ++c) {
+    const uint32_t x = c ? N.right : N.left;
+    if (x & kLeafBit) {
+      wide_entry(nodes, LX, LY, LZ, x, w, slot++);
+    } else {
+      wide_entry(nodes, LX, LY, LZ, nodes[x].left, w, slot++);
+      wide_entry(nodes, LX, LY, LZ, nodes[x].right, w, slot++);
+    }
+  }
+  for (; slot < 4; ++slot) {  // empty slot: a box no query meets
+    for (int a = 0; a < 3; ++a) { w.lo[slot][a] = INFINITY; w.hi[slot][a] = -INFINITY; }
+    w.child[slot] = kInvalid;
+  }
+  out[i] = w;
+}
+
 }  // namespace
 
 void canonicalize_and_build(Context &c) {
@@ -384,6 +428,12 @@ void canonicalize_and_build(Context &c) {
   if (st[0] & 2u) throw std::invalid_argument("a point coordinate is not finite");
   if (st[0] & 1u) throw std::invalid_argument("the cloud contains duplicate geometric points");
   if (st[2]) throw std::runtime_error("an exact Morton bin could not be certified");
+  const int64_t internal = std::max<int64_t>(n - 1, 0);
+  c.wide.ensure(internal);
+  if (internal)
+    hipLaunchKernelGGL(k_wide_nodes, dim3(blocks_for(internal, 256)), dim3(256), 0, s, internal, c.nodes.p, c.LX.p,
+                       c.LY.p, c.LZ.p, c.wide.p);
+  MHGP_LAUNCH_CHECK();
   c.lbvh_ready = true;
 }
 

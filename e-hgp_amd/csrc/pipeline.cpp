@@ -19,7 +19,7 @@ int64_t device_bytes(const Context &c) {
   int64_t b = c.src.bytes() + c.X.bytes() * 3 + c.source_index.bytes() + c.key0.bytes() * 3 +
               c.perm.bytes() + c.codes.bytes() + c.leaf_id.bytes() + c.leaf_pos.bytes() +
               c.LX.bytes() * 3 + c.path.bytes() + c.nodes.bytes() + c.parent_int.bytes() +
-              c.parent_leaf.bytes() + c.wit.bytes() + c.counter.bytes();
+              c.parent_leaf.bytes() + c.wit.bytes() + c.counter.bytes() + c.wide.bytes();
   if (c.catalog) b += catalog_device_bytes(*c.catalog);
   if (c.tower_work) b += c.tower_work->bytes();
   return b;

@@ -908,15 +908,58 @@ __global__ void __launch_bounds__(128) k_gather(int64_t n, const Node *__restric
   if (!EMIT) gcount[u] = cnt;
 }
 
-__global__ void k_initial_radius(int64_t n, const double *__restrict__ X, const double *__restrict__ Y,
-                                 const double *__restrict__ Z, const uint32_t *__restrict__ knn, int k,
-                                 double factor, double *radius) {
-  int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= n) return;
-  uint32_t v = knn[u * k + k - 1];
-  double dx = X[v] - X[u], dy = Y[v] - Y[u], dz = Z[v] - Z[u];
-  radius[u] = factor * sqrt(dx * dx + dy * dy + dz * dz);
+// Initial gather radius of every point: factor x the distance to its kq-th
+// nearest neighbour (kq <= 32), found in FP32 over the two-level LBVH copy
+// with a register-resident sorted list.  The radius only sizes the first
+// gather; the certificate widens any anchor it does not cover.
+constexpr int kRadiusK = 32;
+constexpr int kRadiusStack = 64;
+__global__ void __launch_bounds__(128) k_knn_radius(int64_t n, int kq, float factor, const WideNode *__restrict__ wide,
+                                                    const double *__restrict__ LX, const double *__restrict__ LY,
+                                                    const double *__restrict__ LZ,
+                                                    const uint32_t *__restrict__ leaf_id, double *radius) {
+  __shared__ uint32_t stk[kRadiusStack * 128];
+  const int lane = threadIdx.x;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + lane;
+  if (p >= n) return;
+  const float q[3] = {(float)LX[p], (float)LY[p], (float)LZ[p]};
+  float best[kRadiusK];
+#pragma unroll
+  for (int j = 0; j < kRadiusK; ++j) best[j] = INFINITY;
+  float cut = INFINITY;  // the kq-th smallest so far
+  int sp = 0;
+  if (n > 1) stk[(sp++) * 128 + lane] = 0;
+  while (sp) {
+    const WideNode &N = wide[stk[(--sp) * 128 + lane]];
+    for (int c = 0; c < 4; ++c) {
+      const uint32_t x = N.child[c];
+      if (x == kInvalid) continue;
+      float d2 = 0.f;
+      for (int a = 0; a < 3; ++a) {
+        const float d = fmaxf(fmaxf(N.lo[c][a] - q[a], q[a] - N.hi[c][a]), 0.f);
+        d2 += d * d;
+      }
+      if (!(d2 < cut)) continue;
+      if (x & kLeafBit) {
+        if ((x & ~kLeafBit) == (uint32_t)p) continue;
+        // sorted insertion over static indices (the list stays in registers)
+#pragma unroll
+        for (int j = kRadiusK - 1; j > 0; --j) best[j] = d2 < best[j - 1] ? best[j - 1] : (d2 < best[j] ? d2 : best[j]);
+        best[0] = fminf(best[0], d2);
+        float c2 = INFINITY;
+#pragma unroll
+        for (int j = 0; j < kRadiusK; ++j) c2 = j == kq - 1 ? best[j] : c2;
+        cut = c2;
+      } else if (sp < kRadiusStack) {
+        stk[(sp++) * 128 + lane] = x;
+      }
+    }
+  }
+  // a subtree dropped by a full stack can only raise the kq-th distance found
+  const double d = sqrt((double)cut) * (1.0 + 1e-6);
+  radius[leaf_id[p]] = (double)factor * d;
 }
+
 
 // exclusive scan of gcount over the anchor list into goff (single workgroup)
 __global__ void k_scan_offsets(const uint32_t *__restrict__ anchors, uint32_t count,
@@ -1017,14 +1060,10 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
   }
   // initial gather radius: twice the distance to the min(32, n-1)-th neighbour
   const int kq = (int)std::min<int64_t>(32, n - 1);
-  DevBuf<uint32_t> nb;
-  DevBuf<uint8_t> nbf;
-  nb.ensure((size_t)n * kq);
-  nbf.ensure(n);
   StageTimer tm(s);
-  knn_all(c, kq, nb.p, nbf.p);
-  hipLaunchKernelGGL(k_initial_radius, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, c.X.p, c.Y.p,
-                     c.Z.p, nb.p, kq, 2.2, B.radius.p);
+  hipLaunchKernelGGL(k_knn_radius, dim3(blocks_for(n, 128)), dim3(128), 0, s, n, kq, 2.2f, c.wide.p, c.LX.p,
+                     c.LY.p, c.LZ.p, c.leaf_id.p, B.radius.p);
+  MHGP_LAUNCH_CHECK();
   std::vector<uint32_t> anchors(n);
   std::iota(anchors.begin(), anchors.end(), 0u);
   // anchors in Morton order: neighbouring workgroups share cache lines

@@ -54,50 +54,6 @@ __device__ inline int event_label(const EventRec &e, uint32_t *out) {
 
 __device__ inline int rank_of(const EventRec &e) { return (e.meta >> 8) & 0xff; }
 
-// FP32 copy of the LBVH for ball queries: each internal node lists its
-// grandchildren (a leaf child stands for itself) with boxes rounded outward
-// (a leaf's box is its point).
-__device__ inline void wide_entry(const Node *nodes, const double *LX, const double *LY, const double *LZ,
-                                  uint32_t x, WideNode &w, int slot) {
-  double lo[3], hi[3];
-  if (x & kLeafBit) {
-    const uint32_t lp = x & ~kLeafBit;
-    lo[0] = hi[0] = LX[lp];
-    lo[1] = hi[1] = LY[lp];
-    lo[2] = hi[2] = LZ[lp];
-  } else {
-    for (int a = 0; a < 3; ++a) { lo[a] = nodes[x].lo[a]; hi[a] = nodes[x].hi[a]; }
-  }
-  for (int a = 0; a < 3; ++a) {
-    w.lo[slot][a] = __double2float_rd(lo[a]);
-    w.hi[slot][a] = __double2float_ru(hi[a]);
-  }
-  w.child[slot] = x;
-}
-
-__global__ void k_wide_nodes(int64_t m, const Node *__restrict__ nodes, const double *__restrict__ LX,
-                             const double *__restrict__ LY, const double *__restrict__ LZ, WideNode *out) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  const Node &N = nodes[i];
-  WideNode w;
-  int slot = 0;
-  for (int c = 0; c < 2; ++c) {
-    const uint32_t x = c ? N.right : N.left;
-    if (x & kLeafBit) {
-      wide_entry(nodes, LX, LY, LZ, x, w, slot++);
-    } else {
-      wide_entry(nodes, LX, LY, LZ, nodes[x].left, w, slot++);
-      wide_entry(nodes, LX, LY, LZ, nodes[x].right, w, slot++);
-    }
-  }
-  for (; slot < 4; ++slot) {  // empty slot: a box no query meets
-    for (int a = 0; a < 3; ++a) { w.lo[slot][a] = INFINITY; w.hi[slot][a] = -INFINITY; }
-    w.child[slot] = kInvalid;
-  }
-  out[i] = w;
-}
-
 // Records grouped by closed rank, canonical order kept inside each rank:
 // key = rank (0xff for a degenerate record), value = sorted position.
 __global__ void k_rank_keys(const EventRec *__restrict__ ev, const uint32_t *__restrict__ order, int64_t n,
@@ -865,11 +821,6 @@ void tower_rank_index(Context &c) {
   TowerWork &W = *c.tower_work;
   std::fill(W.rank_lo, W.rank_lo + 256, 0u);
   std::fill(W.rank_hi, W.rank_hi + 256, 0u);
-  const int64_t internal = std::max<int64_t>(c.n - 1, 0);
-  W.wide.ensure(internal);
-  if (internal)
-    hipLaunchKernelGGL(k_wide_nodes, dim3(blocks_for(internal, 256)), dim3(256), 0, s, internal, c.nodes.p, c.LX.p,
-                       c.LY.p, c.LZ.p, W.wide.p);
   if (n == 0) return;
   W.rank_keys.ensure(n);
   W.rank_vals.ensure(n);
@@ -935,7 +886,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   A.arm_saddle = arm_saddle.p; A.arms = arms; A.k = k;
   A.table_keys = tkeys.p; A.table_vals = tvals.p; A.mask = cap - 1; A.labels = labels.p;
   A.X = c.X.p; A.Y = c.Y.p; A.Z = c.Z.p;
-  A.wide = W.wide.p; A.LX = c.LX.p; A.LY = c.LY.p; A.LZ = c.LZ.p; A.leaf_id = c.leaf_id.p; A.leaf_pos = c.leaf_pos.p; A.n = c.n;
+  A.wide = c.wide.p; A.LX = c.LX.p; A.LY = c.LY.p; A.LZ = c.LZ.p; A.leaf_id = c.leaf_id.p; A.leaf_pos = c.leaf_pos.p; A.n = c.n;
   A.carrier = carrier.p;
   A.steps = (unsigned long long *)steps.p;
   auto &queue = W.queue;

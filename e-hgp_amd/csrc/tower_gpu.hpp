@@ -25,15 +25,6 @@ struct TowerOrderData {
   int64_t steps = 0;
 };
 
-// LBVH internal node for FP32 ball queries, two levels collapsed into one
-// 128-byte line: the node's grandchildren (or a child that is a leaf), their
-// boxes rounded outward, children as in Node::left/right (kInvalid: none).
-struct alignas(128) WideNode {
-  float lo[4][3], hi[4][3];
-  uint32_t child[4];
-};
-static_assert(sizeof(WideNode) == 128, "wide node layout");
-
 // Device scratch of the per-order tower work, kept by the context; a buffer
 // is reallocated only when an order needs more than it holds.
 struct TowerWork {
@@ -41,7 +32,6 @@ struct TowerWork {
   DevBuf<uint64_t> rank_keys;
   DevBuf<uint32_t> rank_vals, rank_bounds;
   uint32_t rank_lo[256] = {}, rank_hi[256] = {};
-  DevBuf<WideNode> wide;
   DevBuf<uint32_t> tvals, labels;
   DevBuf<uint64_t> tkeys, steps, qcount, qkey, facet_slots, alias_count;
   DevBuf<uint32_t> alias_arm, alias_rep, dF[2], dH[2], dO[2];
@@ -61,7 +51,7 @@ struct TowerWork {
                                       &changed, &err, &live_n, &Lu[0], &Lu[1], &Lv[0], &Lv[1], &Li[0], &Li[1],
                                       &attach, &rep, &dense, &slab, &ssup, &alias_arm, &alias_rep, &dF[0], &dF[1], &dH[0], &dH[1], &dO[0], &dO[1]})
       b += (int64_t)x->bytes();
-    return b + (int64_t)(wide.bytes() + rank_keys.bytes() + qkey.bytes() + facet_slots.bytes() + alias_count.bytes() + dcount.bytes() + tkeys.bytes() + steps.bytes() + qcount.bytes() + slev.bytes() + c_lev.bytes());
+    return b + (int64_t)(rank_keys.bytes() + qkey.bytes() + facet_slots.bytes() + alias_count.bytes() + dcount.bytes() + tkeys.bytes() + steps.bytes() + qcount.bytes() + slev.bytes() + c_lev.bytes());
   }
 };
 
