@@ -862,8 +862,11 @@ __global__ void __launch_bounds__(kBlock) k_anchor_big(Params P) {
 }
 
 // Ball count / emit around every anchor of the list.
+// Pruning runs in FP32 on the two-level LBVH copy (boxes rounded outward, the
+// radius inflated by the rounding of the query and of the FP32 arithmetic; a
+// non-finite bound prunes nothing); the leaf test is the FP64 one.
 template <bool EMIT>
-__global__ void __launch_bounds__(128) k_gather(int64_t n, const Node *__restrict__ nodes,
+__global__ void __launch_bounds__(128) k_gather(int64_t n, const WideNode *__restrict__ wide,
                                                 const double *__restrict__ LX,
                                                 const double *__restrict__ LY,
                                                 const double *__restrict__ LZ,
@@ -873,35 +876,49 @@ __global__ void __launch_bounds__(128) k_gather(int64_t n, const Node *__restric
                                                 const double *__restrict__ radius, uint32_t *gcount,
                                                 const uint64_t *__restrict__ goff, uint32_t *glist,
                                                 uint32_t *overflow) {
-  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  // depth-first over 4-way nodes: at most 3 pending siblings per level
+  constexpr int kStack = 96;
+  __shared__ uint32_t stk[kStack * 128];
+  const int lane = threadIdx.x;
+  uint32_t a = blockIdx.x * blockDim.x + lane;
   if (a >= count) return;
   const uint32_t u = anchors[a];
   const uint32_t p = leaf_pos[u];
   const double q[3] = {LX[p], LY[p], LZ[p]};
   const double R = radius[u];
   const double R2 = R * R * (1.0 + 1e-9);
+  const float qf[3] = {(float)q[0], (float)q[1], (float)q[2]};
+  const double qmax = fmax(fmax(fabs(q[0]), fabs(q[1])), fabs(q[2]));
+  const double rb = sqrt(R2) * (1.0 + 1e-6) + qmax * 1e-6 + 1e-30;
+  float Tf = (float)(rb * rb * (1.0 + 1e-5));
+  if (!(Tf < 3.0e38f)) Tf = INFINITY;
   uint32_t cnt = 0;
   uint64_t base = EMIT ? goff[u] : 0;
-  uint32_t stack[64];
   int sp = 0;
-  stack[sp++] = 0;
-  if (n == 1) sp = 0;
+  if (n > 1) stk[(sp++) * 128 + lane] = 0;
   while (sp) {
-    const Node &N = nodes[stack[--sp]];
-    uint32_t ch[2] = {N.left, N.right};
-    for (int c = 0; c < 2; ++c) {
-      uint32_t x = ch[c];
+    const WideNode &N = wide[stk[(--sp) * 128 + lane]];
+    for (int c = 0; c < 4; ++c) {
+      const uint32_t x = N.child[c];
+      if (x == kInvalid) continue;
+      float d2 = 0.f;
+      for (int ax = 0; ax < 3; ++ax) {
+        const float d = fmaxf(fmaxf(N.lo[c][ax] - qf[ax], qf[ax] - N.hi[c][ax]), 0.f);
+        d2 += d * d;
+      }
+      if (d2 > Tf) continue;
       if (x & kLeafBit) {
-        uint32_t lp = x & ~kLeafBit;
+        const uint32_t lp = x & ~kLeafBit;
         if (lp == p) continue;
-        double dx = LX[lp] - q[0], dy = LY[lp] - q[1], dz = LZ[lp] - q[2];
+        const double dx = LX[lp] - q[0], dy = LY[lp] - q[1], dz = LZ[lp] - q[2];
         if (dx * dx + dy * dy + dz * dz <= R2) {
           if (EMIT) glist[base + cnt] = leaf_id[lp];
           cnt++;
         }
-      } else if (box_d2(q, nodes[x].lo, nodes[x].hi) <= R2) {
-        if (sp < 64) stack[sp++] = x;
-        else atomicOr(overflow, 1u);
+      } else if (sp < kStack) {
+        stk[(sp++) * 128 + lane] = x;
+      } else {
+        atomicOr(overflow, 1u);
       }
     }
   }
@@ -1094,7 +1111,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     if (iter > 12) throw std::runtime_error("anchor gather radii did not converge");
     const uint32_t cnt = (uint32_t)pending.size();
     MHGP_HIP(hipMemcpyAsync(B.anchors.p, pending.data(), cnt * 4, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_gather<false>, dim3(blocks_for(cnt, 128)), dim3(128), 0, s, n, c.nodes.p,
+    hipLaunchKernelGGL(k_gather<false>, dim3(blocks_for(cnt, 128)), dim3(128), 0, s, n, c.wide.p,
                        c.LX.p, c.LY.p, c.LZ.p, c.leaf_id.p, c.leaf_pos.p, B.anchors.p, cnt,
                        B.radius.p, B.gcount.p, B.goff.p, B.glist.p, overflow.p);
     hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, B.anchors.p, cnt, B.gcount.p,
@@ -1103,7 +1120,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     MHGP_HIP(hipMemcpyAsync(&tot, total.p, 8, hipMemcpyDeviceToHost, s));
     MHGP_HIP(hipStreamSynchronize(s));
     B.glist.ensure(tot + 1);
-    hipLaunchKernelGGL(k_gather<true>, dim3(blocks_for(cnt, 128)), dim3(128), 0, s, n, c.nodes.p,
+    hipLaunchKernelGGL(k_gather<true>, dim3(blocks_for(cnt, 128)), dim3(128), 0, s, n, c.wide.p,
                        c.LX.p, c.LY.p, c.LZ.p, c.leaf_id.p, c.leaf_pos.p, B.anchors.p, cnt,
                        B.radius.p, B.gcount.p, B.goff.p, B.glist.p, overflow.p);
     Params P{};
@@ -1201,7 +1218,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
         const uint32_t ac = (uint32_t)again.size();
         copy_wait(B.radius.p, rad.data(), n * 8, hipMemcpyHostToDevice, s);
         MHGP_HIP(hipMemcpyAsync(B.anchors.p, again.data(), ac * 4, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_gather<false>, dim3(blocks_for(ac, 128)), dim3(128), 0, s, n, c.nodes.p,
+        hipLaunchKernelGGL(k_gather<false>, dim3(blocks_for(ac, 128)), dim3(128), 0, s, n, c.wide.p,
                            c.LX.p, c.LY.p, c.LZ.p, c.leaf_id.p, c.leaf_pos.p, B.anchors.p, ac,
                            B.radius.p, B.gcount.p, B.goff.p, B.glist.p, overflow.p);
         hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, B.anchors.p, ac, B.gcount.p,
@@ -1209,7 +1226,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
         MHGP_HIP(hipMemcpyAsync(&tot, total.p, 8, hipMemcpyDeviceToHost, s));
         MHGP_HIP(hipStreamSynchronize(s));
         B.glist.ensure(tot + 1);
-        hipLaunchKernelGGL(k_gather<true>, dim3(blocks_for(ac, 128)), dim3(128), 0, s, n, c.nodes.p,
+        hipLaunchKernelGGL(k_gather<true>, dim3(blocks_for(ac, 128)), dim3(128), 0, s, n, c.wide.p,
                            c.LX.p, c.LY.p, c.LZ.p, c.leaf_id.p, c.leaf_pos.p, B.anchors.p, ac,
                            B.radius.p, B.gcount.p, B.goff.p, B.glist.p, overflow.p);
         MHGP_HIP(hipMemcpyAsync(gc.data(), B.gcount.p, n * 4, hipMemcpyDeviceToHost, s));
