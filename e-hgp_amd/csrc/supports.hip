@@ -107,6 +107,28 @@ struct Work {
 
 __device__ inline V3 gpt(const Work &w, uint32_t i) { return {w.gx[i], w.gy[i], w.gz[i]}; }
 
+// Interior id lists are written and sorted with static indices only, so
+// they stay in registers: an append is a select over the slots, the sort an
+// odd-even transposition network (unused slots hold kInvalid and sort last).
+__device__ inline void put_static(uint32_t (&a)[kMaxInterior], int pos, uint32_t v) {
+#pragma unroll
+  for (int j = 0; j < kMaxInterior; ++j) a[j] = j == pos ? v : a[j];
+}
+
+__device__ inline void sort_static(uint32_t (&a)[kMaxInterior], int n) {
+#pragma unroll
+  for (int j = 0; j < kMaxInterior; ++j) a[j] = j < n ? a[j] : kInvalid;
+#pragma unroll
+  for (int pass = 0; pass < kMaxInterior; ++pass) {
+#pragma unroll
+    for (int j = pass & 1; j + 1 < kMaxInterior; j += 2) {
+      const uint32_t x = a[j], y = a[j + 1];
+      a[j] = min(x, y);
+      a[j + 1] = max(x, y);
+    }
+  }
+}
+
 __device__ void emit_event(const Params &P, uint32_t u, const uint32_t *sup, int m, int nint,
                            const uint32_t *interior, double level, double lerr) {
   const unsigned long long slot = wave_append(true, &P.counters[0]);
@@ -115,6 +137,7 @@ __device__ void emit_event(const Params &P, uint32_t u, const uint32_t *sup, int
   e.ids[0] = u;
   for (int i = 1; i < 4; ++i) e.ids[i] = i < m ? sup[i - 1] : kInvalid;
   e.meta = (uint32_t)m | ((uint32_t)(nint + m) << 8) | ((uint32_t)nint << 16);
+#pragma unroll
   for (int i = 0; i < kMaxInterior; ++i) e.interior[i] = i < nint ? interior[i] : kInvalid;
   e.level = level;
   e.level_err = (float)fmin(lerr, 1e-3);
@@ -147,7 +170,7 @@ __device__ void emit_fallback(const Params &P, const Work &w, uint32_t u, uint32
 // Returns 1 event, 0 not critical, -1 undecided (host).
 template <class S>
 __device__ int classify(const Params &P, const Work &w, uint32_t M, const S &sph,
-                        const uint32_t *skip, int m, uint32_t *interior, int &nint) {
+                        const uint32_t *skip, int m, uint32_t (&interior)[kMaxInterior], int &nint) {
   nint = 0;
   const int cap = P.s_max - m;
   int undecided = 0;
@@ -158,7 +181,7 @@ __device__ int classify(const Params &P, const Work &w, uint32_t M, const S &sph
     if (on_support) continue;
     int side = sph.side(gpt(w, gi));
     if (side < 0) {
-      if (nint < kMaxInterior) interior[nint] = w.gid[gi];
+      put_static(interior, nint, w.gid[gi]);
       nint++;
       if (nint > cap) return 0;
     } else if (side == 0) {
@@ -168,14 +191,6 @@ __device__ int classify(const Params &P, const Work &w, uint32_t M, const S &sph
   return undecided ? -1 : 1;
 }
 
-__device__ void sort_small(uint32_t *a, int n) {
-  for (int i = 1; i < n; ++i) {
-    uint32_t v = a[i];
-    int j = i - 1;
-    while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; j--; }
-    a[j + 1] = v;
-  }
-}
 
 __device__ void process_pair(const Params &P, const Work &w, uint32_t u, uint32_t M, uint32_t vi,
                              uint32_t *region_slot) {
@@ -192,7 +207,7 @@ __device__ void process_pair(const Params &P, const Work &w, uint32_t u, uint32_
     emit_fallback(P, w, u, 2, &vid, 2, M, region_slot);
     return;
   }
-  sort_small(interior, nint);
+  sort_static(interior, nint);
   emit_event(P, u, &vid, 2, nint, interior, sph.level(), 8 * kCert);
 }
 
@@ -210,7 +225,7 @@ __device__ void process_tet(const Params &P, const Work &w, uint32_t u, uint32_t
   int r = classify(P, w, M, t, skip, 4, interior, nint);
   if (r == 0) return;
   if (r < 0) { emit_fallback(P, w, u, 4, ids, 4, M, region_slot); return; }
-  sort_small(interior, nint);
+  sort_static(interior, nint);
   emit_event(P, u, ids, 4, nint, interior, t.level(), t.level_err());
 }
 
@@ -466,7 +481,7 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
       return;
     }
     if (sp > 0) continue;
-    if (total_in < kMaxInterior) tri_int[total_in] = w.gid[gi];
+    put_static(tri_int, total_in, w.gid[gi]);
     total_in++;
     cin0 += sh > 0;
     cin1 += sh < 0;
@@ -476,7 +491,7 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
   // triangle event
   if (wc > 0 && total_in + 3 <= smax) {
     atomicAdd(w.cand, 1u);
-    sort_small(tri_int, total_in);
+    sort_static(tri_int, total_in);
     emit_event(P, u, ids, 3, total_in, tri_int, f.level(), f.level_err());
   } else if (wc == 0 && total_in + 3 <= smax) {
     emit_fallback(P, w, u, 3, ids, 3, M, region_slot);
@@ -550,17 +565,23 @@ __device__ bool member_mask(const Params &P, const Work &w, uint32_t i, uint64_t
   }
   const float inv = __builtin_amdgcn_rcpf(q.w);
   bool any = false;
-  for (int c = 0; c < kCells; ++c) {
-    const CellF cf = cell_f(P.cells + 8 * c);
-    const float ct = (q.x * cf.dx + q.y * cf.dy + q.z * cf.dz) * inv;
-    float cm;  // cos(max(theta - delta, 0))
-    if (ct >= cf.cosd) cm = 1.0f;
-    else cm = ct * cf.cosd + __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - ct * ct)) * cf.sind;
-    const float rho = (float)w.rho[c];
-    if (q.w <= 2.0f * rho * (cm + 1e-5f) * (1.0f + 1e-5f)) {
-      mask[c >> 6] |= 1ull << (c & 63);
-      any = true;
+  // one mask word per 64 cells, each built in a register
+#pragma unroll
+  for (int wd = 0; wd < kCells / 64; ++wd) {
+    uint64_t bits = 0;
+#pragma unroll 1
+    for (int b = 0; b < 64; ++b) {
+      const int c = 64 * wd + b;
+      const CellF cf = cell_f(P.cells + 8 * c);
+      const float ct = (q.x * cf.dx + q.y * cf.dy + q.z * cf.dz) * inv;
+      float cm;  // cos(max(theta - delta, 0))
+      if (ct >= cf.cosd) cm = 1.0f;
+      else cm = ct * cf.cosd + __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - ct * ct)) * cf.sind;
+      const float rho = (float)w.rho[c];
+      if (q.w <= 2.0f * rho * (cm + 1e-5f) * (1.0f + 1e-5f)) bits |= 1ull << b;
     }
+    mask[wd] = bits;
+    any |= bits != 0;
   }
   return any;
 }
@@ -572,7 +593,6 @@ constexpr int kCoverCap = 10;  // s_max - 1 <= 10
 template <bool BIG>
 __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   __shared__ uint32_t sM, sK, sflag, sregion;
-  __shared__ double smax_rho;
   const int t = threadIdx.x;
   const uint32_t G = P.gcount[u];
   const uint64_t off = P.goff[u];
@@ -671,7 +691,6 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   if (t == 0) {
     double m = 0;
     for (int c = 0; c < kCells; ++c) m = fmax(m, w.rho[c]);
-    smax_rho = m;
     if (2.0 * m > Rg) {
       P.need[u] = 2.0 * m;
       P.status[u] = ST_WIDEN;
@@ -685,9 +704,10 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     // ordered compaction: members keep the gathered (distance) order
     const uint32_t chunk = (G + blockDim.x - 1) / blockDim.x;
     const uint32_t b0 = t * chunk;
-    uint64_t mk[4][6];
+    uint64_t mk[4][6];  // static indices only: the masks stay in registers
     bool mem[4];
     uint32_t cnt = 0;
+#pragma unroll
     for (uint32_t r = 0; r < 4; ++r) {
       mem[r] = false;
       if (r < chunk && b0 + r < G) {
@@ -704,10 +724,12 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
       __syncthreads();
     }
     uint32_t slot = scan_buf[t] - cnt;
+#pragma unroll
     for (uint32_t r = 0; r < 4; ++r) {
       if (!mem[r]) continue;
       if (slot < P.rcap) {
         w.ridx[slot] = b0 + r;
+#pragma unroll
         for (int k = 0; k < 6; ++k) w.rmask[6 * slot + k] = mk[r][k];
       }
       slot++;
