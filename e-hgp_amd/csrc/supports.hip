@@ -52,7 +52,7 @@ int64_t catalog_device_bytes(const CatalogBuffers &b) {
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 512;
 
 // LDS written by some lanes of a wave, then read by other lanes of it
 __device__ inline void wave_sync() {
@@ -829,7 +829,7 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   }
 }
 
-__global__ void __launch_bounds__(kBlock, 3) k_anchor_lds(Params P) {
+__global__ void __launch_bounds__(kBlock, 4) k_anchor_lds(Params P) {
   __shared__ double gx[kGatherCap], gy[kGatherCap], gz[kGatherCap];
   __shared__ uint32_t gid[kGatherCap];
   __shared__ double rho[kCells];
