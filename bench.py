@@ -150,11 +150,13 @@ def batched(count=256, slots=4, plain_subset=32):
     codes = {}
     ctx = mhgp.Context(0)
     t0 = time.perf_counter()
-    for pts in clouds[:plain_subset]:
+    for i, pts in enumerate(clouds[:plain_subset]):
         try:
             run_step(ctx, pts, 10)
         except mhgp.MhgpError:
             pass
+        if (i + 1) % 8 == 0:
+            print(f"batched, plain path: {i + 1}/{plain_subset} done", file=sys.stderr, flush=True)
     plain_s = time.perf_counter() - t0
     ctx.close()
     srv = mhgp.Server(0, slots=slots)

@@ -829,16 +829,20 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   }
 }
 
-__global__ void __launch_bounds__(kBlock, 4) k_anchor_lds(Params P) {
-  __shared__ double gx[kGatherCap], gy[kGatherCap], gz[kGatherCap];
-  __shared__ uint32_t gid[kGatherCap];
+// The LDS path in two sizes: most anchors fit the small one, which leaves
+// room for four waves per SIMD; the wide one (one workgroup per CU) takes the
+// anchors whose gather or region outgrow it, before the global-memory path.
+template <int GCAP, int RCAP>
+__device__ void anchor_lds(const Params &P) {
+  __shared__ double gx[GCAP], gy[GCAP], gz[GCAP];
+  __shared__ uint32_t gid[GCAP];
   __shared__ double rho[kCells];
-  __shared__ uint32_t ridx[kRegionCap];
+  __shared__ uint32_t ridx[RCAP];
   // the FP32 offsets (16 B per gathered point) live in the mask area until
   // the masks are written
-  __shared__ __align__(16) uint64_t rmask[6 * kRegionCap];
-  static_assert(sizeof(uint64_t) * 6 * kRegionCap >= sizeof(float4) * kGatherCap, "fc alias");
-  __shared__ uint32_t kidx[kRegionCap];
+  __shared__ __align__(16) uint64_t rmask[6 * RCAP];
+  static_assert(sizeof(uint64_t) * 6 * RCAP >= sizeof(float4) * GCAP, "fc alias");
+  __shared__ uint32_t kidx[RCAP];
   const uint32_t a = blockIdx.x;
   if (a >= P.anchor_count) return;
   const uint32_t u = P.anchors[a];
@@ -851,6 +855,11 @@ __global__ void __launch_bounds__(kBlock, 4) k_anchor_lds(Params P) {
   __syncthreads();
   Work w{gx, gy, gz, gid, rho, ridx, rmask, kidx, reinterpret_cast<float4 *>(rmask), &cand};
   anchor_work<false>(P, w, u);
+}
+
+__global__ void __launch_bounds__(kBlock, 4) k_anchor_lds(Params P) { anchor_lds<kGatherCap, kRegionCap>(P); }
+__global__ void __launch_bounds__(kBlock, 2) k_anchor_lds_wide(Params P) {
+  anchor_lds<kGatherCapWide, kRegionCapWide>(P);
 }
 
 // Offset (in doubles, even) of the FP32 offsets in a global slab of gather
@@ -1185,15 +1194,33 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     MHGP_HIP(hipMemcpyAsync(rad.data(), B.radius.p, n * 8, hipMemcpyDeviceToHost, s));
     MHGP_HIP(hipMemcpyAsync(gc.data(), B.gcount.p, n * 4, hipMemcpyDeviceToHost, s));
     MHGP_HIP(hipStreamSynchronize(s));
-    std::vector<uint32_t> next;
-    for (uint32_t u : pending) {
-      if (st[u] == ST_WIDEN) {
-        rad[u] = std::max(need[u] * 1.02, rad[u] * 1.05);
-        next.push_back(u);
-        widened++;
-      } else if (st[u] == ST_BIG) {
-        big.push_back(u);
+    std::vector<uint32_t> next, wide_list;
+    for (int tier = 0; tier < 2; ++tier) {
+      std::vector<uint32_t> wide;
+      for (uint32_t u : (tier == 0 ? pending : wide_list)) {
+        if (st[u] == ST_WIDEN) {
+          rad[u] = std::max(need[u] * 1.02, rad[u] * 1.05);
+          next.push_back(u);
+          widened++;
+        } else if (st[u] == ST_BIG) {
+          if (tier == 0 && gc[u] <= (uint32_t)kGatherCapWide) wide.push_back(u);
+          else big.push_back(u);
+        }
       }
+      if (tier == 1 || wide.empty()) break;
+      // anchors that outgrew the small LDS path: the wide one
+      wide_list = wide;
+      const uint32_t wc = (uint32_t)wide.size();
+      MHGP_HIP(hipMemcpyAsync(B.anchors.p, wide.data(), wc * 4, hipMemcpyHostToDevice, s));
+      Params W2 = P;
+      W2.anchor_count = wc;
+      W2.gcap = kGatherCapWide;
+      W2.rcap = kRegionCapWide;
+      hipLaunchKernelGGL(k_anchor_lds_wide, dim3(wc), dim3(kBlock), 0, s, W2);
+      MHGP_LAUNCH_CHECK();
+      MHGP_HIP(hipMemcpyAsync(st.data(), B.status.p, n * 4, hipMemcpyDeviceToHost, s));
+      MHGP_HIP(hipMemcpyAsync(need.data(), B.need.p, n * 8, hipMemcpyDeviceToHost, s));
+      MHGP_HIP(hipStreamSynchronize(s));
     }
     // big anchors: widen inside the big path too
     if (!big.empty()) {
