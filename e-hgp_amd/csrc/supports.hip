@@ -61,7 +61,7 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-enum : uint32_t { ST_PENDING = 0, ST_DONE = 1, ST_WIDEN = 2, ST_BIG = 3, ST_HUGE = 4 };
+enum : uint32_t { ST_PENDING = 0, ST_DONE = 1, ST_WIDEN = 2, ST_BIG = 3, ST_HUGE = 4, ST_PROBED = 5 };
 
 // Regions of the global-memory path are enumerated face by face in
 // O(region^3) work; beyond this size one anchor would keep a workgroup busy
@@ -89,6 +89,7 @@ struct Params {
   const uint32_t *anchors;
   uint32_t anchor_count;
   int big;                       // 1: arrays live in global scratch slabs
+  int probe;                     // 1: stop once the region size is known (ST_PROBED)
   double *scratch;
   uint64_t slab_doubles;
   uint32_t gcap, rcap;
@@ -757,6 +758,10 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     if (t == 0) P.status[u] = ST_HUGE;
     return;
   }
+  if (P.probe) {
+    if (t == 0) P.status[u] = ST_PROBED;
+    return;
+  }
   for (uint32_t s = t; s < M; s += blockDim.x)
     if (w.gid[w.ridx[s]] > u) w.kidx[atomicAdd(&sK, 1u)] = s;
   __syncthreads();
@@ -1236,20 +1241,27 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
         B.scratch.ensure(slab * chunk);
         // the CSR of the big anchors is still valid from this iteration's gather
         std::vector<uint32_t> again;
-        for (uint32_t b0 = 0; b0 < bc; b0 += chunk) {
-          uint32_t bn = std::min(chunk, bc - b0);
-          MHGP_HIP(hipMemcpyAsync(B.anchors.p, big.data() + b0, bn * 4, hipMemcpyHostToDevice, s));
-          Params Q = P;
-          // the gather CSR may have been reallocated by a regather below
-          Q.gcount = B.gcount.p; Q.goff = B.goff.p; Q.glist = B.glist.p; Q.radius = B.radius.p;
-          Q.anchors = B.anchors.p; Q.anchor_count = bn; Q.big = 1;
-          Q.scratch = B.scratch.p; Q.slab_doubles = slab; Q.gcap = gmax; Q.rcap = gmax;
-          hipLaunchKernelGGL(k_anchor_big, dim3(bn), dim3(kBlock), 0, s, Q);
-          MHGP_LAUNCH_CHECK();
-        }
-        MHGP_HIP(hipMemcpyAsync(st.data(), B.status.p, n * 4, hipMemcpyDeviceToHost, s));
-        MHGP_HIP(hipMemcpyAsync(need.data(), B.need.p, n * 8, hipMemcpyDeviceToHost, s));
-        MHGP_HIP(hipStreamSynchronize(s));
+        auto launch_big = [&](const std::vector<uint32_t> &list, int probe) {
+          const uint32_t lc = (uint32_t)list.size();
+          for (uint32_t b0 = 0; b0 < lc; b0 += chunk) {
+            uint32_t bn = std::min(chunk, lc - b0);
+            MHGP_HIP(hipMemcpyAsync(B.anchors.p, list.data() + b0, bn * 4, hipMemcpyHostToDevice, s));
+            Params Q = P;
+            // the gather CSR may have been reallocated by a regather below
+            Q.gcount = B.gcount.p; Q.goff = B.goff.p; Q.glist = B.glist.p; Q.radius = B.radius.p;
+            Q.anchors = B.anchors.p; Q.anchor_count = bn; Q.big = 1; Q.probe = probe;
+            Q.scratch = B.scratch.p; Q.slab_doubles = slab; Q.gcap = gmax; Q.rcap = gmax;
+            hipLaunchKernelGGL(k_anchor_big, dim3(bn), dim3(kBlock), 0, s, Q);
+            MHGP_LAUNCH_CHECK();
+          }
+          MHGP_HIP(hipMemcpyAsync(st.data(), B.status.p, n * 4, hipMemcpyDeviceToHost, s));
+          MHGP_HIP(hipMemcpyAsync(need.data(), B.need.p, n * 8, hipMemcpyDeviceToHost, s));
+          MHGP_HIP(hipStreamSynchronize(s));
+        };
+        // first the region sizes only (no enumeration), so that a region the
+        // path cannot take is refused before any long launch
+        launch_big(big, 1);
+        std::vector<uint32_t> full;
         for (uint32_t u : big) {
           if (st[u] == ST_WIDEN) {
             rad[u] = std::max(need[u] * 1.02, rad[u] * 1.05);
@@ -1258,10 +1270,15 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
           } else if (st[u] == ST_HUGE) {
             throw CapacityError("an anchor's candidate region exceeds " + std::to_string(kBigRegionMax) +
                                 " points (dense anisotropic cluster); the device catalogue does not enumerate it");
-          } else if (st[u] != ST_DONE) {
-            throw std::runtime_error("a big anchor did not complete");
+          } else if (st[u] == ST_PROBED) {
+            full.push_back(u);
+          } else {
+            throw std::runtime_error("a big anchor did not complete its probe");
           }
         }
+        if (!full.empty()) launch_big(full, 0);
+        for (uint32_t u : full)
+          if (st[u] != ST_DONE) throw std::runtime_error("a big anchor did not complete");
         if (again.empty()) break;
         // regather the widened big anchors
         const uint32_t ac = (uint32_t)again.size();
