@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cmath>
 #include <exception>
+#include <mutex>
 #include <thread>
 #include <numeric>
 #include <unordered_map>
@@ -391,48 +392,95 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
     }
   } joiner{workers};
   tower_rank_index(c);
-  // highest order first: its contraction is the longest and overlaps the
-  // device work of all lower orders
-  for (int k = K; k >= 1; --k) {
-    TowerOrderData &D = Ds[k - 1];
-    tower_order_device(c, k, D);
-    tm.mark("tower arms (device)");
-    T.descent_steps += D.steps;
-    // carriers the device could not settle: exact host descents
-    bool any_host = false;
-    for (uint32_t a = 0; a < D.arms; ++a) any_host |= D.carrier[a] == kInvalid;
-    if (any_host) {
-      if (!desc) {
-        xyz.resize(3 * n);
-        canonical_points(c, xyz.data(), nullptr);
-        desc = std::make_unique<Descender>(xyz, n, exact_cloud(c));
+  TowerWork &W0 = *c.tower_work;
+  W0.stream = c.stream;
+  if (K >= 2 && !c.tower_work2) {
+    auto w = std::make_unique<TowerWork>();
+    MHGP_HIP(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+    w->own_stream = true;
+    c.tower_work2 = std::move(w);
+  }
+  std::mutex mu;  // workers, the host descender
+  // Two device lanes take the orders in turn (K, K-2, ... and K-1, K-3, ...),
+  // so one lane's host synchronisations overlap the other's kernels; each
+  // order's contraction then runs on a host thread of its own.  Highest
+  // orders first: their contractions are the longest.
+  auto lane = [&](TowerWork &W, int first_k) {
+    MHGP_HIP(hipSetDevice(c.device));
+    for (int k = first_k; k >= 1; k -= 2) {
+      TowerOrderData &D = Ds[k - 1];
+      tower_order_device(c, W, W0, k, D);
+      {
+        std::lock_guard<std::mutex> g(mu);
+        T.descent_steps += D.steps;
       }
-      std::unordered_map<std::vector<uint32_t>, int64_t, LabelHash> minima;
-      for (uint32_t i = 0; i < D.minima; ++i)
-        minima.emplace(std::vector<uint32_t>(D.minima_labels.begin() + (size_t)i * k,
-                                             D.minima_labels.begin() + (size_t)(i + 1) * k), i);
-      for (uint32_t s = 0; s < D.saddles; ++s)
-        for (uint32_t a = D.arm_base[s]; a < D.arm_base[s + 1]; ++a) {
-          if (D.carrier[a] != kInvalid) continue;
-          std::vector<uint32_t> arm;
-          const uint32_t removed = D.sup[4 * s + (a - D.arm_base[s])];
-          for (int q = 0; q < 11; ++q) {
-            uint32_t x = D.saddle_labels[(size_t)s * 11 + q];
-            if (x != kInvalid && x != removed) arm.push_back(x);
-          }
-          std::sort(arm.begin(), arm.end());
-          D.carrier[a] = (uint32_t)desc->descend(arm, minima);
+      // carriers the device could not settle: exact host descents
+      bool any_host = false;
+      for (uint32_t a = 0; a < D.arms; ++a) any_host |= D.carrier[a] == kInvalid;
+      if (any_host) {
+        std::lock_guard<std::mutex> g(mu);
+        if (!desc) {
+          xyz.resize(3 * n);
+          canonical_points(c, xyz.data(), nullptr);
+          desc = std::make_unique<Descender>(xyz, n, exact_cloud(c));
         }
-      tm.mark("tower arms (host exact)");
+        std::unordered_map<std::vector<uint32_t>, int64_t, LabelHash> minima;
+        for (uint32_t i = 0; i < D.minima; ++i)
+          minima.emplace(std::vector<uint32_t>(D.minima_labels.begin() + (size_t)i * k,
+                                               D.minima_labels.begin() + (size_t)(i + 1) * k), i);
+        for (uint32_t s = 0; s < D.saddles; ++s)
+          for (uint32_t a = D.arm_base[s]; a < D.arm_base[s + 1]; ++a) {
+            if (D.carrier[a] != kInvalid) continue;
+            std::vector<uint32_t> arm;
+            const uint32_t removed = D.sup[4 * s + (a - D.arm_base[s])];
+            for (int q = 0; q < 11; ++q) {
+              uint32_t x = D.saddle_labels[(size_t)s * 11 + q];
+              if (x != kInvalid && x != removed) arm.push_back(x);
+            }
+            std::sort(arm.begin(), arm.end());
+            D.carrier[a] = (uint32_t)desc->descend(arm, minima);
+          }
+      }
+      std::lock_guard<std::mutex> g(mu);
+      workers.emplace_back([&, k] {
+        try {
+          contract_order(k, n, Ds[k - 1], forests[k - 1]);
+        } catch (...) {
+          errors[k - 1] = std::current_exception();
+        }
+      });
     }
-    // the order's contraction runs on the host while the device resolves the next order
-    workers.emplace_back([&, k] {
+  };
+  std::exception_ptr lane_error[2];
+  std::thread second;
+  if (K >= 2)
+    second = std::thread([&] {
       try {
-        contract_order(k, n, Ds[k - 1], forests[k - 1]);
+        lane(*c.tower_work2, K - 1);
       } catch (...) {
-        errors[k - 1] = std::current_exception();
+        lane_error[1] = std::current_exception();
       }
     });
+  try {
+    lane(W0, K);
+  } catch (...) {
+    lane_error[0] = std::current_exception();
+  }
+  if (second.joinable()) second.join();
+  for (auto &e : lane_error)
+    if (e) {
+      for (auto &t : workers)
+        if (t.joinable()) t.join();
+      std::rethrow_exception(e);
+    }
+  for (TowerWork *W : {c.tower_work.get(), c.tower_work2.get()}) {
+    if (!W) continue;
+    c.stats.arm_ms += W->arm_ms;
+    c.stats.arm_launches += W->arm_launches;
+    c.stats.arm_bytes += W->arm_bytes;
+    W->arm_ms = 0;
+    W->arm_launches = 0;
+    W->arm_bytes = 0;
   }
   for (auto &t : workers) t.join();
   for (auto &e : errors)

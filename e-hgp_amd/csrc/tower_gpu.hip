@@ -837,15 +837,14 @@ void tower_rank_index(Context &c) {
   std::copy(h + 256, h + 512, W.rank_hi);
 }
 
-void tower_order_device(Context &c, int k, TowerOrderData &out) {
+void tower_order_device(Context &c, TowerWork &W, const TowerWork &R, int k, TowerOrderData &out) {
   CatalogBuffers &B = *c.catalog;
-  hipStream_t s = c.stream;
+  hipStream_t s = W.stream;
   out = TowerOrderData{};
-  TowerWork &W = *c.tower_work;
   // minima of order k (k >= 2) and saddles of order k: records of closed
   // rank k and k + 1, as runs of the rank index
-  const uint32_t M = W.rank_hi[k] - W.rank_lo[k], S = W.rank_hi[k + 1] - W.rank_lo[k + 1];
-  const uint32_t *minima_pos = W.rank_vals.p + W.rank_lo[k], *saddle_pos = W.rank_vals.p + W.rank_lo[k + 1];
+  const uint32_t M = R.rank_hi[k] - R.rank_lo[k], S = R.rank_hi[k + 1] - R.rank_lo[k + 1];
+  const uint32_t *minima_pos = R.rank_vals.p + R.rank_lo[k], *saddle_pos = R.rank_vals.p + R.rank_lo[k + 1];
   out.minima = M;
   uint64_t cap = 1;
   while (cap < 2ull * M + 16) cap <<= 1;
@@ -866,7 +865,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   sizes.ensure(S);
   hipLaunchKernelGGL(k_arm_sizes, dim3(blocks_for(S, 256)), dim3(256), 0, s, B.events.p, B.order.p,
                      saddle_pos, (int64_t)S, sizes.p);
-  c.scanner.inclusive(sizes.p, S, s);
+  W.scanner.inclusive(sizes.p, S, s);
   uint32_t arms = 0;
   copy_wait(&arms, sizes.p + S - 1, 4, hipMemcpyDeviceToHost, s);
   // exclusive bases: shift the inclusive scan by one
@@ -894,7 +893,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   queue.ensure(arms);
   qcount.ensure(1);
   MHGP_HIP(hipMemsetAsync(qcount.p, 0, 8, s));
-  c.arm_clock.start(s);
+  W.clock.start(s);
   auto &qkey = W.qkey;
   qkey.ensure(arms);
   // distinct facets of the arms to descend (load factor below 1/2)
@@ -918,7 +917,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   if (nq > 1) {  // spatial order: neighbouring lanes walk neighbouring nodes
     int bits = 8;
     while (bits < 32 && (1ll << bits) < c.n) bits += 8;
-    c.sorter.sort(qkey.p, queue.p, (int64_t)nq, 0, bits, s);
+    W.sorter.sort(qkey.p, queue.p, (int64_t)nq, 0, bits, s);
   }
   uint64_t steps_total = 0;
   if (nq) {
@@ -945,13 +944,13 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
   if (na)
     hipLaunchKernelGGL(k_alias_carriers, dim3(blocks_for(na, 256)), dim3(256), 0, s, W.alias_arm.p, W.alias_rep.p,
                        na, carrier.p);
-  c.arm_clock.stop(s);
+  W.clock.stop(s);
   MHGP_LAUNCH_CHECK();
-  c.stats.arm_ms += c.arm_clock.ms();
-  c.stats.arm_launches++;
+  W.arm_ms += W.clock.ms();
+  W.arm_launches++;
   // algorithmic bytes: each saddle record and its order/position entries read
   // once, each arm's carrier written once and one hash probe (key + value)
-  c.stats.arm_bytes += (int64_t)S * ((int64_t)sizeof(EventRec) + 8) + (int64_t)arms * (4 + 12);
+  W.arm_bytes += (int64_t)S * ((int64_t)sizeof(EventRec) + 8) + (int64_t)arms * (4 + 12);
   scls.ensure(S);
   slev.ensure(S);
   hipLaunchKernelGGL(k_saddle_meta, dim3(blocks_for(S, 256)), dim3(256), 0, s, B.events.p, B.order.p, B.cls.p,
@@ -995,7 +994,7 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
     eflag.ensure(arms);
     hipLaunchKernelGGL(k_edge_flags, ga, tb, 0, s, arm_saddle.p, base.p, sizes.p, carrier.p, first.p, rep.p,
                        all_owned, (int64_t)arms, eflag.p);
-    c.scanner.inclusive(eflag.p, arms, s);
+    W.scanner.inclusive(eflag.p, arms, s);
     uint32_t E = 0, bad = 0;
     MHGP_HIP(hipMemcpyAsync(&bad, err.p, 4, hipMemcpyDeviceToHost, s));
     copy_wait(&E, eflag.p + arms - 1, 4, hipMemcpyDeviceToHost, s);
@@ -1065,13 +1064,13 @@ void tower_order_device(Context &c, int k, TowerOrderData &out) {
       dense.ensure(minima_n);
       hipLaunchKernelGGL(k_fill_u32, gm, tb, 0, s, dense.p, minima_n, 0u);
       hipLaunchKernelGGL(k_flag_used, gs, tb, 0, s, (int64_t)S, keep.p, base.p, sizes.p, carrier.p, rep.p, dense.p);
-      c.scanner.inclusive(dense.p, minima_n, s);
+      W.scanner.inclusive(dense.p, minima_n, s);
       copy_wait(&Mdense, dense.p + minima_n - 1, 4, hipMemcpyDeviceToHost, s);
       d_rep = rep.p;
       d_dense = dense.p;
     }
-    c.scanner.inclusive(keep.p, S, s);
-    c.scanner.inclusive(kept.p, S, s);
+    W.scanner.inclusive(keep.p, S, s);
+    W.scanner.inclusive(kept.p, S, s);
     MHGP_HIP(hipMemcpyAsync(&S2, keep.p + S - 1, 4, hipMemcpyDeviceToHost, s));
     MHGP_HIP(hipMemcpyAsync(&arms2, kept.p + S - 1, 4, hipMemcpyDeviceToHost, s));
     MHGP_HIP(hipStreamSynchronize(s));

@@ -5,6 +5,8 @@
 #include <vector>
 
 #include "context.hpp"
+#include "radix_sort.hpp"
+#include "scan.hpp"
 
 namespace mhgp {
 
@@ -28,6 +30,21 @@ struct TowerOrderData {
 // Device scratch of the per-order tower work, kept by the context; a buffer
 // is reallocated only when an order needs more than it holds.
 struct TowerWork {
+  // the lane's stream (the context's for the first lane), scan and sort
+  // scratch, kernel clock and statistics
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  Scanner scanner;
+  RadixSorter sorter;
+  Context::KernelClock clock;
+  double arm_ms = 0;
+  int64_t arm_launches = 0, arm_bytes = 0;
+  TowerWork() = default;
+  TowerWork(const TowerWork &) = delete;
+  TowerWork &operator=(const TowerWork &) = delete;
+  ~TowerWork() {
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+  }
   // records grouped by closed rank: rank r occupies rank_vals[rank_lo[r], rank_hi[r])
   DevBuf<uint64_t> rank_keys;
   DevBuf<uint32_t> rank_vals, rank_bounds;
@@ -57,7 +74,8 @@ struct TowerWork {
 
 // Groups the catalogue's records by closed rank (once per tower call).
 void tower_rank_index(Context &c);
-// Resolves the arms of order k and reduces its saddles (after tower_rank_index).
-void tower_order_device(Context &c, int k, TowerOrderData &out);
+// Resolves the arms of order k and reduces its saddles (after
+// tower_rank_index) on the lane W; R holds the rank index.
+void tower_order_device(Context &c, TowerWork &W, const TowerWork &R, int k, TowerOrderData &out);
 
 }  // namespace mhgp
