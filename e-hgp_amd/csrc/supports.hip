@@ -52,7 +52,7 @@ int64_t catalog_device_bytes(const CatalogBuffers &b) {
 
 namespace {
 
-constexpr int kBlock = 512;
+constexpr int kBlock = 256;
 
 // LDS written by some lanes of a wave, then read by other lanes of it
 __device__ inline void wave_sync() {
@@ -862,7 +862,7 @@ __device__ void anchor_lds(const Params &P) {
   anchor_work<false>(P, w, u);
 }
 
-__global__ void __launch_bounds__(kBlock, 4) k_anchor_lds(Params P) { anchor_lds<kGatherCap, kRegionCap>(P); }
+__global__ void __launch_bounds__(kBlock, 3) k_anchor_lds(Params P) { anchor_lds<kGatherCap, kRegionCap>(P); }
 __global__ void __launch_bounds__(kBlock, 2) k_anchor_lds_wide(Params P) {
   anchor_lds<kGatherCapWide, kRegionCapWide>(P);
 }
