@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(64) k_closed_ball(BallParams B) {
     ts.init(a, b);
     good = ts.D > 1e-6 * ts.DA;
     const double s2 = good ? 0.5 / ts.D : 0.0;
-    off = {(ts.al * a.x + ts.be * b.x) * s2, (ts.al * a.y + ts.be * b.y) * s2, (ts.al * a.z + ts.be * b.z) * s2};
+    off = {ts.g.x * s2, ts.g.y * s2, ts.g.z * s2};
   } else {
     const V3 b = sub(pt(B, sid[2]), o), d = sub(pt(B, sid[3]), o);
     tt.init(a, b, d);

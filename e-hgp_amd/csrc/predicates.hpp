@@ -13,7 +13,9 @@
 //  pair:        P = p.(p - a)                         (diametral sphere)
 //  triangle:    D = aa bb - ab^2,  al = bb (aa - ab),  be = aa (bb - ab)
 //               well-centred <=> al > 0, be > 0, 2D - al - be > 0
-//               P = D pp - al pa - be pb
+//               P = D pp - al pa - be pb = D pp - p.g,  g = al a + be b
+//               (the bound of p.g is sum |p_i| (|al a_i| + |be b_i|), the
+//               same magnitude polynomial as for the expanded form)
 //               r^2 = aa bb |a - b|^2 / (4 D)
 //  tetrahedron: V = a.(b x d),  N = aa (b x d) + bb (d x a) + dd (a x b)
 //               well-centred <=> N.(b x d), N.(d x a), N.(a x b) > 0 and
@@ -66,7 +68,7 @@ struct PairSphere {
 };
 
 struct TriSphere {
-  V3 a, b, n;
+  V3 a, b, n, nA, g, gA;
   double aa, bb, ab, abA, D, DA, al, alA, be, beA;
   // returns +1 certainly well-centred, -1 certainly not or dependent, 0 undecided
   __host__ __device__ int init(const V3 &va, const V3 &vb) {
@@ -83,6 +85,9 @@ struct TriSphere {
     be = aa * (bb - ab);
     beA = aa * (bb + abA);
     n = cross(a, b);
+    nA = across(a, b);
+    g = {al * a.x + be * b.x, al * a.y + be * b.y, al * a.z + be * b.z};
+    gA = {alA * fabs(a.x) + beA * fabs(b.x), alA * fabs(a.y) + beA * fabs(b.y), alA * fabs(a.z) + beA * fabs(b.z)};
     int sd = cert_sign(D, DA);
     if (sd <= 0) return sd < 0 ? -1 : 0;  // D >= 0 always; 0 is dependent or undecided
     int s1 = cert_sign(al, alA), s2 = cert_sign(be, beA);
@@ -92,11 +97,11 @@ struct TriSphere {
     return 1;
   }
   __host__ __device__ bool nondegenerate() const { return cert_sign(D, DA) > 0; }
-  // scaled power P = D pp - al pa - be pb and its magnitude bound
+  // scaled power P = D pp - p.g and its magnitude bound
   __host__ __device__ void power(const V3 &p, double &P, double &PA) const {
-    double pp = dot(p, p);
-    P = D * pp - al * dot(p, a) - be * dot(p, b);
-    PA = DA * pp + alA * adot(p, a) + beA * adot(p, b);
+    const double pp = dot(p, p);
+    P = D * pp - dot(p, g);
+    PA = DA * pp + adot(p, gA);
   }
   __host__ __device__ int side(const V3 &p) const {
     double P, PA;
@@ -106,7 +111,7 @@ struct TriSphere {
   // signed height H = p.(a x b) and its bound
   __host__ __device__ void height(const V3 &p, double &H, double &HA) const {
     H = dot(p, n);
-    HA = adot(vabs(p), across(a, b));
+    HA = adot(p, nA);
   }
   __host__ __device__ double level() const {
     V3 c = sub(a, b);

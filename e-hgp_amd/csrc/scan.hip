@@ -1,5 +1,8 @@
-// Three-phase inclusive scan: per-block scans with block totals, a scan of
-// the totals (recursively), then a uniform add of the block offsets.
+// Three-phase inclusive scan: per-tile scans with tile totals, a scan of the
+// totals (recursively), then a uniform add of the tile offsets.  A tile is
+// 4 096 values: 256 lanes hold 16 consecutive values each (four 16-byte
+// loads), scan them in registers, and combine the lane totals with a
+// shuffle scan per wave and one LDS exchange of the four wave totals.
 #include <stdexcept>
 
 #include "scan.hpp"
@@ -8,41 +11,82 @@ namespace mhgp {
 
 namespace {
 
-constexpr int kScanBlock = 1024;
-constexpr int kPerThread = 4;
+constexpr int kScanBlock = 256;
+constexpr int kPerThread = 16;
 constexpr int kTileItems = kScanBlock * kPerThread;
 
-__global__ void __launch_bounds__(kScanBlock) k_block_scan(uint32_t *data, int64_t n, uint32_t *sums) {
-  __shared__ uint32_t s[kScanBlock];
-  const int t = threadIdx.x;
-  const int64_t base = (int64_t)blockIdx.x * kTileItems + (int64_t)t * kPerThread;
-  uint32_t v[kPerThread];
-  uint32_t run = 0;
-  for (int i = 0; i < kPerThread; ++i) {
-    v[i] = (base + i < n) ? data[base + i] : 0;
-    run += v[i];
-    v[i] = run;
+template <bool VEC>
+__device__ inline void load16(const uint32_t *data, int64_t base, int64_t n, uint32_t (&v)[kPerThread]) {
+  if (VEC && base + kPerThread <= n) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(data + base);
+#pragma unroll
+    for (int j = 0; j < kPerThread / 4; ++j) {
+      const uint4 q = p[j];
+      v[4 * j] = q.x;
+      v[4 * j + 1] = q.y;
+      v[4 * j + 2] = q.z;
+      v[4 * j + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) v[i] = base + i < n ? data[base + i] : 0u;
   }
-  s[t] = run;
-  __syncthreads();
-  for (int off = 1; off < kScanBlock; off <<= 1) {
-    uint32_t x = t >= off ? s[t - off] : 0;
-    __syncthreads();
-    s[t] += x;
-    __syncthreads();
-  }
-  const uint32_t prefix = s[t] - run;
-  for (int i = 0; i < kPerThread; ++i)
-    if (base + i < n) data[base + i] = v[i] + prefix;
-  if (t == kScanBlock - 1) sums[blockIdx.x] = s[t];
 }
 
-__global__ void k_add_offsets(uint32_t *data, int64_t n, const uint32_t *sums) {
+template <bool VEC>
+__device__ inline void store16(uint32_t *data, int64_t base, int64_t n, const uint32_t (&v)[kPerThread]) {
+  if (VEC && base + kPerThread <= n) {
+    uint4 *p = reinterpret_cast<uint4 *>(data + base);
+#pragma unroll
+    for (int j = 0; j < kPerThread / 4; ++j) p[j] = make_uint4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i)
+      if (base + i < n) data[base + i] = v[i];
+  }
+}
+
+template <bool VEC>
+__global__ void __launch_bounds__(kScanBlock) k_block_scan(uint32_t *data, int64_t n, uint32_t *sums) {
+  __shared__ uint32_t wsum[kScanBlock / 64];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int64_t base = (int64_t)blockIdx.x * kTileItems + (int64_t)t * kPerThread;
+  uint32_t v[kPerThread];
+  load16<VEC>(data, base, n, v);
+#pragma unroll
+  for (int i = 1; i < kPerThread; ++i) v[i] += v[i - 1];
+  // inclusive scan of the lane totals across the wave
+  uint32_t x = v[kPerThread - 1];
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, (unsigned)d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kScanBlock / 64; ++w) {
+    before += w < wave ? wsum[w] : 0u;
+    total += wsum[w];
+  }
+  const uint32_t prefix = before + x - v[kPerThread - 1];
+#pragma unroll
+  for (int i = 0; i < kPerThread; ++i) v[i] += prefix;
+  store16<VEC>(data, base, n, v);
+  if (t == 0) sums[blockIdx.x] = total;
+}
+
+template <bool VEC>
+__global__ void __launch_bounds__(kScanBlock) k_add_offsets(uint32_t *data, int64_t n, const uint32_t *sums) {
   if (blockIdx.x == 0) return;
   const uint32_t add = sums[blockIdx.x - 1];
-  const int64_t base = (int64_t)blockIdx.x * kTileItems;
-  for (int i = threadIdx.x; i < kTileItems; i += blockDim.x)
-    if (base + i < n) data[base + i] += add;
+  const int64_t base = (int64_t)blockIdx.x * kTileItems + (int64_t)threadIdx.x * kPerThread;
+  uint32_t v[kPerThread];
+  load16<VEC>(data, base, n, v);
+#pragma unroll
+  for (int i = 0; i < kPerThread; ++i) v[i] += add;
+  store16<VEC>(data, base, n, v);
 }
 
 }  // namespace
@@ -52,10 +96,17 @@ void Scanner::inclusive(uint32_t *data, int64_t n, hipStream_t s, int depth) {
   if (depth >= 4) throw std::runtime_error("scan too deep");
   const int64_t blocks = (n + kTileItems - 1) / kTileItems;
   uint32_t *bs = level[depth].ensure(blocks);
-  hipLaunchKernelGGL(k_block_scan, dim3(blocks), dim3(kScanBlock), 0, s, data, n, bs);
+  const bool vec = ((uintptr_t)data & 15u) == 0;
+  if (vec)
+    hipLaunchKernelGGL(k_block_scan<true>, dim3(blocks), dim3(kScanBlock), 0, s, data, n, bs);
+  else
+    hipLaunchKernelGGL(k_block_scan<false>, dim3(blocks), dim3(kScanBlock), 0, s, data, n, bs);
   if (blocks > 1) {
     inclusive(bs, blocks, s, depth + 1);
-    hipLaunchKernelGGL(k_add_offsets, dim3(blocks), dim3(256), 0, s, data, n, bs);
+    if (vec)
+      hipLaunchKernelGGL(k_add_offsets<true>, dim3(blocks), dim3(kScanBlock), 0, s, data, n, bs);
+    else
+      hipLaunchKernelGGL(k_add_offsets<false>, dim3(blocks), dim3(kScanBlock), 0, s, data, n, bs);
   }
   MHGP_LAUNCH_CHECK();
 }

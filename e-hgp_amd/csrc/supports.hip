@@ -212,10 +212,10 @@ __device__ void process_pair(const Params &P, const Work &w, uint32_t u, uint32_
   emit_event(P, u, &vid, 2, nint, interior, sph.level(), 8 * kCert);
 }
 
-__device__ void process_tet(const Params &P, const Work &w, uint32_t u, uint32_t M, const TriSphere &f,
-                            uint32_t vi, uint32_t wi, uint32_t zi, uint32_t *region_slot) {
+__device__ void process_tet(const Params &P, const Work &w, uint32_t u, uint32_t M, uint32_t vi, uint32_t wi,
+                            uint32_t zi, uint32_t *region_slot) {
   TetSphere t;
-  int wc = t.init(f.a, f.b, gpt(w, zi));
+  int wc = t.init(gpt(w, vi), gpt(w, wi), gpt(w, zi));
   uint32_t ids[3] = {w.gid[vi], w.gid[wi], w.gid[zi]};
   if (wc < 0) return;
   atomicAdd(w.cand, 1u);
@@ -331,8 +331,8 @@ struct WaveQueues {
   uint32_t *nside, *ntet;
 };
 
-__device__ void process_tet(const Params &P, const Work &w, uint32_t u, uint32_t M, const TriSphere &f,
-                            uint32_t vi, uint32_t wi, uint32_t zi, uint32_t *region_slot);
+__device__ void process_tet(const Params &P, const Work &w, uint32_t u, uint32_t M, uint32_t vi, uint32_t wi,
+                            uint32_t zi, uint32_t *region_slot);
 
 __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uint32_t M, const TriSphere &f,
                                  const Capsule &cap, uint32_t vi, uint32_t wi, uint32_t wid, int side,
@@ -389,7 +389,7 @@ __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uin
       sweep_key(Pw, PA, H, HA, tau, terr);
       if (tau - terr > thr) continue;
       if (cin_side + beh.count_above(tau + terr) >= K) continue;
-      process_tet(P, w, u, M, f, vi, wi, zi, region_slot);
+      process_tet(P, w, u, M, vi, wi, zi, region_slot);
     }
     return;
   }
@@ -416,7 +416,7 @@ __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uin
         continue;
       }
     }
-    process_tet(P, w, u, M, f, vi, wi, zi, region_slot);
+    process_tet(P, w, u, M, vi, wi, zi, region_slot);
   }
 }
 
@@ -427,8 +427,7 @@ __device__ inline Capsule face_capsule(const TriSphere &f, double rmax) {
   cap.bound2 = INFINITY;
   if (f.D > 1e-6 * f.DA) {
     const double s2 = 0.5 / f.D;
-    cap.c3 = {(f.al * f.a.x + f.be * f.b.x) * s2, (f.al * f.a.y + f.be * f.b.y) * s2,
-              (f.al * f.a.z + f.be * f.b.z) * s2};
+    cap.c3 = {f.g.x * s2, f.g.y * s2, f.g.z * s2};
     const double r3 = sqrt(dot(cap.c3, cap.c3));
     const double T = sqrt(fmax(0.0, rmax * rmax - r3 * r3));
     const double Rb = fmax(rmax, r3);
@@ -531,9 +530,7 @@ __device__ void side_item(const Params &P, const Work &w, uint32_t u, uint32_t M
 __device__ void tet_item(const Params &P, const Work &w, uint32_t u, uint32_t M, uint32_t item,
                          uint32_t *region_slot) {
   const uint32_t vi = item & 1023u, wi = (item >> 10) & 1023u, zi = item >> 20;
-  TriSphere f;
-  f.init(gpt(w, vi), gpt(w, wi));
-  process_tet(P, w, u, M, f, vi, wi, zi, region_slot);
+  process_tet(P, w, u, M, vi, wi, zi, region_slot);
 }
 
 // Cell direction and angular radius in FP32, rounded so that the cell only
