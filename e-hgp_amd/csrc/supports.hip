@@ -99,7 +99,7 @@ struct Work {
   double *gx, *gy, *gz;
   uint32_t *gid;
   double *rho;
-  uint32_t *ridx;
+  uint32_t *ridx;    // gathered index of each region slot (until the points are compacted)
   uint64_t *rmask;  // 6 words per member
   uint32_t *kidx;
   float4 *fc;       // FP32 offset and distance per gathered point (certificate and masks)
@@ -152,7 +152,7 @@ __device__ void emit_fallback(const Params &P, const Work &w, uint32_t u, uint32
     // first fallback of this anchor publishes its region
     unsigned long long off = atomicAdd(&P.counters[2], (unsigned long long)M);
     if (off + M <= P.fb_region_cap) {
-      for (uint32_t s = 0; s < M; ++s) P.fb_region[off + s] = w.gid[w.ridx[s]];
+      for (uint32_t s = 0; s < M; ++s) P.fb_region[off + s] = w.gid[s];
       atomicCAS(region_slot, expected, (uint32_t)off);
     }
   }
@@ -176,7 +176,7 @@ __device__ int classify(const Params &P, const Work &w, uint32_t M, const S &sph
   const int cap = P.s_max - m;
   int undecided = 0;
   for (uint32_t s = 0; s < M; ++s) {
-    uint32_t gi = w.ridx[s];
+    const uint32_t gi = s;
     bool on_support = false;
     for (int k = 0; k < m - 1; ++k) on_support |= (gi == skip[k]);
     if (on_support) continue;
@@ -325,7 +325,7 @@ struct Capsule {
 constexpr uint32_t kSideQ = 128;  // >= 2 items for each of a wave's 64 faces
 constexpr uint32_t kTetQ = 256;
 struct WaveQueues {
-  uint32_t *side;    // vi | wi << 10 | side << 20 | cin << 21 (gathered indices, < 1024)
+  uint32_t *side;    // vi | wi << 10 | side << 20 | cin << 21 (region slots, < 1024)
   double *side_r;    // the face's rmax: the side pass rebuilds exactly the face pass's capsule
   uint32_t *tet;     // vi | wi << 10 | zi << 20
   uint32_t *nside, *ntet;
@@ -346,7 +346,7 @@ __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uin
   bool undecided = false;
 #pragma unroll 1
   for (uint32_t s = 0; s < M; ++s) {
-    const uint32_t gi = w.ridx[s];
+    const uint32_t gi = s;
     if (gi == vi || gi == wi) continue;
     const V3 p = gpt(w, gi);
     // outside every feasible sphere of the pencil: it can neither be an apex
@@ -375,7 +375,7 @@ __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uin
     // a point outside the list may still qualify (near-equal keys): full scan
 #pragma unroll 1
     for (uint32_t s = 0; s < M; ++s) {
-      const uint32_t zi = w.ridx[s];
+      const uint32_t zi = s;
       if (zi == vi || zi == wi || w.gid[zi] <= wid) continue;
       const V3 p = gpt(w, zi);
       if (cap.outside(p)) continue;
@@ -407,7 +407,7 @@ __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uin
     uint32_t slot = 0;  // selected by static index, the list stays in registers
 #pragma unroll
     for (int q = 0; q < SweepList::kCap; ++q) slot = q == j ? top.slot[q] : slot;
-    const uint32_t zi = w.ridx[slot];
+    const uint32_t zi = slot;
     if (w.gid[zi] <= wid) continue;
     if (q) {
       const uint32_t at = atomicAdd(q->ntet, 1u);
@@ -467,7 +467,7 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
   uint32_t tri_int[kMaxInterior];
 #pragma unroll 1
   for (uint32_t s = 0; s < M; ++s) {
-    const uint32_t gi = w.ridx[s];
+    const uint32_t gi = s;
     if (gi == vi || gi == wi) continue;
     const V3 p = gpt(w, gi);
     if (cap.outside(p)) continue;  // outside the triangle's sphere and every feasible one
@@ -697,18 +697,21 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   }
   __syncthreads();
   if (sflag) return;
-  // 2. region membership masks
-  if (G <= 4 * blockDim.x) {
-    // ordered compaction: members keep the gathered (distance) order
-    const uint32_t chunk = (G + blockDim.x - 1) / blockDim.x;
-    const uint32_t b0 = t * chunk;
+  // 2. region membership masks, compacted in gathered order (chunks of four
+  // points per lane; the LDS path has a single chunk, so every FP32 offset is
+  // read before the first mask overwrites the area they share)
+  uint32_t members = 0;
+  for (uint32_t c0 = 0; c0 < G; c0 += 4 * blockDim.x) {
+    const uint32_t g = min(G - c0, 4 * blockDim.x);
+    const uint32_t chunk = (g + blockDim.x - 1) / blockDim.x;
+    const uint32_t b0 = c0 + t * chunk;
     uint64_t mk[4][6];  // static indices only: the masks stay in registers
     bool mem[4];
     uint32_t cnt = 0;
 #pragma unroll
     for (uint32_t r = 0; r < 4; ++r) {
       mem[r] = false;
-      if (r < chunk && b0 + r < G) {
+      if (r < chunk && t * chunk + r < g) {
         mem[r] = member_mask(P, w, b0 + r, mk[r]);
         cnt += mem[r];
       }
@@ -721,7 +724,7 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
       scan_buf[t] += v;
       __syncthreads();
     }
-    uint32_t slot = scan_buf[t] - cnt;
+    uint32_t slot = members + scan_buf[t] - cnt;
 #pragma unroll
     for (uint32_t r = 0; r < 4; ++r) {
       if (!mem[r]) continue;
@@ -732,19 +735,10 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
       }
       slot++;
     }
-    if (t == (int)blockDim.x - 1) sM = scan_buf[t];
-  } else {
-    for (uint32_t i = t; i < G; i += blockDim.x) {
-      uint64_t mask[6];
-      if (member_mask(P, w, i, mask)) {
-        uint32_t slot = atomicAdd(&sM, 1u);
-        if (slot < P.rcap) {
-          w.ridx[slot] = i;
-          for (int k = 0; k < 6; ++k) w.rmask[6 * slot + k] = mask[k];
-        }
-      }
-    }
+    members += scan_buf[blockDim.x - 1];
+    __syncthreads();
   }
+  if (t == 0) sM = members;
   __syncthreads();
   const uint32_t M = sM;
   if (M > P.rcap) {
@@ -759,12 +753,36 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     if (t == 0) P.status[u] = ST_PROBED;
     return;
   }
+  // the region's points move to the front of the point arrays, in region
+  // order: from here on a region slot indexes them directly.  ridx ascends,
+  // so every source lies at or after its destination and after the
+  // destinations of earlier chunks; each chunk reads before it writes.
+  for (uint32_t c0 = 0; c0 < M; c0 += blockDim.x) {
+    const uint32_t s = c0 + t;
+    double x = 0, y = 0, z = 0;
+    uint32_t id = 0;
+    if (s < M) {
+      const uint32_t gi = w.ridx[s];
+      x = w.gx[gi];
+      y = w.gy[gi];
+      z = w.gz[gi];
+      id = w.gid[gi];
+    }
+    __syncthreads();
+    if (s < M) {
+      w.gx[s] = x;
+      w.gy[s] = y;
+      w.gz[s] = z;
+      w.gid[s] = id;
+    }
+    __syncthreads();
+  }
   for (uint32_t s = t; s < M; s += blockDim.x)
-    if (w.gid[w.ridx[s]] > u) w.kidx[atomicAdd(&sK, 1u)] = s;
+    if (w.gid[s] > u) w.kidx[atomicAdd(&sK, 1u)] = s;
   __syncthreads();
   const uint32_t Kc = sK;
   // 3. pairs (u, v), v > u
-  for (uint32_t k = t; k < Kc; k += blockDim.x) process_pair(P, w, u, M, w.ridx[w.kidx[k]], &sregion);
+  for (uint32_t k = t; k < Kc; k += blockDim.x) process_pair(P, w, u, M, w.kidx[k], &sregion);
   // 4. faces (u, v, w), u < v < w, whose members share a cell.  Each wave
   // takes 64 candidate faces at a time: first passes one per lane, then the
   // queued sides and tetrahedra one per lane (LDS path; the global-memory
@@ -792,7 +810,7 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
       for (int k = 0; k < 6; ++k) share |= (w.rmask[6 * si + k] & w.rmask[6 * sj + k]) != 0;
       if (share) {
         // a sphere through u, v, w has radius >= r3; skip faces beyond every shared cell
-        const uint32_t vi = w.ridx[si], wi = w.ridx[sj];
+        const uint32_t vi = si, wi = sj;
         const V3 a = gpt(w, vi), b = gpt(w, wi);
         const double aa = dot(a, a), bb = dot(b, b), ab = dot(a, b);
         const double D = aa * bb - ab * ab;
