@@ -1,13 +1,17 @@
 // Stable LSD radix sort of (uint64 key, uint32 value) pairs, 8-bit digits.
 //
-// Pass structure (one pass per digit):
-//   1. tile histograms: each 1024-key tile counts its digits in LDS;
+// Pass structure (one pass per digit), on tiles of 4 096 keys:
+//   1. tile histograms: 64-key groups of equal digits are found with eight
+//      ballots and counted with one LDS atomic per group;
 //   2. multi-block inclusive scan over [digit][tile]; the entry before
 //      (digit, tile) is that tile's base offset for the digit (digit-major,
 //      so equal digits keep tile order);
-//   3. scatter: one 64-lane wavefront per tile walks its keys in order, ranks
-//      equal digits inside each 64-key chunk with 8 ballots, and keeps a
-//      per-digit running count in LDS, which makes the pass stable.
+//   3. scatter: each of the four waves of a tile ranks its 1 024 keys in
+//      order (64-key chunks, eight ballots, a per-digit running count in
+//      LDS), the tile is reordered by digit in LDS, and the reordered tile is
+//      written out in index order, so the stores of a digit run are
+//      contiguous.  Key order inside a digit is (wave, chunk, lane), the
+//      input order: the pass is stable.
 // Passes whose digit is the same for every key (detected on the device by
 // the scan) copy instead of scattering, so ping-pong buffers stay consistent.
 #include "radix_sort.hpp"
@@ -16,21 +20,46 @@ namespace mhgp {
 
 namespace {
 
-constexpr int kTile = 1024;
+constexpr int kTile = 4096;
 constexpr int kDigits = 256;
+constexpr int kSortBlock = 256;
+constexpr int kWaves = kSortBlock / 64;
+constexpr int kChunks = kTile / kSortBlock;  // 64-key chunks per wave
 
-__global__ void rs_histogram(const uint64_t *__restrict__ keys, int64_t n, int shift,
-                             uint32_t *__restrict__ counts, int tiles) {
+__device__ inline uint64_t lanes_below() {
+  const int lane = (int)(threadIdx.x & 63u);
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Lanes of the wave holding a valid key with the same digit as this lane.
+__device__ inline uint64_t digit_match(bool valid, uint32_t dig) {
+  uint64_t match = __ballot(valid);
+#pragma unroll
+  for (int bit = 0; bit < 8; ++bit) {
+    const uint64_t bal = __ballot(valid && ((dig >> bit) & 1u));
+    match &= ((dig >> bit) & 1u) ? bal : ~bal;
+  }
+  return match;
+}
+
+__global__ void __launch_bounds__(kSortBlock) rs_histogram(const uint64_t *__restrict__ keys, int64_t n, int shift,
+                                                           uint32_t *__restrict__ counts, int tiles) {
   __shared__ uint32_t h[kDigits];
-  for (int i = threadIdx.x; i < kDigits; i += blockDim.x) h[i] = 0;
+  const int t = threadIdx.x;
+  h[t] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kTile;
-  for (int i = threadIdx.x; i < kTile; i += blockDim.x) {
-    int64_t g = base + i;
-    if (g < n) atomicAdd(&h[(keys[g] >> shift) & 0xff], 1u);
+  const uint64_t below = lanes_below();
+  for (int i = t; i < kTile; i += kSortBlock) {
+    const int64_t g = base + i;
+    const bool valid = g < n;
+    const uint32_t dig = valid ? (uint32_t)((keys[g] >> shift) & 0xff) : 0u;
+    const uint64_t match = digit_match(valid, dig);
+    // the lowest lane of each group adds the group
+    if (valid && (match & below) == 0) atomicAdd(&h[dig], (uint32_t)__popcll(match));
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < kDigits; d += blockDim.x) counts[(size_t)d * tiles + blockIdx.x] = h[d];
+  counts[(size_t)t * tiles + blockIdx.x] = h[t];
 }
 
 // flags[0] = 1 when one digit holds every key (the pass is the identity
@@ -47,52 +76,98 @@ __global__ void rs_identity(const uint32_t *__restrict__ incl, int64_t n, int ti
   if (d == 0) flags[0] = full;
 }
 
-__global__ void __launch_bounds__(64) rs_scatter(const uint64_t *__restrict__ kin,
-                                                 const uint32_t *__restrict__ vin,
-                                                 uint64_t *__restrict__ kout,
-                                                 uint32_t *__restrict__ vout, int64_t n, int shift,
-                                                 const uint32_t *__restrict__ offsets, int tiles,
-                                                 const uint32_t *__restrict__ flags) {
-  __shared__ uint32_t run[kDigits];
-  const int lane = threadIdx.x;
+__global__ void __launch_bounds__(kSortBlock) rs_scatter(const uint64_t *__restrict__ kin,
+                                                         const uint32_t *__restrict__ vin,
+                                                         uint64_t *__restrict__ kout,
+                                                         uint32_t *__restrict__ vout, int64_t n, int shift,
+                                                         const uint32_t *__restrict__ offsets, int tiles,
+                                                         const uint32_t *__restrict__ flags) {
+  __shared__ uint64_t skey[kTile];
+  __shared__ uint32_t sval[kTile];
+  __shared__ uint32_t wrun[kWaves][kDigits];  // per wave: running count, then the wave's base
+  __shared__ uint32_t lbase[kDigits];         // tile-local start of each digit
+  __shared__ uint32_t gbase[kDigits];         // global start of the tile's keys of each digit
+  __shared__ uint32_t wtot[kWaves];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int64_t base = (int64_t)blockIdx.x * kTile;
   if (flags[0]) {  // identity pass
-    for (int i = lane; i < kTile; i += 64) {
-      int64_t g = base + i;
-      if (g < n) { kout[g] = kin[g]; vout[g] = vin[g]; }
+    for (int i = t; i < kTile; i += kSortBlock) {
+      const int64_t g = base + i;
+      if (g < n) {
+        kout[g] = kin[g];
+        vout[g] = vin[g];
+      }
     }
     return;
   }
-  // exclusive offset of (digit, tile) = inclusive scan at the previous entry
-  for (int d = lane; d < kDigits; d += 64) {
-    const size_t e = (size_t)d * tiles + blockIdx.x;
-    run[d] = e ? offsets[e - 1] : 0u;
-  }
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) wrun[w][t] = 0;
   __syncthreads();
-  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (int c = 0; c < kTile; c += 64) {
-    const int64_t g = base + c + lane;
-    const bool valid = g < n;
-    uint64_t key = valid ? kin[g] : 0;
-    uint32_t val = valid ? vin[g] : 0;
-    uint32_t dig = valid ? (uint32_t)((key >> shift) & 0xff) : 0;
-    uint64_t match = __ballot(valid);
-    for (int bit = 0; bit < 8; ++bit) {
-      uint64_t bal = __ballot(valid && ((dig >> bit) & 1u));
-      match &= ((dig >> bit) & 1u) ? bal : ~bal;
-    }
+  uint64_t key[kChunks];
+  uint32_t val[kChunks], pos[kChunks];
+  const int64_t wbase = base + (int64_t)wave * (kChunks * 64) + lane;
+#pragma unroll
+  for (int c = 0; c < kChunks; ++c) {
+    const int64_t g = wbase + c * 64;
+    key[c] = g < n ? kin[g] : 0ull;
+    val[c] = g < n ? vin[g] : 0u;
+  }
+  const uint64_t below = lanes_below();
+  // rank inside the wave's keys of the same digit, in input order; the
+  // running counts are per wave, and one wave's LDS operations take effect
+  // in program order, so each chunk reads the counts the previous one left
+#pragma unroll
+  for (int c = 0; c < kChunks; ++c) {
+    const bool valid = wbase + c * 64 < n;
+    const uint32_t dig = (uint32_t)((key[c] >> shift) & 0xff);
+    const uint64_t match = digit_match(valid, dig);
     const uint32_t rank = (uint32_t)__popcll(match & below);
     const uint32_t cnt = (uint32_t)__popcll(match);
-    uint32_t pos = 0;
-    if (valid) pos = run[dig] + rank;
-    __syncthreads();
-    if (valid) {
-      kout[pos] = key;
-      vout[pos] = val;
-      // the last lane of each digit group advances the running count
-      if (rank + 1 == cnt) run[dig] += cnt;
-    }
-    __syncthreads();
+    const uint32_t r0 = valid ? wrun[wave][dig] : 0u;
+    pos[c] = r0 + rank;
+    if (valid && rank + 1 == cnt) wrun[wave][dig] = r0 + cnt;
+  }
+  __syncthreads();
+  // digit t: wave bases, tile-local start (exclusive scan over the digits)
+  // and global start
+  uint32_t tot = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    const uint32_t c = wrun[w][t];
+    wrun[w][t] = tot;
+    tot += c;
+  }
+  uint32_t x = tot;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, (unsigned)d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) wtot[wave] = x;
+  const size_t e = (size_t)t * tiles + blockIdx.x;
+  gbase[t] = e ? offsets[e - 1] : 0u;
+  __syncthreads();
+  uint32_t before = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) before += w < wave ? wtot[w] : 0u;
+  lbase[t] = before + x - tot;
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < kChunks; ++c) {
+    if (wbase + c * 64 >= n) continue;
+    const uint32_t dig = (uint32_t)((key[c] >> shift) & 0xff);
+    const uint32_t lp = lbase[dig] + wrun[wave][dig] + pos[c];
+    skey[lp] = key[c];
+    sval[lp] = val[c];
+  }
+  __syncthreads();
+  const int valid_n = (int)min((int64_t)kTile, n - base);
+  for (int i = t; i < valid_n; i += kSortBlock) {
+    const uint64_t k = skey[i];
+    const uint32_t dig = (uint32_t)((k >> shift) & 0xff);
+    const uint32_t gp = gbase[dig] + (uint32_t)i - lbase[dig];
+    kout[gp] = k;
+    vout[gp] = sval[i];
   }
 }
 
@@ -110,12 +185,11 @@ void RadixSorter::sort(uint64_t *keys, uint32_t *vals, int64_t n, int begin_bit,
   uint32_t *va = vals, *vb = tmp_vals.p;
   int passes = 0;
   for (int shift = begin_bit; shift < end_bit; shift += 8) {
-    hipLaunchKernelGGL(rs_histogram, dim3(tiles), dim3(256), 0, stream, ka, n, shift, counts.p,
-                       tiles);
+    hipLaunchKernelGGL(rs_histogram, dim3(tiles), dim3(kSortBlock), 0, stream, ka, n, shift, counts.p, tiles);
     scanner.inclusive(counts.p, (int64_t)kDigits * tiles, stream);
     hipLaunchKernelGGL(rs_identity, dim3(1), dim3(kDigits), 0, stream, counts.p, n, tiles, flags.p);
-    hipLaunchKernelGGL(rs_scatter, dim3(tiles), dim3(64), 0, stream, ka, va, kb, vb, n, shift,
-                       counts.p, tiles, flags.p);
+    hipLaunchKernelGGL(rs_scatter, dim3(tiles), dim3(kSortBlock), 0, stream, ka, va, kb, vb, n, shift, counts.p,
+                       tiles, flags.p);
     MHGP_LAUNCH_CHECK();
     std::swap(ka, kb);
     std::swap(va, vb);
