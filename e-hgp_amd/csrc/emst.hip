@@ -272,6 +272,19 @@ static uint64_t exact_best_edge(Context &c, const std::vector<uint32_t> &label, 
   return ((uint64_t)std::min<int64_t>(u, best) << 32) | (uint64_t)std::max<int64_t>(u, best);
 }
 
+// Sort keys of the forest's edges: the bits of the FP64 squared length (a
+// non-negative double orders like its bit pattern) with the edge index.
+__global__ void k_kappa_keys(int64_t m, const uint32_t *__restrict__ edges, const double *__restrict__ X,
+                             const double *__restrict__ Y, const double *__restrict__ Z, uint64_t *key,
+                             uint32_t *idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t a = edges[2 * i], b = edges[2 * i + 1];
+  const double dx = X[a] - X[b], dy = Y[a] - Y[b], dz = Z[a] - Z[b];
+  key[i] = (uint64_t)__double_as_longlong(dx * dx + dy * dy + dz * dz);
+  idx[i] = (uint32_t)i;
+}
+
 void emst_host(Context &c, uint32_t *edges_out) {
   const int64_t n = c.n;
   if (n <= 1) return;
@@ -363,23 +376,31 @@ void emst_host(Context &c, uint32_t *edges_out) {
     if (cnt[0] + components != (uint64_t)n) throw std::runtime_error("Borůvka edge count mismatch");
   }
   StageTimer tm(s);
-  // kappa order: FP64 squared length, exact comparison inside rounding ties
-  std::vector<uint32_t> e(2 * (n - 1));
-  copy_wait(e.data(), B.edges.p, e.size() * 4, hipMemcpyDeviceToHost, s);
-  std::vector<double> xyz(3 * n);
-  if (c.host) xyz = c.host->xyz; else canonical_points(c, xyz.data(), nullptr);
-  std::vector<std::pair<double, uint64_t>> ord(n - 1);
-  for (int64_t i = 0; i < n - 1; ++i) {
-    uint32_t a = std::min(e[2 * i], e[2 * i + 1]), b = std::max(e[2 * i], e[2 * i + 1]);
-    double s2 = 0;
-    for (int k = 0; k < 3; ++k) { double t = xyz[3 * a + k] - xyz[3 * b + k]; s2 += t * t; }
-    ord[i] = {s2, ((uint64_t)a << 32) | b};
+  // kappa order: FP64 squared lengths sorted on the device; runs of values
+  // within rounding of their neighbours are ordered exactly on the host
+  const int64_t m = n - 1;
+  B.kappa_key.ensure(m);
+  B.kappa_idx.ensure(m);
+  hipLaunchKernelGGL(k_kappa_keys, dim3(blocks_for(m, T)), dim3(T), 0, s, m, B.edges.p, c.X.p, c.Y.p, c.Z.p,
+                     B.kappa_key.p, B.kappa_idx.p);
+  MHGP_LAUNCH_CHECK();
+  c.sorter.sort(B.kappa_key.p, B.kappa_idx.p, m, 0, 64, s);
+  std::vector<uint32_t> e(2 * m), idx(m);
+  std::vector<double> len(m);
+  MHGP_HIP(hipMemcpyAsync(e.data(), B.edges.p, e.size() * 4, hipMemcpyDeviceToHost, s));
+  MHGP_HIP(hipMemcpyAsync(idx.data(), B.kappa_idx.p, m * 4, hipMemcpyDeviceToHost, s));
+  copy_wait(len.data(), B.kappa_key.p, m * 8, hipMemcpyDeviceToHost, s);
+  std::vector<std::pair<double, uint64_t>> ord(m);
+  for (int64_t i = 0; i < m; ++i) {
+    const uint32_t x = e[2 * (size_t)idx[i]], y = e[2 * (size_t)idx[i] + 1];
+    ord[i] = {len[i], ((uint64_t)std::min(x, y) << 32) | std::max(x, y)};
   }
-  std::sort(ord.begin(), ord.end());
-  // settle neighbours whose FP64 values are within rounding
-  for (int64_t i = 0; i + 1 < n - 1; ++i) {
+  // a run continues while each value is within 1e-12 (relative) of the one
+  // before it, so values in different runs differ by far more than the
+  // rounding of the FP64 lengths and their FP64 order is the exact one
+  for (int64_t i = 0; i + 1 < m; ++i) {
     int64_t j = i;
-    while (j + 1 < n - 1 && ord[j + 1].first - ord[i].first <= 1e-12 * ord[j + 1].first) j++;
+    while (j + 1 < m && ord[j + 1].first - ord[j].first <= 1e-12 * ord[j + 1].first) j++;
     if (j > i) {
       const ExactCloud &ex = exact_cloud(c);
       std::sort(ord.begin() + i, ord.begin() + j + 1, [&](const auto &x, const auto &y) {
@@ -391,7 +412,7 @@ void emst_host(Context &c, uint32_t *edges_out) {
     }
   }
   tm.mark("emst kappa sort");
-  for (int64_t i = 0; i < n - 1; ++i) {
+  for (int64_t i = 0; i < m; ++i) {
     edges_out[2 * i] = (uint32_t)(ord[i].second >> 32);
     edges_out[2 * i + 1] = (uint32_t)ord[i].second;
   }

@@ -26,6 +26,8 @@ struct EmstBuffers {
   DevBuf<uint32_t> comp_cw, comp_ce; // entries in the rounding window / exact equal
   DevBuf<uint32_t> edges;          // accepted edges (u, v)
   DevBuf<uint32_t> counters;
+  DevBuf<uint64_t> kappa_key;      // FP64 squared length bits of each edge
+  DevBuf<uint32_t> kappa_idx;      // edge index, sorted with the keys
 };
 
 struct CatalogBuffers;

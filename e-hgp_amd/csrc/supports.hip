@@ -994,13 +994,37 @@ __global__ void __launch_bounds__(128) k_knn_radius(int64_t n, int kq, float fac
   float best[kRadiusK];
 #pragma unroll
   for (int j = 0; j < kRadiusK; ++j) best[j] = INFINITY;
-  float cut = INFINITY;  // the kq-th smallest so far
+  // starting bound: the farthest of kq Morton-order neighbours, a set of kq
+  // other points, so the kq-th nearest distance cannot exceed it (inflated
+  // past the rounding of the outward FP32 leaf boxes, so the traversal still
+  // meets those kq points below the bound)
+  float bound = INFINITY;
+  if (n > kq) {
+    int64_t lo = p - kq / 2;
+    if (lo < 0) lo = 0;
+    if (lo + kq >= n) lo = n - 1 - kq;
+    float m = 0.f;
+    for (int64_t j = lo; j <= lo + kq; ++j) {
+      if (j == p) continue;
+      const float dx = (float)LX[j] - q[0], dy = (float)LY[j] - q[1], dz = (float)LZ[j] - q[2];
+      m = fmaxf(m, dx * dx + dy * dy + dz * dz);
+    }
+    bound = m * (1.0f + 1e-4f) + 1e-30f;
+  }
+  float cut = bound;  // min(bound, the kq-th smallest so far)
   int sp = 0;
   if (n > 1) stk[(sp++) * 128 + lane] = 0;
   while (sp) {
     const WideNode &N = wide[stk[(--sp) * 128 + lane]];
+    // leaves are taken at once; inner children are pushed far to near, so
+    // the nearest is expanded first and the bound shrinks early
+    float cd[4];
+    uint32_t cx[4];
+#pragma unroll
     for (int c = 0; c < 4; ++c) {
       const uint32_t x = N.child[c];
+      cd[c] = -1.f;
+      cx[c] = x;
       if (x == kInvalid) continue;
       float d2 = 0.f;
       for (int a = 0; a < 3; ++a) {
@@ -1017,10 +1041,29 @@ __global__ void __launch_bounds__(128) k_knn_radius(int64_t n, int kq, float fac
         float c2 = INFINITY;
 #pragma unroll
         for (int j = 0; j < kRadiusK; ++j) c2 = j == kq - 1 ? best[j] : c2;
-        cut = c2;
-      } else if (sp < kRadiusStack) {
-        stk[(sp++) * 128 + lane] = x;
+        cut = fminf(bound, c2);
+      } else {
+        cd[c] = d2;
       }
+    }
+    // descending distance (a five-comparator network), then push
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      const int i = e == 0 ? 0 : e == 1 ? 2 : e == 2 ? 0 : e == 3 ? 1 : 1;
+      const int j = e == 0 ? 1 : e == 1 ? 3 : e == 2 ? 2 : e == 3 ? 3 : 2;
+      if (cd[i] < cd[j]) {
+        const float td = cd[i];
+        cd[i] = cd[j];
+        cd[j] = td;
+        const uint32_t tx = cx[i];
+        cx[i] = cx[j];
+        cx[j] = tx;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (!(cd[c] >= 0.f) || !(cd[c] < cut)) continue;
+      if (sp < kRadiusStack) stk[(sp++) * 128 + lane] = cx[c];
     }
   }
   // a subtree dropped by a full stack can only raise the kq-th distance found
