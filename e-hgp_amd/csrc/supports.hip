@@ -294,10 +294,26 @@ struct BehindList {
 };
 
 // Sweep key of a point against the face (tau = P / |H|) and its error bound.
+// The reciprocal of |H| is the hardware estimate refined by two Newton
+// steps; the residual e = 1 - |H| r of the result bounds its relative error
+// (up to the rounding of e), which joins the key's error bound.  Outside the
+// range where the estimate is reliable the key is unknown: tau = 0 with an
+// infinite bound, which meets every threshold and sends the side to its full
+// scan.
 __device__ inline void sweep_key(double Pw, double PA, double H, double HA, double &tau, double &terr) {
-  const double rH = 1.0 / fabs(H);
-  tau = Pw * rH;
-  terr = (kCert * PA + kCert * HA * fabs(tau)) * rH * (1.0 + 1e-12) + 1e-15 * fabs(tau);
+  const double h = fabs(H);
+  double r = __builtin_amdgcn_rcp(h);
+  double e = __builtin_fma(-h, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-h, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  const double rerr = 2.0 * fabs(__builtin_fma(-h, r, 1.0));
+  tau = Pw * r;
+  terr = (kCert * PA + kCert * HA * fabs(tau)) * r * (1.0 + 1e-12) + (1e-15 + rerr) * fabs(tau);
+  if (!(h > 1e-290 && h < 1e290)) {
+    tau = 0.0;
+    terr = INFINITY;
+  }
 }
 
 // Apexes of one side of a face: the outside points ahead of the side whose
