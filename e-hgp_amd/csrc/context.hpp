@@ -73,8 +73,8 @@ struct Context {
   std::unique_ptr<EmstBuffers> emst;
   std::unique_ptr<HostMirror> host;
   std::unique_ptr<TowerState> tower;
-  std::unique_ptr<TowerWork> tower_work;   // first lane of the tower (context stream)
-  std::unique_ptr<TowerWork> tower_work2;  // second lane (own stream)
+  std::unique_ptr<TowerWork> tower_work;                 // first lane of the tower (context stream)
+  std::vector<std::unique_ptr<TowerWork>> tower_lanes;  // further lanes (own streams)
   PinnedBuf tower_pinned[11];  // per tower order: saddle results on the host
 
   Context();

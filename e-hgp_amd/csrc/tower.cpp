@@ -15,6 +15,7 @@
 //    with no prior reduced root creates a reduced birth (order k >= 2), one
 //    root grows silently, two or more roots create a multifusion.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -35,6 +36,9 @@
 namespace mhgp {
 
 namespace {
+
+// Device lanes of the tower (streams with their own workspaces).
+constexpr int kTowerLanes = 2;
 
 struct LabelHash {
   size_t operator()(const std::vector<uint32_t> &v) const {
@@ -394,20 +398,23 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
   tower_rank_index(c);
   TowerWork &W0 = *c.tower_work;
   W0.stream = c.stream;
-  if (K >= 2 && !c.tower_work2) {
+  const int lanes = std::min(kTowerLanes, K);
+  while ((int)c.tower_lanes.size() < lanes - 1) {
     auto w = std::make_unique<TowerWork>();
     MHGP_HIP(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
     w->own_stream = true;
-    c.tower_work2 = std::move(w);
+    c.tower_lanes.push_back(std::move(w));
   }
   std::mutex mu;  // workers, the host descender
-  // Two device lanes take the orders in turn (K, K-2, ... and K-1, K-3, ...),
-  // so one lane's host synchronisations overlap the other's kernels; each
-  // order's contraction then runs on a host thread of its own.  Highest
-  // orders first: their contractions are the longest.
-  auto lane = [&](TowerWork &W, int first_k) {
+  // The device lanes take the orders from a shared counter, highest first
+  // (their contractions are the longest), so one lane's host
+  // synchronisations overlap the other's kernels and a lane that finishes
+  // early takes the next order; each order's contraction then runs on a
+  // host thread of its own.
+  std::atomic<int> next_k{K};
+  auto lane = [&](TowerWork &W) {
     MHGP_HIP(hipSetDevice(c.device));
-    for (int k = first_k; k >= 1; k -= 2) {
+    for (int k = next_k.fetch_sub(1); k >= 1; k = next_k.fetch_sub(1)) {
       TowerOrderData &D = Ds[k - 1];
       tower_order_device(c, W, W0, k, D);
       {
@@ -451,30 +458,31 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
       });
     }
   };
-  std::exception_ptr lane_error[2];
-  std::thread second;
-  if (K >= 2)
-    second = std::thread([&] {
+  std::vector<std::exception_ptr> lane_error(lanes);
+  std::vector<std::thread> others;
+  for (int l = 1; l < lanes; ++l)
+    others.emplace_back([&, l] {
       try {
-        lane(*c.tower_work2, K - 1);
+        lane(*c.tower_lanes[l - 1]);
       } catch (...) {
-        lane_error[1] = std::current_exception();
+        lane_error[l] = std::current_exception();
       }
     });
   try {
-    lane(W0, K);
+    lane(W0);
   } catch (...) {
     lane_error[0] = std::current_exception();
   }
-  if (second.joinable()) second.join();
+  for (auto &t : others) t.join();
   for (auto &e : lane_error)
     if (e) {
       for (auto &t : workers)
         if (t.joinable()) t.join();
       std::rethrow_exception(e);
     }
-  for (TowerWork *W : {c.tower_work.get(), c.tower_work2.get()}) {
-    if (!W) continue;
+  std::vector<TowerWork *> all{c.tower_work.get()};
+  for (auto &w : c.tower_lanes) all.push_back(w.get());
+  for (TowerWork *W : all) {
     c.stats.arm_ms += W->arm_ms;
     c.stats.arm_launches += W->arm_launches;
     c.stats.arm_bytes += W->arm_bytes;

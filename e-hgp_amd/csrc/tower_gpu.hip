@@ -11,6 +11,7 @@
 //    least point strictly inside (the miniball strictly shrinks);
 //  * anything the certified filters cannot decide is flagged for the host's
 //    exact descent.
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 
@@ -841,6 +842,10 @@ void tower_order_device(Context &c, TowerWork &W, const TowerWork &R, int k, Tow
   CatalogBuffers &B = *c.catalog;
   hipStream_t s = W.stream;
   out = TowerOrderData{};
+  // host clock at the synchronisations that end each phase (MHGP_TIMING)
+  using clk = std::chrono::steady_clock;
+  clk::time_point ph[8];
+  for (auto &x : ph) x = clk::now();
   // minima of order k (k >= 2) and saddles of order k: records of closed
   // rank k and k + 1, as runs of the rank index
   const uint32_t M = R.rank_hi[k] - R.rank_lo[k], S = R.rank_hi[k + 1] - R.rank_lo[k + 1];
@@ -868,6 +873,7 @@ void tower_order_device(Context &c, TowerWork &W, const TowerWork &R, int k, Tow
   W.scanner.inclusive(sizes.p, S, s);
   uint32_t arms = 0;
   copy_wait(&arms, sizes.p + S - 1, 4, hipMemcpyDeviceToHost, s);
+  ph[1] = ph[2] = ph[3] = clk::now();
   // exclusive bases: shift the inclusive scan by one
   auto &base = W.base;
   base.ensure(S);
@@ -914,6 +920,7 @@ void tower_order_device(Context &c, TowerWork &W, const TowerWork &R, int k, Tow
   MHGP_HIP(hipMemcpyAsync(&nq, qcount.p, 8, hipMemcpyDeviceToHost, s));
   MHGP_HIP(hipMemcpyAsync(&na, W.alias_count.p, 8, hipMemcpyDeviceToHost, s));
   MHGP_HIP(hipStreamSynchronize(s));
+  ph[2] = ph[3] = clk::now();
   if (nq > 1) {  // spatial order: neighbouring lanes walk neighbouring nodes
     int bits = 8;
     while (bits < 32 && (1ll << bits) < c.n) bits += 8;
@@ -940,6 +947,7 @@ void tower_order_device(Context &c, TowerWork &W, const TowerWork &R, int k, Tow
       steps_total += nxt;
       active = (uint32_t)nxt;
     }
+    ph[3] = clk::now();
   }
   if (na)
     hipLaunchKernelGGL(k_alias_carriers, dim3(blocks_for(na, 256)), dim3(256), 0, s, W.alias_arm.p, W.alias_rep.p,
@@ -969,6 +977,7 @@ void tower_order_device(Context &c, TowerWork &W, const TowerWork &R, int k, Tow
                      (int64_t)arms, minima_n, first.p, flag1.p);
   uint32_t any_invalid = 0;
   copy_wait(&any_invalid, flag1.p, 4, hipMemcpyDeviceToHost, s);
+  ph[4] = ph[5] = ph[6] = clk::now();
   uint32_t S2 = S, arms2 = arms;
   const uint32_t *d_cls = scls.p, *d_base = base.p, *d_car = carrier.p;
   const double *d_lev = slev.p;
@@ -999,6 +1008,7 @@ void tower_order_device(Context &c, TowerWork &W, const TowerWork &R, int k, Tow
     MHGP_HIP(hipMemcpyAsync(&bad, err.p, 4, hipMemcpyDeviceToHost, s));
     copy_wait(&E, eflag.p + arms - 1, 4, hipMemcpyDeviceToHost, s);
     if (bad) throw std::runtime_error("tower attachment forest is cyclic");
+    ph[5] = ph[6] = clk::now();
     eu.ensure(E);
     ev.ensure(E);
     es.ensure(E);
@@ -1054,6 +1064,7 @@ void tower_order_device(Context &c, TowerWork &W, const TowerWork &R, int k, Tow
       lv = Lv[q].p;
       li = Li[q].p;
     }
+    ph[6] = clk::now();
     if (E) hipLaunchKernelGGL(k_msf_count, ge, tb, 0, s, (int64_t)E, msf.p, es.p, mcount.p);
     // keep births, multifusions and shared levels
     hipLaunchKernelGGL(k_reduce_saddles, gs, tb, 0, s, (int64_t)S, base.p, sizes.p, carrier.p, scls.p, first.p,
@@ -1100,6 +1111,7 @@ void tower_order_device(Context &c, TowerWork &W, const TowerWork &R, int k, Tow
   }
   const uint64_t st = steps_total;
   MHGP_HIP(hipStreamSynchronize(s));
+  ph[7] = clk::now();
   h_base[S2] = arms2;
   out.saddles = S2;
   out.cls = h_cls;
@@ -1111,8 +1123,16 @@ void tower_order_device(Context &c, TowerWork &W, const TowerWork &R, int k, Tow
   if (std::getenv("MHGP_TIMING"))
     std::fprintf(stderr,
                  "[mhgp] arms order %2d: %u saddles, %u arms, %llu descending, %llu steps, %u forest edges in %d "
-                 "rounds, %u kept\n",
-                 k, S, arms, (unsigned long long)nq, (unsigned long long)st, msf_edges, msf_rounds, S2);
+                 "rounds, %u kept | ms: setup %.2f lookup %.2f descents %.2f first %.2f edges %.2f forest %.2f "
+                 "tail %.2f\n",
+                 k, S, arms, (unsigned long long)nq, (unsigned long long)st, msf_edges, msf_rounds, S2,
+                 std::chrono::duration<double, std::milli>(ph[1] - ph[0]).count(),
+                 std::chrono::duration<double, std::milli>(ph[2] - ph[1]).count(),
+                 std::chrono::duration<double, std::milli>(ph[3] - ph[2]).count(),
+                 std::chrono::duration<double, std::milli>(ph[4] - ph[3]).count(),
+                 std::chrono::duration<double, std::milli>(ph[5] - ph[4]).count(),
+                 std::chrono::duration<double, std::milli>(ph[6] - ph[5]).count(),
+                 std::chrono::duration<double, std::milli>(ph[7] - ph[6]).count());
   const bool need_labels = any_invalid != 0;  // then the saddles were not filtered
   if (need_labels) {
     // the host descends unresolved arms; it needs the saddle labels, their
