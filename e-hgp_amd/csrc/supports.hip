@@ -570,13 +570,16 @@ __device__ inline CellF cell_f(const double *cd) {
 // point i: cos of the smallest angle between its direction and the cell.
 // FP32 with relative and absolute slack; a point may only be included too
 // often, which costs work but never a support.
-__device__ bool member_mask(const Params &P, const Work &w, uint32_t i, uint64_t mask[6]) {
+// rmax2 bounds 2 rho'_C over every cell from above: a point beyond it is in
+// no cell's balls.
+__device__ bool member_mask(const Params &P, const Work &w, uint32_t i, float rmax2, uint64_t mask[6]) {
   const float4 q = w.fc[i];
   for (int k = 0; k < 6; ++k) mask[k] = 0;
   if (!(q.w > kTinyLen)) {  // distance lost to underflow: member everywhere
     for (int k = 0; k < 6; ++k) mask[k] = ~0ull;
     return true;
   }
+  if (q.w > rmax2) return false;
   const float inv = __builtin_amdgcn_rcpf(q.w);
   bool any = false;
   // one mask word per 64 cells, each built in a register
@@ -665,6 +668,11 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   __syncthreads();
   // 1. direction-cell certificate (FP32; every rounding errs towards larger radii)
   const int need_cover = P.s_max - 1;
+  // the points come in increasing distance when the rank sort ran: a point at
+  // distance l covers at radius >= l / 2 (up to the FP32 rounding, inside the
+  // factor below), so once l / 2 exceeds the list's last entry no later point
+  // can enter it (and dropping points only enlarges the radius anyway)
+  const bool by_distance = G <= 4 * blockDim.x;
   for (int c = t; c < kCells; c += blockDim.x) {
     const double *cd = P.cells + 8 * c;
     const CellF cf = cell_f(cd);
@@ -675,6 +683,7 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     for (uint32_t i = 0; i < G; ++i) {
       const float4 q = w.fc[i];
       if (!(q.w > kTinyLen)) continue;  // skipping a point only enlarges the radius
+      if (by_distance && q.w * 0.49999f > best[kCoverCap - 1]) break;
       const float ct = (q.x * cf.dx + q.y * cf.dy + q.z * cf.dz) * __builtin_amdgcn_rcpf(q.w);
       if (ct <= 0.0f) continue;
       const float st = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
@@ -702,9 +711,12 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     w.rho[c] = fmin(rc, e);
   }
   __syncthreads();
+  __shared__ float s_rmax2;
   if (t == 0) {
     double m = 0;
     for (int c = 0; c < kCells; ++c) m = fmax(m, w.rho[c]);
+    // 2 max rho in FP32, rounded up past the slack of the membership test
+    s_rmax2 = (float)(2.0 * m) * 1.0001f;
     if (2.0 * m > Rg) {
       P.need[u] = 2.0 * m;
       P.status[u] = ST_WIDEN;
@@ -713,6 +725,7 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   }
   __syncthreads();
   if (sflag) return;
+  const float rmax2 = s_rmax2;
   // 2. region membership masks, compacted in gathered order (chunks of four
   // points per lane; the LDS path has a single chunk, so every FP32 offset is
   // read before the first mask overwrites the area they share)
@@ -728,7 +741,7 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     for (uint32_t r = 0; r < 4; ++r) {
       mem[r] = false;
       if (r < chunk && t * chunk + r < g) {
-        mem[r] = member_mask(P, w, b0 + r, mk[r]);
+        mem[r] = member_mask(P, w, b0 + r, rmax2, mk[r]);
         cnt += mem[r];
       }
     }
