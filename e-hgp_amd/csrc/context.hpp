@@ -19,6 +19,9 @@ struct TowerWork;        // tower_gpu.hpp
 
 struct Context {
   int device = 0;
+  // device lanes of the tower (streams with their own workspaces); a server
+  // slot uses one, since the server already overlaps whole requests
+  int tower_lane_count = 2;
   hipStream_t stream = nullptr;
   bool poisoned = false;
   std::string last_error;

@@ -65,6 +65,7 @@ struct Server {
 void Server::worker_main() {
   Context c;
   c.device = device;
+  c.tower_lane_count = 1;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
     std::lock_guard<std::mutex> g(mu);
     error = "worker could not create its stream";

@@ -37,9 +37,6 @@ namespace mhgp {
 
 namespace {
 
-// Device lanes of the tower (streams with their own workspaces).
-constexpr int kTowerLanes = 2;
-
 struct LabelHash {
   size_t operator()(const std::vector<uint32_t> &v) const {
     uint64_t h = 1469598103934665603ull;
@@ -398,7 +395,7 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
   tower_rank_index(c);
   TowerWork &W0 = *c.tower_work;
   W0.stream = c.stream;
-  const int lanes = std::min(kTowerLanes, K);
+  const int lanes = std::max(1, std::min(c.tower_lane_count, K));
   while ((int)c.tower_lanes.size() < lanes - 1) {
     auto w = std::make_unique<TowerWork>();
     MHGP_HIP(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
