@@ -20,6 +20,7 @@ namespace {
 struct ExactJob {
   const ExactCloud &ex;
   int s_max;
+  int scale_exp;
   std::vector<EventRec> &out;
   int64_t predicates = 0;
 
@@ -52,7 +53,8 @@ struct ExactJob {
     e.meta = (uint32_t)m | ((uint32_t)(nint + m) << 8) | ((uint32_t)nint << 16) |
              ((extra ? 1u : 0u) << 24) | ((uint32_t)std::min(extra, 127) << 25);
     for (int k = 0; k < kMaxInterior; ++k) e.interior[k] = k < nint ? interior[k] : kInvalid;
-    e.level = S.lnum.to_double() / S.lden.to_double() * std::ldexp(1.0, 2 * ex.e0());
+    // in the device's frame: coordinates there are scaled by 2^scale_exp
+    e.level = ratio_to_double(S.lnum, S.lden, 2 * (ex.e0() + scale_exp));
     e.level_err = 1e-12f;
     out.push_back(e);
   }
@@ -75,7 +77,7 @@ void supports(Context &c, int k_max, mhgp_catalog_info *info) {
   if (!fbs.empty()) {
     std::vector<EventRec> settled;
     const ExactCloud &ex = exact_cloud(c);
-    ExactJob job{ex, smax, settled};
+    ExactJob job{ex, smax, c.scale_exp, settled};
     for (const FallbackRec &f : fbs) {
       if (f.region_off == kInvalid) throw std::runtime_error("fallback without a published region");
       const uint32_t *reg = region.data() + f.region_off;
@@ -143,7 +145,7 @@ void supports_fetch(Context &c, uint32_t *ids, uint32_t *meta, uint32_t *interio
     if (meta) meta[i] = e.meta;
     if (interior)
       for (int k = 0; k < width; ++k) interior[width * i + k] = k < kMaxInterior ? e.interior[k] : kInvalid;
-    if (level) level[i] = e.level;
+    if (level) level[i] = std::ldexp(e.level, -2 * c.scale_exp);
     if (level_class) level_class[i] = cls[i];
   }
 }

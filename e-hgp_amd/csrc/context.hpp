@@ -28,6 +28,13 @@ struct Context {
 
   int64_t n = 0;
   bool lbvh_ready = false;
+  // Device coordinates (X..Z, LX..LZ, node boxes, bounds) are the canonical
+  // ones times 2^scale_exp, an exact power-of-two change of unit chosen at
+  // upload so that the cloud's extent is near 1 for clouds far from that
+  // scale (their FP32 bounds and FP64 predicates would otherwise leave the
+  // normal range).  Signs, orders and ranks do not change; squared levels
+  // are divided by 4^scale_exp on their way out.
+  int scale_exp = 0;
 
   // input (source order, AoS) and canonical cloud (PointId order, SoA)
   DevBuf<double> src;

@@ -98,13 +98,13 @@ __global__ void __launch_bounds__(128) k_search(int64_t n, const Node *__restric
         if (label[v] == lu) continue;
         bool ex;
         double d = d2_flag(q, LX[lp], LY[lp], LZ[lp], ex);
-        if (d > bd * (1.0 + kSlack)) continue;
+        if (d > bd * (1.0 + kSlack) + kSlackAbs) continue;
         bool both = ex && bx;
         bool better;
         if (d == bd && both) {
           better = edge_key(u, v) < edge_key(u, bv);
         } else {
-          if (!both && bd < INFINITY && fabs(d - bd) <= kSlack * bd) flag = 1;  // order not provable
+          if (!both && bd < INFINITY && fabs(d - bd) <= kSlack * bd + kSlackAbs) flag = 1;  // order not provable
           better = d < bd || (d == bd && edge_key(u, v) < edge_key(u, bv));
         }
         if (better) { bd = d; bv = v; bx = ex; }
@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(128) k_search(int64_t n, const Node *__restric
     int first = cd[0] <= cd[1] ? 0 : 1;
     for (int o = 1; o >= 0; --o) {
       int c = o ? 1 - first : first;
-      if (cd[c] == INFINITY || cd[c] > bd * (1.0 + kSlack)) continue;
+      if (cd[c] == INFINITY || cd[c] > bd * (1.0 + kSlack) + kSlackAbs) continue;
       if (sp < 64) stack[sp++] = ch[c]; else flag = 2;
     }
   }
@@ -157,7 +157,7 @@ __global__ void k_comp_tie(int64_t n, const uint32_t *__restrict__ label,
   memcpy(&m, &mb, 8);
   const double d = best_d[u];
   if (best_flag[u] & 2) atomicOr(&comp_amb[l], 16u);  // truncated search: best is only a bound
-  if (d - m > kSlack * m) return;
+  if (d - m > kSlack * m + kSlackAbs) return;
   atomicAdd(&comp_cw[l], 1u);
   if (best_flag[u] & 3) atomicOr(&comp_amb[l], 1u);
   if (dbits(d) == mb) {

@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(128) k_knn(int64_t n, int k, const Node *__res
       int c = o ? 1 - first : first;
       uint32_t x = ch[c];
       if (x & kLeafBit) continue;
-      if (cd[c] > cut * (1.0 + kSlack)) continue;
+      if (cd[c] > cut * (1.0 + kSlack) + kSlackAbs) continue;
       if (sp < 64) stack[sp++] = x;
       else flags[self] |= 4;  // stack overflow: host recomputes
     }
@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(128) k_knn(int64_t n, int k, const Node *__res
   uint8_t f = 0;
   for (int i = 0; i + 1 < cnt; ++i) {
     double a = bd[i], b = bd[i + 1];
-    if (!(bx[i] && bx[i + 1]) && (b - a) <= kSlack * b) f |= 1;
+    if (!(bx[i] && bx[i + 1]) && (b - a) <= kSlack * b + kSlackAbs) f |= 1;
   }
   const int64_t o = (int64_t)self * k;
   for (int i = 0; i < k; ++i) out[o + i] = i < cnt ? bi[i] : kInvalid;

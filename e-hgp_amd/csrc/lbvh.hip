@@ -9,6 +9,9 @@
 // (lbvh.cpp:831-906), built in one pass with Karras' radix-tree construction
 // over the augmented key (code, midpoint path bits).  Boxes carry PointId
 // witnesses with the reference's tie rule (lbvh.cpp:46-76).
+#include <algorithm>
+#include <cmath>
+
 #include "context.hpp"
 #include "exact_device.hpp"
 
@@ -368,10 +371,71 @@ __global__ void k_wide_nodes(int64_t m, const Node *__restrict__ nodes, const do
   out[i] = w;
 }
 
+// Power-of-two change of unit (context.hpp, scale_exp): possible only when
+// every scaled coordinate stays a normal (or zero) finite binary64.
+__global__ void k_scale_check(const double *__restrict__ X, const double *__restrict__ Y,
+                              const double *__restrict__ Z, int64_t n, int e, uint32_t *bad) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double v[3] = {X[i], Y[i], Z[i]};
+  bool ok = true;
+  for (int a = 0; a < 3; ++a) {
+    const double t = fabs(ldexp(v[a], e));
+    ok = ok && (v[a] == 0.0 || (t >= 0x1p-1022 && t < INFINITY));
+  }
+  if (!ok) atomicOr(bad, 1u);
+}
+
+__global__ void k_scale_values(double *v, int64_t n, int e) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = ldexp(v[i], e);
+}
+
+__global__ void k_scale_nodes(Node *nodes, int64_t m, int e) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  for (int a = 0; a < 3; ++a) {
+    nodes[i].lo[a] = ldexp(nodes[i].lo[a], e);
+    nodes[i].hi[a] = ldexp(nodes[i].hi[a], e);
+  }
+}
+
+// Unit exponent for a cloud of extent E: none inside [2^-16, 2^16], else the
+// one that brings E into [1, 2).
+int scale_exponent_for(double extent) {
+  if (!(extent > 0.0) || !(extent < INFINITY)) return 0;
+  if (extent >= 0x1p-16 && extent <= 0x1p16) return 0;
+  return -std::ilogb(extent);
+}
+
+void apply_unit_scale(Context &c, const double bounds[6]) {
+  const int64_t n = c.n;
+  hipStream_t s = c.stream;
+  c.scale_exp = 0;
+  double extent = 0.0;
+  for (int a = 0; a < 3; ++a) extent = std::max(extent, bounds[3 + a] - bounds[a]);
+  const int e = scale_exponent_for(extent);
+  if (e == 0) return;
+  MHGP_HIP(hipMemsetAsync(c.status.p + 4, 0, sizeof(uint32_t), s));
+  hipLaunchKernelGGL(k_scale_check, dim3(blocks_for(n, 256)), dim3(256), 0, s, c.X.p, c.Y.p, c.Z.p, n, e,
+                     c.status.p + 4);
+  uint32_t bad = 0;
+  copy_wait(&bad, c.status.p + 4, 4, hipMemcpyDeviceToHost, s);
+  if (bad) return;  // keep the input unit: exactness first
+  for (double *v : {c.X.p, c.Y.p, c.Z.p, c.LX.p, c.LY.p, c.LZ.p})
+    hipLaunchKernelGGL(k_scale_values, dim3(blocks_for(n, 256)), dim3(256), 0, s, v, n, e);
+  hipLaunchKernelGGL(k_scale_values, dim3(1), dim3(64), 0, s, c.bounds.p, (int64_t)6, e);
+  if (n > 1)
+    hipLaunchKernelGGL(k_scale_nodes, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, c.nodes.p, n - 1, e);
+  MHGP_LAUNCH_CHECK();
+  c.scale_exp = e;
+}
+
 }  // namespace
 
 void canonicalize_and_build(Context &c) {
   const int64_t n = c.n;
+  c.scale_exp = 0;
   hipStream_t s = c.stream;
   const int T = 256;
   c.X.ensure(n); c.Y.ensure(n); c.Z.ensure(n);
@@ -428,6 +492,9 @@ void canonicalize_and_build(Context &c) {
   if (st[0] & 2u) throw std::invalid_argument("a point coordinate is not finite");
   if (st[0] & 1u) throw std::invalid_argument("the cloud contains duplicate geometric points");
   if (st[2]) throw std::runtime_error("an exact Morton bin could not be certified");
+  double bounds[6];
+  copy_wait(bounds, c.bounds.p, sizeof(bounds), hipMemcpyDeviceToHost, s);
+  apply_unit_scale(c, bounds);
   const int64_t internal = std::max<int64_t>(n - 1, 0);
   c.wide.ensure(internal);
   if (internal)

@@ -117,6 +117,7 @@ __host__ __device__ inline void dd_cross(const DDE *a, const DDE *b, DDE *out) {
 // the exact differences to ids[0].
 __host__ __device__ inline DDE dd_level(const double *X, const double *Y, const double *Z,
                                         const uint32_t *ids, int m) {
+  if (m <= 1) return {0.0, 0.0, 0.0};  // a single point: level zero, exactly
   const uint32_t u = ids[0];
   DDE a[3], b[3], d[3];
   a[0] = dd_diff(X[ids[1]], X[u]);

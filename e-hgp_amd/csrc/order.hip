@@ -15,6 +15,7 @@
 #include "pipeline.hpp"
 #include "scan.hpp"
 #include "level_dd.hpp"
+#include "level_links.hpp"
 #include "supports.hpp"
 #include "timing.hpp"
 
@@ -30,15 +31,24 @@ __global__ void k_level_keys(const EventRec *__restrict__ ev, int64_t n, uint64_
   vals[i] = (uint32_t)i;
 }
 
-__global__ void k_links(const EventRec *__restrict__ ev, const uint32_t *__restrict__ order, int64_t n,
-                        uint32_t *link) {
+// Upper bound of every record's level interval, in sorted order (the bits of
+// a non-negative double order like the double).
+__global__ void k_level_upper(const EventRec *__restrict__ ev, const uint32_t *__restrict__ order, int64_t n,
+                              uint64_t *hi) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const EventRec &a = ev[order[i]];
+  hi[i] = dbits(level_upper(a.level, a.level_err));
+}
+
+// prefmax: inclusive prefix maximum of the upper bounds (level_links.hpp).
+__global__ void k_links(const EventRec *__restrict__ ev, const uint32_t *__restrict__ order,
+                        const uint64_t *__restrict__ prefmax, int64_t n, uint32_t *link) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (i == 0) { link[0] = 1; return; }
-  const EventRec &a = ev[order[i - 1]], &b = ev[order[i]];
-  const double ahi = a.level * (1.0 + 2.0 * (double)a.level_err);
-  const double blo = b.level * (1.0 - 2.0 * (double)b.level_err);
-  link[i] = blo > ahi ? 1u : 0u;
+  const EventRec &b = ev[order[i]];
+  link[i] = level_link(level_lower(b.level, b.level_err), __longlong_as_double((long long)prefmax[i - 1])) ? 1u : 0u;
 }
 
 // Start of every run of records whose mutual order the FP64 intervals leave open.
@@ -196,7 +206,12 @@ void order_catalogue(Context &c, CatalogBuffers &B) {
                      B.sort_keys.p, B.order.p);
   c.sorter.sort(B.sort_keys.p, B.order.p, n, 0, 64, s);
   tm.mark("catalogue level sort");
-  hipLaunchKernelGGL(k_links, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.events.p, B.order.p, n, B.cls.p);
+  // the sort keys are free once sorted: they take the prefix maxima
+  hipLaunchKernelGGL(k_level_upper, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.events.p, B.order.p, n,
+                     B.sort_keys.p);
+  c.scanner.inclusive_max(B.sort_keys.p, n, s);
+  hipLaunchKernelGGL(k_links, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.events.p, B.order.p, B.sort_keys.p, n,
+                     B.cls.p);
   // open runs: starts, then a double-double pass over each
   const uint64_t cap = std::max<int64_t>(1 << 16, n / 4);
   B.ambiguous.ensure(3 * cap);

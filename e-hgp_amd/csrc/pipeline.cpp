@@ -13,7 +13,18 @@
 namespace mhgp {
 
 Context::Context() = default;
-Context::~Context() = default;
+Context::~Context() {
+  // device buffers first (their owners are members), then the stream
+  tower_lanes.clear();
+  tower_work.reset();
+  tower.reset();
+  host.reset();
+  emst.reset();
+  catalog.reset();
+  if (anchor_clock.a) { (void)hipEventDestroy(anchor_clock.a); (void)hipEventDestroy(anchor_clock.b); }
+  if (arm_clock.a) { (void)hipEventDestroy(arm_clock.a); (void)hipEventDestroy(arm_clock.b); }
+  if (stream) (void)hipStreamDestroy(stream);
+}
 
 int64_t device_bytes(const Context &c) {
   int64_t b = c.src.bytes() + c.X.bytes() * 3 + c.source_index.bytes() + c.key0.bytes() * 3 +
@@ -68,11 +79,13 @@ void canonical_points(Context &c, double *xyz_out, uint32_t *source_index_out) {
   if (source_index_out)
     MHGP_HIP(hipMemcpyAsync(source_index_out, c.source_index.p, n * 4, hipMemcpyDeviceToHost, c.stream));
   MHGP_HIP(hipStreamSynchronize(c.stream));
+  // back to the input unit (an exact power of two)
+  const int e = -c.scale_exp;
   if (xyz_out)
     for (int64_t i = 0; i < n; ++i) {
-      xyz_out[3 * i] = x[i];
-      xyz_out[3 * i + 1] = y[i];
-      xyz_out[3 * i + 2] = z[i];
+      xyz_out[3 * i] = std::ldexp(x[i], e);
+      xyz_out[3 * i + 1] = std::ldexp(y[i], e);
+      xyz_out[3 * i + 2] = std::ldexp(z[i], e);
     }
 }
 

@@ -6,8 +6,8 @@
 // replaced by an addition of magnitudes (Pabs).  With every input and every
 // operation rounded to nearest, |fl(P) - P| <= k u Pabs for k operations on
 // the longest chain; k u stays below 1e-14 for these polynomials, so a sign
-// is certified when |fl(P)| > kCert * Pabs (kCert = 1e-12) and left to the
-// exact host path otherwise.
+// is certified when |fl(P)| > kCert * Pabs + kCertAbs (kCert = 1e-12; the
+// absolute term covers underflow) and left to the exact host path otherwise.
 //
 // Formulas (a, b, d offsets of the other support points, p of the query):
 //  pair:        P = p.(p - a)                         (diametral sphere)
@@ -31,9 +31,17 @@ namespace mhgp {
 
 constexpr double kCert = 1e-12;
 
+// Absolute part of the error bound.  Once partial products reach the
+// subnormal range each rounding adds up to an absolute 2^-1074 (and up to
+// 2^-1022 where a unit flushes subnormals), which no relative bound covers;
+// the polynomials here take fewer than 200 roundings, so kCertAbs exceeds
+// their total absolute error with a wide margin.  Values below it are left
+// undecided, i.e. to the exact path.
+constexpr double kCertAbs = 1e-300;
+
 // -1 / +1 when certain, 0 when undecided
 __host__ __device__ inline int cert_sign(double v, double vabs) {
-  const double e = kCert * vabs;
+  const double e = kCert * vabs + kCertAbs;
   return v > e ? 1 : (v < -e ? -1 : 0);
 }
 

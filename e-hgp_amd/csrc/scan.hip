@@ -89,7 +89,69 @@ __global__ void __launch_bounds__(kScanBlock) k_add_offsets(uint32_t *data, int6
   store16<VEC>(data, base, n, v);
 }
 
+// Prefix maximum over uint64: 256 lanes x 8 consecutive values per tile,
+// the same three phases as the sum.
+constexpr int kMaxPerThread = 8;
+constexpr int kMaxTileItems = kScanBlock * kMaxPerThread;
+
+__device__ inline uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b; }
+
+__global__ void __launch_bounds__(kScanBlock) k_block_max(uint64_t *data, int64_t n, uint64_t *tops) {
+  __shared__ uint64_t wtop[kScanBlock / 64];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int64_t base = (int64_t)blockIdx.x * kMaxTileItems + (int64_t)t * kMaxPerThread;
+  uint64_t v[kMaxPerThread];
+#pragma unroll
+  for (int i = 0; i < kMaxPerThread; ++i) v[i] = base + i < n ? data[base + i] : 0ull;
+#pragma unroll
+  for (int i = 1; i < kMaxPerThread; ++i) v[i] = umax64(v[i], v[i - 1]);
+  uint64_t x = v[kMaxPerThread - 1];
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t y = __shfl_up(x, (unsigned)d, 64);
+    if (lane >= d) x = umax64(x, y);
+  }
+  if (lane == 63) wtop[wave] = x;
+  __syncthreads();
+  uint64_t before = 0, top = 0;
+#pragma unroll
+  for (int w = 0; w < kScanBlock / 64; ++w) {
+    before = w < wave ? umax64(before, wtop[w]) : before;
+    top = umax64(top, wtop[w]);
+  }
+  // lanes before this one in the wave
+  uint64_t prev = __shfl_up(x, 1u, 64);
+  if (lane == 0) prev = 0;
+  const uint64_t prefix = umax64(before, prev);
+#pragma unroll
+  for (int i = 0; i < kMaxPerThread; ++i)
+    if (base + i < n) data[base + i] = umax64(v[i], prefix);
+  if (t == 0) tops[blockIdx.x] = top;
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_apply_max(uint64_t *data, int64_t n, const uint64_t *tops) {
+  if (blockIdx.x == 0) return;
+  const uint64_t m = tops[blockIdx.x - 1];
+  const int64_t base = (int64_t)blockIdx.x * kMaxTileItems + (int64_t)threadIdx.x * kMaxPerThread;
+#pragma unroll
+  for (int i = 0; i < kMaxPerThread; ++i)
+    if (base + i < n) data[base + i] = umax64(data[base + i], m);
+}
+
 }  // namespace
+
+void Scanner::inclusive_max(uint64_t *data, int64_t n, hipStream_t s, int depth) {
+  if (n <= 0) return;
+  if (depth >= 4) throw std::runtime_error("scan too deep");
+  const int64_t blocks = (n + kMaxTileItems - 1) / kMaxTileItems;
+  uint64_t *tops = level64[depth].ensure(blocks);
+  hipLaunchKernelGGL(k_block_max, dim3(blocks), dim3(kScanBlock), 0, s, data, n, tops);
+  if (blocks > 1) {
+    inclusive_max(tops, blocks, s, depth + 1);
+    hipLaunchKernelGGL(k_apply_max, dim3(blocks), dim3(kScanBlock), 0, s, data, n, tops);
+  }
+  MHGP_LAUNCH_CHECK();
+}
 
 void Scanner::inclusive(uint32_t *data, int64_t n, hipStream_t s, int depth) {
   if (n <= 0) return;

@@ -51,51 +51,117 @@ struct Server {
   int64_t cap = 0;
   std::vector<std::thread> workers;
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
+  std::condition_variable cv_work, cv_done, cv_start;
   std::deque<std::shared_ptr<Request>> queue;
   std::unordered_map<int64_t, std::shared_ptr<Request>> all;
   int64_t next_ticket = 1;
-  int64_t in_flight_bytes = 0;
+  // device bytes reserved per slot: what its context retains between
+  // requests, or, while it serves one, the larger of that and the request's
+  // prediction (contexts keep their grown buffers for the next request)
+  std::vector<int64_t> reserved;
+  int started = 0, live = 0, start_failures = 0;
   bool stopping = false;
   std::string error;
 
-  void worker_main();
+  int64_t reserved_total() const {
+    int64_t t = 0;
+    for (int64_t r : reserved) t += r;
+    return t;
+  }
+  void worker_main(int slot);
+  void fail_queue_locked(int code);
 };
 
-void Server::worker_main() {
-  Context c;
-  c.device = device;
-  c.tower_lane_count = 1;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
-    std::lock_guard<std::mutex> g(mu);
-    error = "worker could not create its stream";
-    return;
+// A fresh context on the slot's device with its own stream.
+static std::unique_ptr<Context> fresh_context(int device) {
+  auto c = std::make_unique<Context>();
+  c->device = device;
+  c->tower_lane_count = 1;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return nullptr;
+  return c;
+}
+
+// Requests nobody can serve any more end with MHGP_E_DEVICE (caller holds mu).
+void Server::fail_queue_locked(int code) {
+  const auto now = Clock::now();
+  for (auto &r : queue) {
+    r->result.code = code;
+    r->start = r->end = now;
+    r->result.latency_ms = std::chrono::duration<double, std::milli>(now - r->arrival).count();
+    r->done = true;
   }
+  queue.clear();
+}
+
+void Server::worker_main(int slot) {
+  std::unique_ptr<Context> ctx = fresh_context(device);
+  {
+    std::lock_guard<std::mutex> g(mu);
+    started++;
+    if (ctx) live++;
+    else {
+      start_failures++;
+      error = "worker could not create its stream";
+    }
+  }
+  cv_start.notify_all();
+  if (!ctx) return;
   while (true) {
     std::shared_ptr<Request> r;
     {
       std::unique_lock<std::mutex> g(mu);
+      // admission: the slot's share after taking r must fit beside the other
+      // slots' reservations; a request larger than the cap runs alone
+      auto fits = [&](const Request &q) {
+        const int64_t others = reserved_total() - reserved[slot];
+        const int64_t mine = std::max(reserved[slot], q.predicted);
+        return others + mine <= cap || others == 0;
+      };
       cv_work.wait(g, [&] {
         if (stopping) return true;
-        for (auto &q : queue)
-          if (in_flight_bytes + q->predicted <= cap || in_flight_bytes == 0) return true;
+        for (auto &q : queue) {
+          if (fits(*q)) return true;
+          // the retained buffers alone are in the way: they are released below
+          if (reserved_total() - reserved[slot] + q->predicted <= cap) return true;
+        }
         return false;
       });
       if (stopping && queue.empty()) break;
-      for (auto it = queue.begin(); it != queue.end(); ++it)
-        if (in_flight_bytes + (*it)->predicted <= cap || in_flight_bytes == 0) {
+      bool trim = false;
+      for (auto it = queue.begin(); it != queue.end(); ++it) {
+        if (fits(**it)) {
           r = *it;
           queue.erase(it);
           break;
         }
+        if (reserved_total() - reserved[slot] + (*it)->predicted <= cap) trim = true;
+      }
+      if (!r && trim) {
+        // only what this slot retains from earlier requests is in the way:
+        // release it (outside the lock) and scan the queue again
+        reserved[slot] = 0;
+        g.unlock();
+        ctx.reset();
+        ctx = fresh_context(device);
+        g.lock();
+      }
+      if (!ctx) {
+        live--;
+        error = "worker could not recreate its stream";
+        if (live == 0) fail_queue_locked(MHGP_E_DEVICE);
+        cv_done.notify_all();
+        cv_work.notify_all();
+        return;
+      }
       if (!r) continue;
-      in_flight_bytes += r->predicted;
+      reserved[slot] = std::max(reserved[slot], r->predicted);
     }
     r->start = Clock::now();
     int status = 0;
+    Context &c = *ctx;
     try {
       MHGP_HIP(hipSetDevice(device));
-      c.poisoned = false;
       upload_host(c, r->xyz.data(), (int64_t)(r->xyz.size() / 3));
       std::vector<uint32_t> edges(2 * std::max<int64_t>(c.n - 1, 0));
       emst_host(c, edges.data());
@@ -121,27 +187,43 @@ void Server::worker_main() {
       status = MHGP_E_CAPACITY;
     } catch (const HipError &e) {
       status = MHGP_E_DEVICE;
+      c.poisoned = true;
       std::lock_guard<std::mutex> g(mu);
       error = e.what();
     } catch (const std::exception &e) {
       status = MHGP_E_INTERNAL;
+      c.poisoned = true;
       std::lock_guard<std::mutex> g(mu);
       error = e.what();
+    }
+    // fail closed: a context whose device state may be inconsistent never
+    // serves again; the slot continues on a fresh one
+    const bool retire = c.poisoned;
+    const int64_t retained = retire ? 0 : device_bytes(c);
+    if (retire) {
+      ctx.reset();
+      ctx = fresh_context(device);
     }
     r->end = Clock::now();
     {
       std::lock_guard<std::mutex> g(mu);
-      in_flight_bytes -= r->predicted;
+      reserved[slot] = retained;
       r->result.code = status;
       r->result.queue_ms = std::chrono::duration<double, std::milli>(r->start - r->arrival).count();
       r->result.service_ms = std::chrono::duration<double, std::milli>(r->end - r->start).count();
       r->result.latency_ms = std::chrono::duration<double, std::milli>(r->end - r->arrival).count();
       r->done = true;
+      if (!ctx) {
+        live--;
+        error = "worker could not recreate its stream after a device error";
+        if (live == 0) fail_queue_locked(MHGP_E_DEVICE);
+      }
     }
     cv_done.notify_all();
     cv_work.notify_all();
+    if (!ctx) return;
   }
-  (void)hipStreamSynchronize(c.stream);
+  (void)hipStreamSynchronize(ctx->stream);
 }
 
 }  // namespace mhgp
@@ -163,7 +245,18 @@ int mhgp_server_create(int device, int slots, int64_t memory_cap_bytes, mhgp_ser
   auto *h = new mhgp_server();
   h->s.device = device;
   h->s.cap = memory_cap_bytes > 0 ? memory_cap_bytes : (int64_t)(0.8 * (double)total_b);
-  for (int i = 0; i < slots; ++i) h->s.workers.emplace_back([h] { h->s.worker_main(); });
+  h->s.reserved.assign(slots, 0);
+  for (int i = 0; i < slots; ++i) h->s.workers.emplace_back([h, i] { h->s.worker_main(i); });
+  bool failed = false;
+  {
+    std::unique_lock<std::mutex> g(h->s.mu);
+    h->s.cv_start.wait(g, [&] { return h->s.started == slots; });
+    failed = h->s.start_failures > 0;
+  }
+  if (failed) {
+    mhgp_server_destroy(h);
+    return MHGP_E_DEVICE;
+  }
   *out = h;
   return MHGP_OK;
 }
@@ -177,6 +270,7 @@ int mhgp_server_submit(mhgp_server *srv, const double *xyz, int64_t n, int k_max
   r->arrival = mhgp::Clock::now();
   {
     std::lock_guard<std::mutex> g(srv->s.mu);
+    if (srv->s.live == 0) return MHGP_E_DEVICE;
     r->ticket = srv->s.next_ticket++;
     srv->s.queue.push_back(r);
     srv->s.all[r->ticket] = r;
@@ -192,7 +286,8 @@ int mhgp_server_wait(mhgp_server *srv, int64_t ticket, mhgp_request_result *resu
   auto it = srv->s.all.find(ticket);
   if (it == srv->s.all.end()) return MHGP_E_ARGUMENT;
   auto r = it->second;
-  srv->s.cv_done.wait(g, [&] { return r->done; });
+  srv->s.cv_done.wait(g, [&] { return r->done || srv->s.live == 0; });
+  if (!r->done) return MHGP_E_DEVICE;  // no live worker is left to serve it
   *result = r->result;
   return MHGP_OK;
 }

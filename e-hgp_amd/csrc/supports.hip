@@ -141,7 +141,9 @@ __device__ void emit_event(const Params &P, uint32_t u, const uint32_t *sup, int
 #pragma unroll
   for (int i = 0; i < kMaxInterior; ++i) e.interior[i] = i < nint ? interior[i] : kInvalid;
   e.level = level;
-  e.level_err = (float)fmin(lerr, 1e-3);
+  // rounded up into float (an infinite bound only sends the record to the
+  // exact host ordering)
+  e.level_err = (float)lerr * (1.0f + 1e-6f);
 }
 
 // Queues undecided work and, once per anchor, its region for the host.
@@ -309,7 +311,7 @@ __device__ inline void sweep_key(double Pw, double PA, double H, double HA, doub
   r = __builtin_fma(r, e, r);
   const double rerr = 2.0 * fabs(__builtin_fma(-h, r, 1.0));
   tau = Pw * r;
-  terr = (kCert * PA + kCert * HA * fabs(tau)) * r * (1.0 + 1e-12) + (1e-15 + rerr) * fabs(tau);
+  terr = (kCert * PA + kCertAbs + (kCert * HA + kCertAbs) * fabs(tau)) * r * (1.0 + 1e-12) + (1e-15 + rerr) * fabs(tau);
   if (!(h > 1e-290 && h < 1e290)) {
     tau = 0.0;
     terr = INFINITY;
@@ -1217,21 +1219,27 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
   uint64_t event_cap = std::max<uint64_t>(1024, (uint64_t)n * (smax >= 11 ? 640 : 64 * smax));
   uint64_t fb_cap = std::max<uint64_t>(4096, (uint64_t)n / 4);
   uint64_t reg_cap = std::max<uint64_t>(1 << 20, (uint64_t)n * 64);
+  DevBuf<uint32_t> overflow;
+  DevBuf<uint64_t> total;
+  overflow.ensure(1);
+  total.ensure(1);
+  int64_t widened = 0;
+  uint64_t ctr[8];
+  // The record queues have a fixed capacity per launch; records past it are
+  // dropped on the device and counted.  A catalogue that outgrows them runs
+  // again with queues sized from the counts (the widened gather radii are
+  // kept), unless the device cannot hold them.
+  for (int attempt = 0;; ++attempt) {
   B.events.ensure(event_cap);
   B.fallbacks.ensure(fb_cap);
   B.fb_region.ensure(reg_cap);
   MHGP_HIP(hipMemsetAsync(B.counters.p, 0, 8 * 8, s));
   MHGP_HIP(hipMemsetAsync(B.status.p, 0, n * 4, s));
-  DevBuf<uint32_t> overflow;
-  DevBuf<uint64_t> total;
-  overflow.ensure(1);
-  total.ensure(1);
   MHGP_HIP(hipMemsetAsync(overflow.p, 0, 4, s));
-  int64_t widened = 0;
   std::vector<uint32_t> pending = anchors;
   std::vector<uint32_t> big;
   for (int iter = 0; !pending.empty(); ++iter) {
-    if (iter > 12) throw std::runtime_error("anchor gather radii did not converge");
+    if (iter > 12) throw CapacityError("anchor gather radii did not converge");
     const uint32_t cnt = (uint32_t)pending.size();
     MHGP_HIP(hipMemcpyAsync(B.anchors.p, pending.data(), cnt * 4, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_gather<false>, dim3(blocks_for(cnt, 128)), dim3(128), 0, s, n, c.wide.p,
@@ -1393,13 +1401,29 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
   }
   tm.mark("catalogue big anchors");
   uint32_t ovf = 0;
-  uint64_t ctr[8];
   copy_wait(&ovf, overflow.p, 4, hipMemcpyDeviceToHost, s);
   copy_wait(ctr, B.counters.p, sizeof(ctr), hipMemcpyDeviceToHost, s);
   if (ovf) throw std::runtime_error("LBVH traversal stack overflow in the gather");
-  if (ctr[0] > event_cap || ctr[1] > fb_cap || ctr[2] > reg_cap)
-    throw std::runtime_error("catalogue queue capacity exceeded");
   if (ctr[4]) throw std::runtime_error("face side queue overflow");
+  if (ctr[0] <= event_cap && ctr[1] <= fb_cap && ctr[2] <= reg_cap) break;
+  if (attempt >= 2) throw CapacityError("catalogue record queues did not settle after regrowth");
+  const uint64_t ev2 = std::max(event_cap, ctr[0] + ctr[0] / 8), fb2 = std::max(fb_cap, ctr[1] + ctr[1] / 8),
+                 rg2 = std::max(reg_cap, ctr[2] + ctr[2] / 8);
+  size_t free_b = 0, total_b = 0;
+  MHGP_HIP(hipMemGetInfo(&free_b, &total_b));
+  const uint64_t more = (ev2 - std::min<uint64_t>(ev2, B.events.cap)) * sizeof(EventRec) +
+                        (fb2 - std::min<uint64_t>(fb2, B.fallbacks.cap)) * sizeof(FallbackRec) +
+                        (rg2 - std::min<uint64_t>(rg2, B.fb_region.cap)) * 4;
+  if (more > free_b / 10 * 9)
+    throw CapacityError("the catalogue needs " + std::to_string(ctr[0]) + " record slots, more than the device can hold");
+  event_cap = ev2;
+  fb_cap = fb2;
+  reg_cap = rg2;
+  // grown buffers start empty: release before the larger allocation
+  if (B.events.cap < event_cap) B.events.release();
+  if (B.fallbacks.cap < fb_cap) B.fallbacks.release();
+  if (B.fb_region.cap < reg_cap) B.fb_region.release();
+  }
   // records stay on the device; only the host's share of the work moves
   (void)events;
   B.count = (int64_t)ctr[0];

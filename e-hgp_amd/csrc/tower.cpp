@@ -499,6 +499,7 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
   T.forests = std::move(forests);
   T.descent_steps += desc ? desc->steps : 0;
   T.orders = K;
+  T.level_exp = -2 * c.scale_exp;
   T.status = 0;
   T.valid = true;
   info->node_count = T.node_count;

@@ -1,6 +1,7 @@
 // Host state of the hgp_reduced tower.
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <thread>
 #include <vector>
@@ -46,6 +47,7 @@ struct TowerState {
   int64_t descent_steps = 0;
   int32_t orders = 0;
   int32_t status = 0;
+  int level_exp = 0;  // levels are reported times 2^level_exp (device unit -> input unit)
 
   // The tower as one node list: orders ascending, node ids and child offsets
   // shifted by the nodes and children of the orders below (one thread per
@@ -56,12 +58,13 @@ struct TowerState {
     std::vector<std::thread> th;
     int64_t nb = 0, cb = 0;
     for (const Forest &F : forests) {
-      th.emplace_back([&F, nb, cb, order, kind, level_class, level, child_offset, children] {
+      const int lexp = level_exp;
+      th.emplace_back([&F, nb, cb, lexp, order, kind, level_class, level, child_offset, children] {
         const size_t m = F.order.size();
         std::copy(F.order.begin(), F.order.end(), order + nb);
         std::copy(F.kind.begin(), F.kind.end(), kind + nb);
         std::copy(F.level_class.begin(), F.level_class.end(), level_class + nb);
-        std::copy(F.level.begin(), F.level.end(), level + nb);
+        for (size_t i = 0; i < m; ++i) level[nb + (int64_t)i] = std::ldexp(F.level[i], lexp);
         for (size_t i = 1; i <= m; ++i) child_offset[nb + (int64_t)i] = cb + F.child_offset[i];
         for (size_t j = 0; j < F.children.size(); ++j) children[cb + (int64_t)j] = nb + F.children[j];
       });

@@ -18,8 +18,10 @@ __device__ inline double box_d2(const double q[3], const double lo[3], const dou
 }
 
 // Relative slack dominating every FP64 rounding error of the distance and
-// bound computations used for pruning (errors are below 1e-15 relative).
+// bound computations used for pruning (errors are below 1e-15 relative),
+// and an absolute one for the underflow of squares in the subnormal range.
 constexpr double kSlack = 1e-12;
+constexpr double kSlackAbs = 1e-300;
 
 // FP64 squared distance with an exactness flag: exact when every difference,
 // square and partial sum is error-free.
@@ -37,6 +39,8 @@ __device__ inline double d2_flag(const double p[3], double x, double y, double z
     exact = exact && de == 0.0 && se == 0.0 && te == 0.0;
     s = t;
   }
+  // below the normal range the residuals themselves underflow to zero
+  exact = exact && (s == 0.0 || s >= 0x1p-960);
   return s;
 }
 

@@ -125,6 +125,31 @@ double BigInt::to_double() const {
   return sign_ < 0 ? -v : v;
 }
 
+double BigInt::top_bits(int &shift) const {
+  shift = 0;
+  if (mag_.empty()) return 0.0;
+  const size_t w = mag_.size();
+  uint64_t top = mag_[w - 1];
+  if (w >= 2) top = (top << 32) | mag_[w - 2];
+  int lead = 64 - __builtin_clzll(top);  // bits of top in use
+  // fill up to 64 leading bits from the next word
+  if (w >= 3 && lead < 64) {
+    const int need = 64 - lead;
+    top = (top << need) | ((uint64_t)mag_[w - 3] >> (32 - need));
+    shift = (int)(32 * (w - 3)) + (32 - need);
+  } else {
+    shift = w >= 2 ? (int)(32 * (w - 2)) : 0;
+  }
+  const double v = (double)top;
+  return sign_ < 0 ? -v : v;
+}
+
+double ratio_to_double(const BigInt &num, const BigInt &den, int exp2) {
+  int sn = 0, sd = 0;
+  const double tn = num.top_bits(sn), td = den.top_bits(sd);
+  return std::ldexp(tn / td, sn - sd + exp2);
+}
+
 static void decode_binary64(double x, int64_t &sig, int &exp) {
   uint64_t b;
   std::memcpy(&b, &x, 8);

@@ -25,6 +25,9 @@ class BigInt {
   BigInt operator-() const;
   friend int compare(const BigInt &a, const BigInt &b);
   double to_double() const;
+  // Leading 64 bits as a double and the count of bits below them:
+  // value = top * 2^shift up to a relative 2^-52 (no overflow at any size).
+  double top_bits(int &shift) const;
 
  private:
   static int cmp_mag(const std::vector<uint32_t> &a, const std::vector<uint32_t> &b);
@@ -73,5 +76,8 @@ struct ExactSupport {
 
 // sign(l1 - l2) for squared radii given as num/den with den > 0
 int compare_levels(const BigInt &n1, const BigInt &d1, const BigInt &n2, const BigInt &d2);
+
+// num / den * 2^exp2 in binary64, relative error below 1e-15 (num, den > 0).
+double ratio_to_double(const BigInt &num, const BigInt &den, int exp2);
 
 }  // namespace mhgp

@@ -106,10 +106,9 @@ def test_capacity_error_is_clean_and_context_survives():
     print("capacity case: snapped mixture n=50000 k=10", flush=True)
     with mhgp.Context(0) as ctx:
         ctx.upload(pts)
-        try:
+        with pytest.raises(mhgp.MhgpError) as err:
             ctx.supports(10, fetch=False)
-        except mhgp.MhgpError as err:
-            assert err.code == mhgp.E_CAPACITY, err
+        assert err.value.code == mhgp.E_CAPACITY, err.value
         small = workloads.uniform_cloud(400, seed=9)
         ctx.upload(small)
         cat = ctx.supports(4)
