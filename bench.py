@@ -33,6 +33,7 @@ import workloads  # noqa: E402
 
 METRIC = "p95 warm_e2e ms (50k pts, Kmax=10); served clouds/s + p99 vs plain path"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+FP64_PEAK = 78.6e12  # MI355X FP64 vector, FLOP/s
 
 
 def run_step(ctx, pts, k_max):
@@ -43,33 +44,102 @@ def run_step(ctx, pts, k_max):
     return edges, info, tower
 
 
-def cpu_baseline(pts, k_max, budget_s=20.0):
-    """The exact CPU oracle on a bounded sample of anchors of the same cloud,
-    scaled to the whole cloud (the oracle enumerates each anchor's supports
-    exhaustively inside a fixed neighbourhood, single thread)."""
+C1_N, C1_K = 256, 3
+
+
+def c1_cloud():
+    """The plain-method CPU reference configuration: 256 points i.i.d. uniform
+    in [0, 1)^3 (seeded), K_max = 3."""
+    return workloads.uniform_cloud(C1_N, seed=0)
+
+
+def cpu_baseline_c1(pts, k_max=C1_K):
+    """The exact CPU oracle (one thread) over the whole pipeline of the
+    256-point cloud: canonical PointIds, exact Morton LBVH, exact EMST, the
+    exhaustive exact catalogue (every anchor, every support of 2..4 points,
+    exact classification against every point) and the direct hgp_reduced
+    tower by exact facet descents."""
+    import oracle
+    from oracle import tower as otower
+
+    t0 = time.perf_counter()
+    canon = oracle.canonicalize(pts)
+    oracle.lbvh_snapshot(canon)
+    edges = oracle.emst(canon)
+    # every sphere through a point of the unit cube with centre inside it has
+    # radius below sqrt(3): radius 4 encloses every support and its interior
+    recs = []
+    for a in range(len(canon)):
+        recs += oracle.anchor_records(canon, k_max, a, 4.0)
+    forests = otower.direct_forests(canon, k_max, events=recs)
+    elapsed = time.perf_counter() - t0
+    return {
+        "value": round(1e3 * elapsed, 1),
+        "unit": "ms",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"one complete run of the 256-point uniform cloud at K_max={k_max} (canonical ids, LBVH, "
+                  f"EMST, exhaustive exact catalogue of {len(recs)} records, direct tower), single thread",
+    }, (canon, edges, recs, forests)
+
+
+def c1_gpu(pts, oracle_result, steps=20, warmup=3, k_max=C1_K):
+    """The same 256-point cloud on the GPU (warm_e2e), checked against the
+    oracle's EMST, catalogue and tower."""
+    from oracle import tower as otower
+
+    canon, ref_edges, recs, forests = oracle_result
+    ctx = mhgp.Context(0)
+    for _ in range(warmup):
+        run_step(ctx, pts, k_max)
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        edges, _, tw = run_step(ctx, pts, k_max)
+        times.append(1e3 * (time.perf_counter() - t0))
+    cat = ctx.supports(k_max)
+    ctx.close()
+    ok = np.array_equal(edges.astype(np.int64), ref_edges)
+    exact = {(r.support, r.interior): r.level for r in recs}
+    mine = {(tuple(int(x) for x in cat.ids[i, : cat.size[i]]), cat.interior[i]) for i in range(len(cat.size))}
+    ok &= mine == set(exact)
+    level_of = {-1: 0}
+    for i in range(len(cat.size)):
+        level_of[int(cat.level_class[i])] = exact[(tuple(int(x) for x in cat.ids[i, : cat.size[i]]), cat.interior[i])]
+    off = tw["child_offset"]
+    for k in range(1, k_max + 1):
+        idx = [i for i in range(len(tw["order"])) if tw["order"][i] == k]
+        remap = {old: new for new, old in enumerate(idx)}
+        nodes = [{"order": k, "kind": "merge" if tw["kind"][i] == 2 else "birth",
+                  "level": level_of[int(tw["level_class"][i])],
+                  "children": tuple(sorted(remap[int(c)] for c in tw["children"][off[i]:off[i + 1]])),
+                  "covered": None} for i in idx]
+        ok &= otower.signatures(nodes) == otower.signatures(forests[k])
+    t = np.asarray(times)
+    return {"gpu_p50_ms": round(float(np.percentile(t, 50)), 3), "gpu_p95_ms": round(float(np.percentile(t, 95)), 3),
+            "steps": steps, "equal_to_oracle": bool(ok)}
+
+
+def c2_cpu_extrapolated(pts, radii, k_max, budget_s=8.0):
+    """The exhaustive exact oracle on a bounded sample of anchors of the
+    50 000-point cloud (each anchor's records among the points within its
+    certified radius), scaled to every anchor: an extrapolation, not a run."""
     import oracle
 
     canon = oracle.canonicalize(pts)
     n = len(canon)
     rng = np.random.default_rng(0)
-    anchors = rng.choice(n, size=n, replace=False)
     done, t0 = 0, time.perf_counter()
-    for a in anchors:
-        oracle.anchor_catalog(canon, k_max, int(a), radius=3.0)
+    for a in rng.choice(n, size=n, replace=False):
+        oracle.anchor_records(canon, k_max, int(a), float(radii[a]))
         done += 1
         if time.perf_counter() - t0 > budget_s:
             break
-    elapsed = time.perf_counter() - t0
-    per_anchor_ms = 1e3 * elapsed / done
-    return {
-        "value": per_anchor_ms * n,
-        "unit": "ms",
-        "cores": 1,
-        "kind": "port",
-        "sample": f"{done} random anchors of the 50k lattice: exhaustive exact supports "
-                  f"(anchor = least id) within radius 3 spacings, {per_anchor_ms:.1f} ms/anchor, "
-                  f"scaled x{n}",
-    }
+    per_anchor_ms = 1e3 * (time.perf_counter() - t0) / done
+    return {"value": round(per_anchor_ms * n, 1), "unit": "ms", "cores": 1, "kind": "port",
+            "extrapolated": True,
+            "sample": f"{done} random anchors of the 50k lattice, exhaustive exact records within the certified "
+                      f"radius, {per_anchor_ms:.1f} ms per anchor, scaled x{n}"}
 
 
 def copy_calibration(mib: int) -> None:
@@ -197,8 +267,8 @@ def main() -> None:
     ap.add_argument("--n", type=int, default=50_000)
     ap.add_argument("--kmax", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--served", action="store_true", help="add the served-stream leg")
-    ap.add_argument("--batched", action="store_true", help="add the 256-cloud batch leg")
+    ap.add_argument("--no-served", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-cloud batch leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain kernel of known traffic (counter scaling)")
     args = ap.parse_args()
@@ -222,11 +292,16 @@ def main() -> None:
         times.append(1e3 * (time.perf_counter() - t0))
         print(f"step {s}: {times[-1]:.1f} ms", file=sys.stderr, flush=True)
     st = ctx.stats()
+    radii = ctx.radii()
     edges, info, tower = last
     times_a = np.asarray(times)
     p95 = float(np.percentile(times_a, 95))
-    anchor_ms = st.anchor_ms / max(st.anchor_launches, 1)
-    achieved = (st.anchor_bytes / max(st.anchor_launches, 1)) / (anchor_ms * 1e-3)
+    launches = max(st.anchor_launches, 1)
+    anchor_ms = st.anchor_ms / launches
+    flops = st.anchor_flops / launches
+    bytes_alg = st.anchor_bytes / launches
+    tflops = flops / (anchor_ms * 1e-3) / 1e12
+    gbs = bytes_alg / (anchor_ms * 1e-3) / 1e9
     line = {
         "metric": METRIC,
         "value": round(p95, 3),
@@ -247,6 +322,7 @@ def main() -> None:
             "events": int(info.event_count),
             "diagnostics": int(info.diagnostic_count),
             "host_fallback_predicates": int(info.fallback_count),
+            "status": "exact" if info.status == 0 and tower["info"].status == 0 else "conditional",
             "tower_nodes": int(tower["info"].node_count),
             "descent_steps": int(tower["info"].descent_steps),
             "p50_ms": round(float(np.percentile(times_a, 50)), 3),
@@ -254,23 +330,36 @@ def main() -> None:
             "max_ms": round(float(times_a.max()), 3),
             "samples_ms": [round(t, 3) for t in times],
         },
+        # The catalogue anchor kernel is bound by FP64 vector work (dependent
+        # certified predicates, divergent lanes), not by HBM: its roofline is
+        # the FP64 vector peak, with the HBM figures beside it.
         "roofline": {
             "kernel": "k_anchor_lds (critical-sphere catalogue, one workgroup per anchor)",
-            "bound": "hbm",
-            "achieved": round(achieved / 1e9, 3),
-            "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK, 6),
+            "bound": "valu",
+            "achieved": round(tflops, 4),
+            "peak": FP64_PEAK / 1e12,
+            "unit": "TFLOP/s",
+            "frac": round(tflops * 1e12 / FP64_PEAK, 5),
             "traffic": traffic_from_profile(),
             "avg_launch_ms": round(anchor_ms, 3),
-            "algorithmic_bytes_per_launch": int(st.anchor_bytes / max(st.anchor_launches, 1)),
+            "fp64_ops_per_launch": int(flops),
+            "hbm": {"algorithmic_bytes_per_launch": int(bytes_alg), "achieved": round(gbs, 3),
+                    "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(gbs * 1e9 / HBM_PEAK, 6)},
         },
-        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(pts, args.kmax),
+        "cpu_baseline": None,
     }
     ctx.close()
-    if args.served:
+    if not args.no_cpu_baseline:
+        c1 = c1_cloud()
+        base, ref = cpu_baseline_c1(c1)
+        base["gpu_same_cloud"] = c1_gpu(c1, ref)
+        base["c2_extrapolated"] = c2_cpu_extrapolated(pts, radii, args.kmax)
+        line["cpu_baseline"] = base
+        print(f"cpu baseline: {base['value']} ms (256 points), gpu {base['gpu_same_cloud']}", file=sys.stderr,
+              flush=True)
+    if not args.no_served:
         line["config"]["served"] = served_stream()
-    if args.batched:
+    if not args.no_batched:
         line["config"]["batched"] = batched()
     print(json.dumps(line))
 

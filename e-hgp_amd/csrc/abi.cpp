@@ -169,6 +169,13 @@ int mhgp_supports_fetch(mhgp_ctx *ctx, uint32_t *ids, uint32_t *meta, uint32_t *
   });
 }
 
+int mhgp_supports_radii(mhgp_ctx *ctx, double *radius_out) {
+  return guarded(ctx ? &ctx->c : nullptr, [&] {
+    mhgp::supports_radii(ctx->c, radius_out);
+    return MHGP_OK;
+  });
+}
+
 int mhgp_tower(mhgp_ctx *ctx, int k_max, mhgp_tower_info *info) {
   return guarded(ctx ? &ctx->c : nullptr, [&] {
     if (!info) throw std::invalid_argument("null info");

@@ -151,3 +151,21 @@ void supports_fetch(Context &c, uint32_t *ids, uint32_t *meta, uint32_t *interio
 }
 
 }  // namespace mhgp
+
+namespace mhgp {
+
+void supports_radii(Context &c, double *radius_out) {
+  if (!c.catalog || !c.catalog->valid) throw StateError("no catalogue has been built");
+  if (!radius_out) throw std::invalid_argument("null output buffer");
+  const int64_t n = c.n;
+  if (n < 2 || c.catalog->s_max < 2) {
+    for (int64_t i = 0; i < n; ++i) radius_out[i] = 0.0;
+    return;
+  }
+  copy_wait(radius_out, c.catalog->radius.p, n * 8, hipMemcpyDeviceToHost, c.stream);
+  // back to the input unit (an exact power of two); rounded up past the ulp
+  // the scaling cannot lose
+  for (int64_t i = 0; i < n; ++i) radius_out[i] = std::ldexp(radius_out[i], -c.scale_exp);
+}
+
+}  // namespace mhgp

@@ -49,6 +49,8 @@ void emst_host(Context &c, uint32_t *edges_out);
 void supports(Context &c, int k_max, mhgp_catalog_info *info);
 void supports_fetch(Context &c, uint32_t *ids, uint32_t *meta, uint32_t *interior, double *level,
                     int64_t *level_class);
+// per-PointId enumeration radius of the last catalogue (input unit)
+void supports_radii(Context &c, double *radius_out);
 
 // tower.cpp
 void tower(Context &c, int k_max, mhgp_tower_info *info);

@@ -104,7 +104,23 @@ struct Work {
   uint32_t *kidx;
   float4 *fc;       // FP32 offset and distance per gathered point (certificate and masks)
   uint32_t *cand;   // candidate supports classified (LDS, flushed once per anchor)
+  unsigned long long *flops;  // FP64 operations of the certified predicates (LDS, flushed once per anchor)
 };
+
+// FP64 operations (add, subtract, multiply; a fused multiply-add counts two)
+// of one evaluation of each certified predicate, counted from predicates.hpp:
+// the value, its magnitude bound and the certification test.
+constexpr uint32_t kFlopPairSide = 18;   // PairSphere::side
+constexpr uint32_t kFlopTetSide = 21;    // TetSphere::side
+constexpr uint32_t kFlopCapsule = 8;     // Capsule::outside
+constexpr uint32_t kFlopFacePoint = 33;  // TriSphere::power + height, two certified signs
+constexpr uint32_t kFlopSweepKey = 24;   // sweep_key
+constexpr uint32_t kFlopTriInit = 66;    // TriSphere::init
+constexpr uint32_t kFlopTetInit = 150;   // TetSphere::init
+
+__device__ inline void add_flops(const Work &w, uint64_t v) {
+  if (v) atomicAdd(w.flops, (unsigned long long)v);
+}
 
 __device__ inline V3 gpt(const Work &w, uint32_t i) { return {w.gx[i], w.gy[i], w.gz[i]}; }
 
@@ -177,20 +193,24 @@ __device__ int classify(const Params &P, const Work &w, uint32_t M, const S &sph
   nint = 0;
   const int cap = P.s_max - m;
   int undecided = 0;
+  constexpr uint32_t per = sizeof(S) == sizeof(PairSphere) ? kFlopPairSide : kFlopTetSide;
+  uint32_t evals = 0;
   for (uint32_t s = 0; s < M; ++s) {
     const uint32_t gi = s;
     bool on_support = false;
     for (int k = 0; k < m - 1; ++k) on_support |= (gi == skip[k]);
     if (on_support) continue;
     int side = sph.side(gpt(w, gi));
+    evals++;
     if (side < 0) {
       put_static(interior, nint, w.gid[gi]);
       nint++;
-      if (nint > cap) return 0;
+      if (nint > cap) { add_flops(w, (uint64_t)evals * per); return 0; }
     } else if (side == 0) {
       undecided++;
     }
   }
+  add_flops(w, (uint64_t)evals * per);
   return undecided ? -1 : 1;
 }
 
@@ -218,6 +238,7 @@ __device__ void process_tet(const Params &P, const Work &w, uint32_t u, uint32_t
                             uint32_t zi, uint32_t *region_slot) {
   TetSphere t;
   int wc = t.init(gpt(w, vi), gpt(w, wi), gpt(w, zi));
+  add_flops(w, kFlopTetInit);
   uint32_t ids[3] = {w.gid[vi], w.gid[wi], w.gid[zi]};
   if (wc < 0) return;
   atomicAdd(w.cand, 1u);
@@ -362,6 +383,7 @@ __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uin
   top.init();
   beh.init();
   bool undecided = false;
+  uint32_t tests = 0, evals = 0;
 #pragma unroll 1
   for (uint32_t s = 0; s < M; ++s) {
     const uint32_t gi = s;
@@ -369,7 +391,9 @@ __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uin
     const V3 p = gpt(w, gi);
     // outside every feasible sphere of the pencil: it can neither be an apex
     // nor be swept before one (dropping it only loosens the threshold)
+    tests++;
     if (cap.outside(p)) continue;
+    evals++;
     double Pw, PA, H, HA;
     f.power(p, Pw, PA);
     f.height(p, H, HA);
@@ -386,17 +410,21 @@ __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uin
       top.insert(tau, terr, s);
     }
   }
+  add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * (kFlopFacePoint + kFlopSweepKey));
   // the face pass certified every sign with the same arithmetic
   if (undecided) return;
   const double thr = top.threshold(need);
   if (top.evicted_lo <= thr) {
     // a point outside the list may still qualify (near-equal keys): full scan
+    tests = evals = 0;
 #pragma unroll 1
     for (uint32_t s = 0; s < M; ++s) {
       const uint32_t zi = s;
       if (zi == vi || zi == wi || w.gid[zi] <= wid) continue;
       const V3 p = gpt(w, zi);
+      tests++;
       if (cap.outside(p)) continue;
+      evals++;
       double Pw, PA, H, HA;
       f.power(p, Pw, PA);
       f.height(p, H, HA);
@@ -409,6 +437,7 @@ __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uin
       if (cin_side + beh.count_above(tau + terr) >= K) continue;
       process_tet(P, w, u, M, vi, wi, zi, region_slot);
     }
+    add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * (kFlopFacePoint + kFlopSweepKey));
     return;
   }
   uint32_t cand = 0;
@@ -467,6 +496,7 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
   if (ids[0] > ids[1]) { uint32_t t = ids[0]; ids[0] = ids[1]; ids[1] = t; uint32_t q = vi; vi = wi; wi = q; }
   TriSphere f;
   const int wc = f.init(gpt(w, vi), gpt(w, wi));
+  add_flops(w, kFlopTriInit);
   if (!f.nondegenerate()) {
     // collinear or undecided: a triangle cannot be a support; an undecided
     // face goes to the host together with its apexes
@@ -483,18 +513,22 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
   int cin0 = 0, cin1 = 0;  // inside the triangle's sphere, ahead of side 0 / 1
   int total_in = 0;
   uint32_t tri_int[kMaxInterior];
+  uint32_t tests = 0, evals = 0;
 #pragma unroll 1
   for (uint32_t s = 0; s < M; ++s) {
     const uint32_t gi = s;
     if (gi == vi || gi == wi) continue;
     const V3 p = gpt(w, gi);
+    tests++;
     if (cap.outside(p)) continue;  // outside the triangle's sphere and every feasible one
+    evals++;
     double Pw, PA, H, HA;
     f.power(p, Pw, PA);
     f.height(p, H, HA);
     const int sp = cert_sign(Pw, PA), sh = cert_sign(H, HA);
     // every sign the side passes use is certified here, before anything is emitted
     if (sp == 0 || sh == 0) {
+      add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopFacePoint);
       emit_fallback(P, w, u, 3, ids, 3, M, region_slot);
       return;
     }
@@ -504,8 +538,12 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
     cin0 += sh > 0;
     cin1 += sh < 0;
     // both sides saturated and the triangle out of rank: nothing left here
-    if (cin0 >= K && cin1 >= K && total_in + 3 > smax) return;
+    if (cin0 >= K && cin1 >= K && total_in + 3 > smax) {
+      add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopFacePoint);
+      return;
+    }
   }
+  add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopFacePoint);
   // triangle event
   if (wc > 0 && total_in + 3 <= smax) {
     atomicAdd(w.cand, 1u);
@@ -541,6 +579,7 @@ __device__ void side_item(const Params &P, const Work &w, uint32_t u, uint32_t M
   const uint32_t vi = lo & 1023u, wi = (lo >> 10) & 1023u, side = (lo >> 20) & 1u, cin_side = lo >> 21;
   TriSphere f;
   f.init(gpt(w, vi), gpt(w, wi));
+  add_flops(w, kFlopTriInit);
   const Capsule cap = face_capsule(f, rmax);
   face_side_apexes(P, w, u, M, f, cap, vi, wi, w.gid[wi], (int)side, (int)cin_side, P.s_max - 3, region_slot, q);
 }
@@ -876,6 +915,7 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   __syncthreads();
   if (t == 0) {
     if (*w.cand) atomicAdd(&P.counters[3], (unsigned long long)*w.cand);
+    if (*w.flops) atomicAdd(&P.counters[5], *w.flops);
     P.status[u] = ST_DONE;
   }
 }
@@ -902,9 +942,10 @@ __device__ void anchor_lds(const Params &P) {
     return;
   }
   __shared__ uint32_t cand;
-  if (threadIdx.x == 0) cand = 0;
+  __shared__ unsigned long long flops;
+  if (threadIdx.x == 0) { cand = 0; flops = 0; }
   __syncthreads();
-  Work w{gx, gy, gz, gid, rho, ridx, rmask, kidx, reinterpret_cast<float4 *>(rmask), &cand};
+  Work w{gx, gy, gz, gid, rho, ridx, rmask, kidx, reinterpret_cast<float4 *>(rmask), &cand, &flops};
   anchor_work<false>(P, w, u);
 }
 
@@ -937,9 +978,11 @@ __global__ void __launch_bounds__(kBlock) k_anchor_big(Params P) {
   w.kidx = w.ridx + G;
   w.fc = reinterpret_cast<float4 *>(slab + big_fc_offset(G));
   __shared__ uint32_t cand;
-  if (threadIdx.x == 0) cand = 0;
+  __shared__ unsigned long long flops;
+  if (threadIdx.x == 0) { cand = 0; flops = 0; }
   __syncthreads();
   w.cand = &cand;
+  w.flops = &flops;
   anchor_work<true>(P, w, u);
 }
 
@@ -1268,22 +1311,25 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     P.anchors = B.anchors.p; P.anchor_count = cnt;
     P.big = 0; P.gcap = kGatherCap; P.rcap = kRegionCap;
     tm.mark("catalogue gather");
-    uint64_t ev_before = 0;
-    copy_wait(&ev_before, B.counters.p, 8, hipMemcpyDeviceToHost, s);
+    uint64_t ctr_before[8];
+    copy_wait(ctr_before, B.counters.p, sizeof(ctr_before), hipMemcpyDeviceToHost, s);
     c.anchor_clock.start(s);
     hipLaunchKernelGGL(k_anchor_lds, dim3(cnt), dim3(kBlock), 0, s, P);
     c.anchor_clock.stop(s);
     MHGP_LAUNCH_CHECK();
     {
-      uint64_t ev_after = 0;
-      copy_wait(&ev_after, B.counters.p, 8, hipMemcpyDeviceToHost, s);
+      uint64_t ctr_after[8];
+      copy_wait(ctr_after, B.counters.p, sizeof(ctr_after), hipMemcpyDeviceToHost, s);
+      const uint64_t ev = ctr_after[0] - ctr_before[0];
       c.stats.anchor_ms += c.anchor_clock.ms();
       c.stats.anchor_launches++;
       c.stats.gathered_points += (int64_t)tot;
-      c.stats.events += (int64_t)(ev_after - ev_before);
+      c.stats.events += (int64_t)ev;
       // algorithmic bytes: every gathered neighbour read once (id + FP64 xyz),
       // every record written once
-      c.stats.anchor_bytes += (int64_t)tot * 28 + (int64_t)(ev_after - ev_before) * (int64_t)sizeof(EventRec);
+      c.stats.anchor_bytes += (int64_t)tot * 28 + (int64_t)ev * (int64_t)sizeof(EventRec);
+      // FP64 operations of the certified predicates the launch evaluated
+      c.stats.anchor_flops += (int64_t)(ctr_after[5] - ctr_before[5]);
     }
     tm.mark("catalogue anchors");
     std::vector<uint32_t> st(n);

@@ -45,7 +45,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("anchor_ms", ctypes.c_double), ("anchor_launches", ctypes.c_int64),
                 ("anchor_bytes", ctypes.c_int64), ("arm_ms", ctypes.c_double),
                 ("arm_launches", ctypes.c_int64), ("arm_bytes", ctypes.c_int64),
-                ("gathered_points", ctypes.c_int64), ("events", ctypes.c_int64)]
+                ("gathered_points", ctypes.c_int64), ("events", ctypes.c_int64),
+                ("anchor_flops", ctypes.c_int64)]
 
 
 class RequestResult(ctypes.Structure):
@@ -77,6 +78,7 @@ def lib() -> ctypes.CDLL:
         L.mhgp_emst.argtypes = [p, p]
         L.mhgp_supports.argtypes = [p, i32, ctypes.POINTER(CatalogInfo)]
         L.mhgp_supports_fetch.argtypes = [p, p, p, p, p, p]
+        L.mhgp_supports_radii.argtypes = [p, p]
         L.mhgp_tower.argtypes = [p, i32, ctypes.POINTER(TowerInfo)]
         L.mhgp_tower_fetch.argtypes = [p, p, p, p, p, p, p]
         L.mhgp_ctx_stats.argtypes = [p, ctypes.POINTER(Stats), i32]
@@ -96,7 +98,7 @@ EXPORTED = [
     "mhgp_last_error", "mhgp_ctx_create", "mhgp_ctx_destroy", "mhgp_ctx_device_bytes",
     "mhgp_cloud_upload", "mhgp_cloud_upload_device", "mhgp_point_count",
     "mhgp_canonical_points", "mhgp_lbvh_snapshot", "mhgp_knn", "mhgp_emst", "mhgp_supports",
-    "mhgp_supports_fetch", "mhgp_tower", "mhgp_tower_fetch", "mhgp_ctx_stats",
+    "mhgp_supports_fetch", "mhgp_supports_radii", "mhgp_tower", "mhgp_tower_fetch", "mhgp_ctx_stats",
     "mhgp_closed_ball", "mhgp_server_create", "mhgp_server_submit", "mhgp_server_wait", "mhgp_server_fetch",
     "mhgp_server_release", "mhgp_server_destroy",
 ]
@@ -228,6 +230,27 @@ class Context:
         deg = (meta >> 24) & 0xFF
         inter = [tuple(int(x) for x in interior[i, : nint[i]]) for i in range(e)]
         return Catalog(ids, size, rank, inter, deg.astype(bool), level, cls, info)
+
+    def supports_raw(self, k_max: int):
+        """The catalogue as flat arrays (no per-record Python objects): ids
+        (E, 4), meta (E,), interior (E, s_max - 1), level (E,), level_class
+        (E,) and the info block."""
+        info = CatalogInfo()
+        _check(lib().mhgp_supports(self._h, k_max, ctypes.byref(info)))
+        e = info.event_count + info.diagnostic_count
+        smax = max(info.s_max, 2)
+        out = {"ids": np.zeros((e, 4), dtype=np.uint32), "meta": np.zeros(e, dtype=np.uint32),
+               "interior": np.zeros((e, smax - 1), dtype=np.uint32), "level": np.zeros(e, dtype=np.float64),
+               "level_class": np.zeros(e, dtype=np.int64), "info": info}
+        _check(lib().mhgp_supports_fetch(self._h, _ptr(out["ids"]), _ptr(out["meta"]), _ptr(out["interior"]),
+                                         _ptr(out["level"]), _ptr(out["level_class"])))
+        return out
+
+    def radii(self) -> np.ndarray:
+        """Per-PointId enumeration radius of the last catalogue."""
+        out = np.zeros(max(self.n, 1), dtype=np.float64)
+        _check(lib().mhgp_supports_radii(self._h, _ptr(out)))
+        return out[: self.n]
 
     def tower(self, k_max: int, fetch: bool = True):
         info = TowerInfo()

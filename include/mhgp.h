@@ -132,6 +132,16 @@ int mhgp_supports(mhgp_ctx *ctx, int k_max, mhgp_catalog_info *info);
 int mhgp_supports_fetch(mhgp_ctx *ctx, uint32_t *ids, uint32_t *meta, uint32_t *interior,
                         double *level, int64_t *level_class);
 
+/* Enumeration radius of every PointId u in the last catalogue (n doubles,
+ * input unit): every point of every critical sphere whose least PointId is u
+ * lies within radius[u] of u.  The catalogue proves this per anchor with its
+ * direction-cell certificate (an upper bound on the radius of any critical
+ * sphere through u, doubled) before it enumerates; an exhaustive search among
+ * the points within radius[u] of u therefore finds every record of u.  No
+ * reference counterpart: the reference's streams bound their frontier by
+ * node-box prunes instead (pair_support_stream.cpp:446). */
+int mhgp_supports_radii(mhgp_ctx *ctx, double *radius_out);
+
 /* hgp_reduced merge-forest tower for orders 1..k_max from the last catalogue
  * (reference build_exact_direct_morse_forest_journal,
  * morsehgp3d/src/cpu/hierarchy/direct_morse_forest_journal.cpp:1042). */
@@ -164,6 +174,8 @@ typedef struct {
   int64_t arm_bytes;         /* algorithmic bytes: saddle records read + carriers written */
   int64_t gathered_points;   /* neighbour entries gathered for the anchors */
   int64_t events;            /* catalogue records written by the device */
+  int64_t anchor_flops;      /* FP64 operations of the certified predicates the anchor kernel evaluated
+                                (per evaluation: the value, its magnitude bound, the certified sign) */
 } mhgp_stats;
 
 int mhgp_ctx_stats(mhgp_ctx *ctx, mhgp_stats *stats, int reset);

@@ -201,3 +201,70 @@ def support_info(ids) -> tuple:
 def support_side(ids, x: int) -> int:
     arr = np.ascontiguousarray(np.asarray(ids, dtype=np.int64))
     return int(lib().or_support_side(arr.ctypes.data, len(arr), int(x)))
+
+
+def anchor_records(canonical_points, k_max: int, anchor: int, radius: float) -> list:
+    """Exhaustive exact records whose least PointId is `anchor` among the
+    points within `radius` of it (complete for circumradii <= radius / 2).
+    Reentrant in the C library: threads may call it concurrently once the
+    cloud is prepared (the call releases the GIL)."""
+    L = lib()
+    if not hasattr(L.or_anchor_records, "_typed"):
+        L.or_anchor_prepare.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+        p = ctypes.c_void_p
+        L.or_anchor_records.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_int64,
+                                        p, p, p, p, p, p, p, p, p, ctypes.c_int]
+        L.or_anchor_records.restype = ctypes.c_int64
+        L.or_anchor_records._typed = True
+    a = _pts(canonical_points)
+    key = (a.ctypes.data, len(a))
+    if _anchor_cloud["key"] != key:
+        _check(L.or_anchor_prepare(a.ctypes.data, len(a)), "anchor_prepare")
+        _anchor_cloud["key"], _anchor_cloud["arr"] = key, a
+    cap, strcap = 2048, 160
+    while True:
+        m = np.zeros(cap, np.int32); sup = np.zeros((cap, 4), np.int64); rank = np.zeros(cap, np.int32)
+        nint = np.zeros(cap, np.int32); inter = np.zeros((cap, 10), np.int64); deg = np.zeros(cap, np.int32)
+        shell = np.zeros(cap, np.int32)
+        num = ctypes.create_string_buffer(cap * strcap); den = ctypes.create_string_buffer(cap * strcap)
+        cnt = L.or_anchor_records(int(k_max), int(anchor), float(radius), cap, m.ctypes.data, sup.ctypes.data,
+                                  rank.ctypes.data, nint.ctypes.data, inter.ctypes.data, deg.ctypes.data,
+                                  shell.ctypes.data, num, den, strcap)
+        if cnt < 0:
+            raise OracleError(f"anchor records failed with code {cnt}")
+        if cnt <= cap:
+            break
+        cap = int(cnt) + 16
+    out = []
+    for i in range(cnt):
+        k = int(m[i])
+        nu = num.raw[i * strcap:(i + 1) * strcap].split(b"\0", 1)[0]
+        de = den.raw[i * strcap:(i + 1) * strcap].split(b"\0", 1)[0]
+        out.append(CatalogRecord(
+            support=tuple(int(x) for x in sup[i, :k]),
+            interior=tuple(int(x) for x in inter[i, : int(nint[i])]),
+            rank=int(rank[i]),
+            level=Fraction(int(nu), int(de)),
+            degenerate=bool(deg[i]),
+            shell_count=int(shell[i]),
+            shell_witness=-1,
+        ))
+    return out
+
+
+def emst_graph(canonical_points, cand_edges) -> np.ndarray:
+    """Exact minimum spanning forest under kappa = (d^2, u, v) of the given
+    candidate edges (rows (u, v)), in kappa order."""
+    L = lib()
+    if not hasattr(L.or_emst_graph, "_typed"):
+        L.or_emst_graph.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                                    ctypes.c_void_p]
+        L.or_emst_graph.restype = ctypes.c_int64
+        L.or_emst_graph._typed = True
+    a = _pts(canonical_points)
+    c = np.ascontiguousarray(np.asarray(cand_edges, dtype=np.int64).reshape(-1, 2))
+    out = np.zeros((max(len(a) - 1, 1), 2), dtype=np.int64)
+    k = L.or_emst_graph(a.ctypes.data, len(a), c.ctypes.data, len(c), out.ctypes.data)
+    if k < 0:
+        raise OracleError(f"emst_graph failed with code {k}")
+    return out[:k]

@@ -878,3 +878,268 @@ int64_t or_anchor_catalog(int k_max, int64_t anchor, double radius) {
   free(nb);
   return events;
 }
+
+/* ------------------------------------------------------------------------ */
+/* anchored exhaustive records (parity of the device catalogue at scale)     */
+/* Every support U of 2..4 points whose least PointId is `anchor`, all of    */
+/* whose points lie within `radius` of it, classified exactly against every  */
+/* point within `radius` (a sphere through the anchor of radius r lies in    */
+/* the ball of radius 2r around it, so the classification of supports with  */
+/* r <= radius / 2 is complete).  Same record definition as                  */
+/* or_catalog_build: well-centred, closed rank <= s_max, extra shell points  */
+/* flag the record as degenerate.  FP64 filters only skip work they can      */
+/* certify is not a record (well-conditioned cases, margins far above their  */
+/* rounding); everything else is decided exactly.  Reentrant: several        */
+/* threads may run it at once on the cloud of or_anchor_prepare.             */
+/* ------------------------------------------------------------------------ */
+
+#define ANC_INTERIOR 10 /* s_max - 2 for s_max <= 11 */
+
+typedef struct {
+  const ecloud *cl;
+  int64_t anchor, smax;
+  const int64_t *nb;  /* every point within radius except the anchor */
+  int64_t nnb;
+  const double *rel;  /* offsets of nb from the anchor, 3 per point */
+  double rmax2;
+  sup_t *S;
+  int64_t count, cap;
+  int32_t *m_out, *rank_out, *nint_out, *deg_out, *shell_out;
+  int64_t *sup_out, *int_out;
+  char *num_out, *den_out;
+  int strcap;
+  int err;
+} anc_ctx;
+
+/* FP64 circumsphere of the anchor (origin) and 1..3 offsets: centre offset and
+ * squared radius; returns 1 when the support is well conditioned (the values
+ * are then accurate to far better than 1e-9 relative), 0 otherwise. */
+static int anc_fp_sphere(const double *a, const double *b, const double *d, int m, double c[3], double *r2,
+                         int *clearly_not_well) {
+  *clearly_not_well = 0;
+  if (m == 2) {
+    for (int k = 0; k < 3; ++k) c[k] = 0.5 * a[k];
+    *r2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+    return 1;
+  }
+  double aa = a[0] * a[0] + a[1] * a[1] + a[2] * a[2], bb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+  double ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+  if (m == 3) {
+    double D = aa * bb - ab * ab;
+    if (!(D > 1e-6 * aa * bb)) return 0;
+    double al = bb * (aa - ab), be = aa * (bb - ab), ga = 2 * D - al - be;
+    for (int k = 0; k < 3; ++k) c[k] = (al * a[k] + be * b[k]) / (2 * D);
+    *r2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+    double tol = 1e-9 * aa * bb;
+    *clearly_not_well = al < -tol || be < -tol || ga < -tol;
+    return 1;
+  }
+  double bxd[3] = {b[1] * d[2] - b[2] * d[1], b[2] * d[0] - b[0] * d[2], b[0] * d[1] - b[1] * d[0]};
+  double dxa[3] = {d[1] * a[2] - d[2] * a[1], d[2] * a[0] - d[0] * a[2], d[0] * a[1] - d[1] * a[0]};
+  double axb[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+  double V = a[0] * bxd[0] + a[1] * bxd[1] + a[2] * bxd[2];
+  double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  double scale = sqrt(aa * bb * dd);
+  if (!(fabs(V) > 1e-6 * scale)) return 0;
+  double N[3];
+  for (int k = 0; k < 3; ++k) N[k] = aa * bxd[k] + bb * dxa[k] + dd * axb[k];
+  for (int k = 0; k < 3; ++k) c[k] = N[k] / (2 * V);
+  *r2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+  /* barycentric coordinates of the centre times 2 V^2 */
+  double nv = N[0] * bxd[0] + N[1] * bxd[1] + N[2] * bxd[2];
+  double nw = N[0] * dxa[0] + N[1] * dxa[1] + N[2] * dxa[2];
+  double nz = N[0] * axb[0] + N[1] * axb[1] + N[2] * axb[2];
+  double n0 = 2 * V * V - nv - nw - nz;
+  double tol = 1e-9 * scale * scale;
+  *clearly_not_well = nv < -tol || nw < -tol || nz < -tol || n0 < -tol;
+  return 1;
+}
+
+/* FP64 count of the points certainly strictly inside a well-conditioned sphere;
+ * stops past `limit`. */
+static int64_t anc_fp_inside(const anc_ctx *A, const double c[3], double r2, int64_t limit) {
+  int64_t cnt = 0;
+  const double lim2 = r2 * (1.0 - 1e-7);
+  for (int64_t t = 0; t < A->nnb; ++t) {
+    const double *p = A->rel + 3 * t;
+    double x = p[0] - c[0], y = p[1] - c[1], z = p[2] - c[2];
+    if (x * x + y * y + z * z < lim2 && ++cnt > limit) break;
+  }
+  return cnt;
+}
+
+static void anc_exact(anc_ctx *A, const int64_t *ids, int m) {
+  if (A->err) return;
+  sup_t *S = A->S;
+  if (!support_analyze(S, A->cl, ids, m) || !S->well) return;
+  int64_t nint = 0, nshell = 0, inter[ANC_INTERIOR];
+  for (int64_t t = 0; t < A->nnb; ++t) {
+    int64_t x = A->nb[t];
+    int on = 0;
+    for (int q = 1; q < m; ++q) on |= ids[q] == x;
+    if (on) continue;
+    int sd = support_side(S, A->cl, x);
+    if (sd < 0) {
+      if (nint < ANC_INTERIOR) inter[nint] = x;
+      if (++nint + m > A->smax) return;
+    } else if (sd == 0) {
+      nshell++;
+    }
+  }
+  int64_t i = A->count++;
+  if (i >= A->cap) return;
+  A->m_out[i] = m;
+  for (int q = 0; q < 4; ++q) A->sup_out[4 * i + q] = q < m ? ids[q] : -1;
+  A->rank_out[i] = (int32_t)(nint + m);
+  A->nint_out[i] = (int32_t)nint;
+  /* interior ascending */
+  for (int64_t a = 1; a < nint; ++a)
+    for (int64_t b = a; b > 0 && inter[b - 1] > inter[b]; --b) { int64_t t = inter[b]; inter[b] = inter[b - 1]; inter[b - 1] = t; }
+  for (int q = 0; q < ANC_INTERIOR; ++q) A->int_out[ANC_INTERIOR * i + q] = q < nint ? inter[q] : -1;
+  A->deg_out[i] = nshell > 0;
+  A->shell_out[i] = (int32_t)(m + nshell);
+  bi nn, dd, g;
+  bi_copy(&nn, &S->lnum);
+  bi_copy(&dd, &S->lden);
+  int sh = 2 * A->cl->e0;
+  if (sh > 0) bi_shl(&nn, &nn, sh); else if (sh < 0) bi_shl(&dd, &dd, -sh);
+  if (nn.s == 0) bi_set_i64(&dd, 1);
+  else { bi_gcd(&g, &nn, &dd); bi_divmod(&nn, NULL, &nn, &g); bi_divmod(&dd, NULL, &dd, &g); }
+  if (bi_to_dec(&nn, A->num_out + (size_t)i * A->strcap, A->strcap) < 0 ||
+      bi_to_dec(&dd, A->den_out + (size_t)i * A->strcap, A->strcap) < 0)
+    A->err = OR_CAPACITY;
+}
+
+/* Returns the number of records (written up to cap; call again with a larger
+ * cap when it exceeds it) or a negative error.  Per record: support size,
+ * 4 ids (-1 padded), rank, interior count, 10 interior ids (-1 padded),
+ * degenerate flag, shell count, and the squared radius as reduced decimal
+ * num / den strings of strcap bytes each. */
+int64_t or_anchor_records(int k_max, int64_t anchor, double radius, int64_t cap, int32_t *m_out,
+                          int64_t *sup_out, int32_t *rank_out, int32_t *nint_out, int64_t *int_out,
+                          int32_t *deg_out, int32_t *shell_out, char *num_out, char *den_out, int strcap) {
+  if (!g_anc_valid) return OR_ARGUMENT;
+  const ecloud *cl = &g_anc_cl;
+  const double *X = cl->X;
+  const int64_t n = cl->n;
+  if (anchor < 0 || anchor >= n || !(radius > 0)) return OR_ARGUMENT;
+  anc_ctx A;
+  memset(&A, 0, sizeof(A));
+  A.cl = cl;
+  A.anchor = anchor;
+  int64_t keff = k_max < n ? k_max : n;
+  A.smax = keff + 1 < n ? keff + 1 : n;
+  if (A.smax > 11) return OR_ARGUMENT;
+  int64_t *nb = malloc(sizeof(int64_t) * (size_t)n);
+  double *rel = malloc(sizeof(double) * 3 * (size_t)n);
+  int64_t *cand = malloc(sizeof(int64_t) * (size_t)n);
+  int64_t *cidx = malloc(sizeof(int64_t) * (size_t)n);
+  A.S = malloc(sizeof(sup_t));
+  if (!nb || !rel || !cand || !cidx || !A.S) { free(nb); free(rel); free(cand); free(cidx); free(A.S); return OR_CAPACITY; }
+  const double R2 = radius * radius * (1.0 + 1e-9);
+  int64_t m = 0, mc = 0;
+  for (int64_t j = 0; j < n; ++j) {
+    if (j == anchor) continue;
+    double d[3], s = 0;
+    for (int k = 0; k < 3; ++k) { d[k] = X[3 * j + k] - X[3 * anchor + k]; s += d[k] * d[k]; }
+    if (s > R2) continue;
+    nb[m] = j;
+    for (int k = 0; k < 3; ++k) rel[3 * m + k] = d[k];
+    if (j > anchor) { cand[mc] = j; cidx[mc] = m; mc++; }
+    m++;
+  }
+  A.nb = nb; A.nnb = m; A.rel = rel;
+  A.rmax2 = 0.25 * radius * radius * (1.0 + 1e-6);
+  A.cap = cap;
+  A.m_out = m_out; A.sup_out = sup_out; A.rank_out = rank_out; A.nint_out = nint_out; A.int_out = int_out;
+  A.deg_out = deg_out; A.shell_out = shell_out; A.num_out = num_out; A.den_out = den_out; A.strcap = strcap;
+  const double pair2 = radius * radius * (1.0 + 1e-6);
+  int64_t ids[4];
+  ids[0] = anchor;
+  for (int64_t i = 0; i < mc && A.smax >= 2; ++i) {
+    const double *a = rel + 3 * cidx[i];
+    double c[3], r2;
+    int notwell;
+    anc_fp_sphere(a, NULL, NULL, 2, c, &r2, &notwell);
+    if (r2 > A.rmax2) continue;
+    if (anc_fp_inside(&A, c, r2, A.smax - 2) > A.smax - 2) continue;
+    ids[1] = cand[i];
+    anc_exact(&A, ids, 2);
+  }
+  for (int64_t i = 0; i < mc && A.smax >= 3; ++i) {
+    const double *a = rel + 3 * cidx[i];
+    for (int64_t j = i + 1; j < mc; ++j) {
+      const double *b = rel + 3 * cidx[j];
+      double ab2 = 0;
+      for (int k = 0; k < 3; ++k) ab2 += (a[k] - b[k]) * (a[k] - b[k]);
+      if (ab2 > pair2) continue;  /* two points of a sphere of radius <= radius/2 */
+      double c[3], r2;
+      int notwell;
+      int good = anc_fp_sphere(a, b, NULL, 3, c, &r2, &notwell);
+      /* a tetrahedron's circumradius is at least that of each face */
+      if (good && r2 > A.rmax2) continue;
+      ids[1] = cand[i];
+      ids[2] = cand[j];
+      if (!(good && (notwell || anc_fp_inside(&A, c, r2, A.smax - 3) > A.smax - 3))) anc_exact(&A, ids, 3);
+      if (A.smax < 4) continue;
+      for (int64_t l = j + 1; l < mc; ++l) {
+        const double *d = rel + 3 * cidx[l];
+        double ad2 = 0, bd2 = 0;
+        for (int k = 0; k < 3; ++k) { ad2 += (a[k] - d[k]) * (a[k] - d[k]); bd2 += (b[k] - d[k]) * (b[k] - d[k]); }
+        if (ad2 > pair2 || bd2 > pair2) continue;
+        double c4[3], r24;
+        int notwell4;
+        int good4 = anc_fp_sphere(a, b, d, 4, c4, &r24, &notwell4);
+        if (good4 && (r24 > A.rmax2 || notwell4)) continue;
+        if (good4 && anc_fp_inside(&A, c4, r24, A.smax - 4) > A.smax - 4) continue;
+        ids[3] = cand[l];
+        anc_exact(&A, ids, 4);
+      }
+    }
+  }
+  free(nb); free(rel); free(cand); free(cidx); free(A.S);
+  if (A.err) return A.err;
+  return A.count;
+}
+
+/* ------------------------------------------------------------------------ */
+/* exact minimum spanning forest of a given edge set under kappa=(d^2,u,v)   */
+/* (Kruskal as in or_emst, over candidate edges instead of every pair; the   */
+/* EMST when the candidates contain it, e.g. a k-nearest-neighbour graph)    */
+/* ------------------------------------------------------------------------ */
+
+/* cand: 2m ids (any order inside a pair, duplicates allowed); forest edges
+ * (u < v) are written in kappa order; returns their number or an error. */
+int64_t or_emst_graph(const double *xyz, int64_t n, const int64_t *cand, int64_t m, int64_t *edges) {
+  if (n < 1 || m < 0) return OR_ARGUMENT;
+  ecloud cl;
+  int rc = ecloud_init(&cl, xyz, n);
+  if (rc) return rc;
+  fedge *E = malloc(sizeof(fedge) * (size_t)(m ? m : 1));
+  int64_t *par = malloc(sizeof(int64_t) * (size_t)n);
+  if (!E || !par) { free(E); free(par); ecloud_free(&cl); return OR_CAPACITY; }
+  int64_t t = 0;
+  for (int64_t i = 0; i < m; ++i) {
+    int64_t u = cand[2 * i], v = cand[2 * i + 1];
+    if (u == v || u < 0 || v < 0 || u >= n || v >= n) continue;
+    if (u > v) { int64_t s = u; u = v; v = s; }
+    E[t].f = fdist2(xyz, u, v); E[t].u = u; E[t].v = v; t++;
+  }
+  g_emst_cl = &cl;
+  qsort(E, (size_t)t, sizeof(fedge), edge_cmp);
+  for (int64_t i = 0; i < n; ++i) par[i] = i;
+  int64_t k = 0;
+  for (int64_t i = 0; i < t && k < n - 1; ++i) {
+    if (i > 0 && E[i].u == E[i - 1].u && E[i].v == E[i - 1].v) continue;
+    int64_t a = uf_find(par, E[i].u), b = uf_find(par, E[i].v);
+    if (a == b) continue;
+    par[a] = b;
+    edges[2 * k] = E[i].u;
+    edges[2 * k + 1] = E[i].v;
+    k++;
+  }
+  free(par);
+  free(E);
+  ecloud_free(&cl);
+  return k;
+}

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import oracle
+from helpers import workloads
 from oracle import tower
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -162,3 +163,60 @@ def test_morton_quantisation_closed_form():
     for pid, p in enumerate(pts):
         q = [min(int(p[a] / [1, 2, 4][a] * 2**21), 2**21 - 1) for a in range(3)]
         assert int(snap.codes_by_id[pid]) == inter(q)
+
+
+@pytest.mark.parametrize("seed,n", [(3, 40), (4, 56)])
+def test_anchor_records_equal_exhaustive_catalogue(seed, n):
+    """The anchored enumeration (used at scale) against the exhaustive
+    catalogue: every record of every anchor, with a radius covering the cloud,
+    at K = 3 and K = 10; with a small radius it can only lose records whose
+    circumradius exceeds half of it, never invent or misclassify one."""
+    pts = oracle.canonicalize(workloads.uniform_cloud(n, seed=seed))
+    for k in (3, 10):
+        ref = [r for r in oracle.catalog(pts, k) if len(r.support) >= 2]
+        full = {(r.support, r.interior, r.rank, r.degenerate, r.level) for r in ref}
+        got = set()
+        for a in range(n):
+            got |= {(r.support, r.interior, r.rank, r.degenerate, r.level)
+                    for r in oracle.anchor_records(pts, k, a, 4.0)}
+        assert got == full
+        part = set()
+        for a in range(n):
+            part |= {(r.support, r.interior, r.rank, r.degenerate, r.level)
+                     for r in oracle.anchor_records(pts, k, a, 0.5)}
+        assert part <= full
+        assert part == {r for r in full if r[4] <= Fraction(1, 16)}
+
+
+def test_anchor_records_planted_mistake_is_caught():
+    """A planted wrong interior (one record's interior id changed) is seen by
+    the comparison the scale-parity test makes."""
+    pts = oracle.canonicalize(workloads.uniform_cloud(30, seed=8))
+    recs = oracle.anchor_records(pts, 10, 0, 4.0)
+    ref = [r for r in oracle.catalog(pts, 10) if len(r.support) >= 2 and r.support[0] == 0]
+    key = lambda r: (r.support, r.interior, r.rank, r.degenerate)
+    assert {key(r) for r in recs} == {key(r) for r in ref}
+    bad = [r for r in recs if r.interior]
+    assert bad, "need a record with an interior"
+    planted = {key(r) for r in recs if r is not bad[0]} | {(bad[0].support, bad[0].interior[:-1] + (29,),
+                                                             bad[0].rank, bad[0].degenerate)}
+    assert planted != {key(r) for r in ref}
+
+
+def test_emst_graph_equals_complete_graph_kruskal():
+    from scipy.spatial import cKDTree
+
+    for n, seed in ((40, 1), (300, 2)):
+        pts = oracle.canonicalize(workloads.uniform_cloud(n, seed=seed))
+        full = oracle.emst(pts)
+        u, v = np.triu_indices(n, 1)
+        assert np.array_equal(oracle.emst_graph(pts, np.stack([u, v], 1)), full)
+        _, nb = cKDTree(pts).query(pts, k=min(n, 17))
+        cand = np.stack([np.repeat(np.arange(n), nb.shape[1] - 1), nb[:, 1:].ravel()], 1)
+        assert np.array_equal(oracle.emst_graph(pts, cand), full)
+    # a graph without the tree's edges gives a different (heavier) forest
+    pts = oracle.canonicalize(workloads.uniform_cloud(60, seed=3))
+    full = oracle.emst(pts)
+    u, v = np.triu_indices(60, 1)
+    keep = ~((u == full[0, 0]) & (v == full[0, 1]))
+    assert not np.array_equal(oracle.emst_graph(pts, np.stack([u[keep], v[keep]], 1)), full)
