@@ -157,25 +157,57 @@ def copy_calibration(mib: int) -> None:
     print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4, "launches": 3}))
 
 
-def served_stream(k_values=(4, 10), count=48, rate_mult=4.0, slots=4, seed=7):
-    """Poisson stream of clouds (sizes log-uniform in [2 000, 50 000]) through
-    the continuous-batching server, against the same clouds run one after
-    another on one context (the plain path).  The offered rate is rate_mult
-    times the plain path's rate; served clouds/s is the number of clouds over
-    the span from the first arrival to the last completion."""
+def plain_request(ctx, pts, k_max):
+    """One request on the plain path, as the server runs it: catalogue, then
+    the tower unless the catalogue is conditional; returns (code, status)."""
+    try:
+        ctx.upload(pts)
+        ctx.emst()
+        info = ctx.supports(k_max, fetch=False)
+        if info.status != 0:
+            return 0, 1
+        ctx.tower(k_max, fetch=True)
+        return 0, 0
+    except mhgp.MhgpError as e:
+        return e.code, 0
+
+
+def c5_cloud(size: int, seed: int):
+    """The served-stream generators: a jittered lattice, a 5-20 blob
+    anisotropic mixture with 15 % uniform noise, or the same snapped to the
+    2^-12 grid (chosen by seed)."""
+    rng = workloads.SplitMix64(seed).numpy_rng()
+    kind = int(rng.integers(3))
+    if kind == 0:
+        side = int(np.ceil((size * 1.3) ** (1 / 3)))
+        return workloads.jittered_lattice(size, side=side, sigma=0.1, seed=seed)
+    return workloads.gaussian_mixture(size, blobs=int(rng.integers(5, 21)), noise_fraction=0.15, box=10.0,
+                                      snap_bits=12 if kind == 2 else None, seed=seed)
+
+
+def served_stream(k_values=(4, 10), count=32, lo=2_000, hi=200_000, rate_mult=4.0, slots=4, seed=7):
+    """Poisson stream of clouds (sizes log-uniform in [lo, hi], K_max drawn
+    from k_values, lattice / mixture / snapped-mixture generators) through the
+    continuous-batching server, against the same clouds run one after another
+    on one context (the plain path).  The offered rate is rate_mult times the
+    plain path's rate; served clouds/s is the number of clouds over the span
+    from the first arrival to the last completion.  Results per request:
+    exact, conditional (catalogue budget stop, no tower) or refused tower
+    (extra-shell catalogue)."""
     rng = np.random.default_rng(seed)
-    sizes = np.exp(rng.uniform(np.log(2_000), np.log(50_000), size=count)).astype(int)
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), size=count)).astype(int)
     ks = rng.choice(np.asarray(k_values), size=count)
-    clouds = [workloads.jittered_lattice(int(m), side=int(np.ceil((m * 1.3) ** (1 / 3))), sigma=0.1,
-                                         seed=1000 + i) for i, m in enumerate(sizes)]
+    clouds = [c5_cloud(int(m), 1000 + i) for i, m in enumerate(sizes)]
     ctx = mhgp.Context(0)
-    run_step(ctx, clouds[0], int(ks[0]))  # warm
+    plain_request(ctx, clouds[0], int(ks[0]))  # warm
     t0 = time.perf_counter()
-    plain_lat = []
-    for pts, k in zip(clouds, ks):
+    plain_lat, plain_out = [], []
+    for i, (pts, k) in enumerate(zip(clouds, ks)):
         a = time.perf_counter()
-        run_step(ctx, pts, int(k))
+        plain_out.append(plain_request(ctx, pts, int(k)))
         plain_lat.append(1e3 * (time.perf_counter() - a))
+        if (i + 1) % 8 == 0:
+            print(f"served, plain path: {i + 1}/{count} done", file=sys.stderr, flush=True)
     plain_s = time.perf_counter() - t0
     ctx.close()
     plain_rate = count / plain_s
@@ -192,22 +224,29 @@ def served_stream(k_values=(4, 10), count=48, rate_mult=4.0, slots=4, seed=7):
         while time.perf_counter() < due:
             time.sleep(min(0.001, max(0.0, due - time.perf_counter())))
         tickets.append(srv.submit(pts, int(k)))
-    lat = []
-    for t in tickets:
+    lat, outcome = [], {}
+    for i, t in enumerate(tickets):
         r = srv.wait(t)
-        if r.code != 0:
-            raise mhgp.MhgpError(r.code, "served request failed")
+        key = "exact" if r.code == 0 and r.status == 0 else "conditional" if r.code == 0 else f"code {r.code}"
+        outcome[key] = outcome.get(key, 0) + 1
+        if (r.code, r.status) != plain_out[i]:
+            raise mhgp.MhgpError(r.code, "a served request ended unlike the same request on the plain path")
         lat.append(r.latency_ms)
         srv.release(t)
     served_s = time.perf_counter() - t0
     srv.close()
     return {
-        "clouds": count, "slots": slots, "offered_rate_hz": round(rate, 2),
+        "clouds": count, "sizes": [lo, hi], "points": int(sizes.sum()), "slots": slots,
+        "offered_rate_hz": round(rate, 2),
         "served_clouds_per_s": round(count / served_s, 3),
         "plain_clouds_per_s": round(plain_rate, 3),
+        "served_vs_plain": round(plain_s / served_s, 3),
         "served_p50_ms": round(float(np.percentile(lat, 50)), 2),
+        "served_p95_ms": round(float(np.percentile(lat, 95)), 2),
         "served_p99_ms": round(float(np.percentile(lat, 99)), 2),
+        "plain_p50_ms": round(float(np.percentile(plain_lat, 50)), 2),
         "plain_p99_ms": round(float(np.percentile(plain_lat, 99)), 2),
+        "outcomes": outcome,
     }
 
 
@@ -221,10 +260,7 @@ def batched(count=256, slots=4, plain_subset=32):
     ctx = mhgp.Context(0)
     t0 = time.perf_counter()
     for i, pts in enumerate(clouds[:plain_subset]):
-        try:
-            run_step(ctx, pts, 10)
-        except mhgp.MhgpError:
-            pass
+        plain_request(ctx, pts, 10)
         if (i + 1) % 8 == 0:
             print(f"batched, plain path: {i + 1}/{plain_subset} done", file=sys.stderr, flush=True)
     plain_s = time.perf_counter() - t0
@@ -235,7 +271,8 @@ def batched(count=256, slots=4, plain_subset=32):
     lat = []
     for i, t in enumerate(tickets):
         r = srv.wait(t)
-        codes[int(r.code)] = codes.get(int(r.code), 0) + 1
+        key = "exact" if r.code == 0 and r.status == 0 else "conditional" if r.code == 0 else f"code {r.code}"
+        codes[key] = codes.get(key, 0) + 1
         if (i + 1) % 32 == 0:
             print(f"batched: {i + 1}/{count} done", file=sys.stderr, flush=True)
         lat.append(r.latency_ms)
@@ -246,7 +283,7 @@ def batched(count=256, slots=4, plain_subset=32):
         "clouds": count, "points": int(sizes.sum()), "slots": slots,
         "served_clouds_per_s": round(count / served_s, 3),
         "plain_clouds_per_s_first_%d" % plain_subset: round(plain_subset / plain_s, 3),
-        "result_codes": {str(k): v for k, v in sorted(codes.items())},
+        "outcomes": codes,
         "served_total_s": round(served_s, 2),
     }
 

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "../../include/mhgp.h"
+#include "../host/certify.hpp"
 #include "../host/exact.hpp"
 #include "context.hpp"
 #include "pipeline.hpp"
@@ -32,6 +33,13 @@ int guarded(mhgp::Context *c, F &&f) {
   try {
     MHGP_HIP(hipSetDevice(c->device));
     int rc = f();
+    if (rc == MHGP_OK && c->capacity_bytes > 0 && mhgp::device_bytes(*c) > c->capacity_bytes) {
+      const int64_t held = mhgp::device_bytes(*c);
+      mhgp::release_device_state(*c);
+      return fail(c, MHGP_E_CAPACITY, "the call needed " + std::to_string(held) + " device bytes, more than the " +
+                  "context's capacity of " + std::to_string(c->capacity_bytes) + "; its device buffers were released",
+                  false);
+    }
     if (rc == MHGP_OK) g_last_error.clear();
     return rc;
   } catch (const mhgp::DuplicateError &e) {
@@ -63,8 +71,9 @@ extern "C" {
 
 const char *mhgp_last_error(void) { return g_last_error.c_str(); }
 
-int mhgp_ctx_create(int device, mhgp_ctx **out) {
+int mhgp_ctx_create(int device, int64_t capacity_bytes, mhgp_ctx **out) {
   if (!out) return fail(nullptr, MHGP_E_ARGUMENT, "null output pointer", false);
+  if (capacity_bytes < 0) return fail(nullptr, MHGP_E_ARGUMENT, "negative capacity", false);
   *out = nullptr;
   try {
     int count = 0;
@@ -73,6 +82,7 @@ int mhgp_ctx_create(int device, mhgp_ctx **out) {
     MHGP_HIP(hipSetDevice(device));
     auto *h = new mhgp_ctx();
     h->c.device = device;
+    h->c.capacity_bytes = capacity_bytes;
     MHGP_HIP(hipStreamCreateWithFlags(&h->c.stream, hipStreamNonBlocking));
     *out = h;
     g_last_error.clear();
@@ -135,6 +145,31 @@ int mhgp_knn(mhgp_ctx *ctx, int k, uint32_t *ids_out) {
   });
 }
 
+int mhgp_lbvh_certify(const double *xyz, int64_t n, const uint64_t *leaves, const uint64_t *nodes,
+                      const uint64_t *counters) {
+  if (!xyz || !leaves || !nodes || !counters || n <= 0)
+    return fail(nullptr, MHGP_E_ARGUMENT, "invalid snapshot arrays", false);
+  try {
+    const std::string why = mhgp::certify_snapshot(xyz, n, leaves, nodes, counters);
+    if (!why.empty()) return fail(nullptr, MHGP_E_ARGUMENT, "snapshot rejected: " + why, false);
+    g_last_error.clear();
+    return MHGP_OK;
+  } catch (const std::exception &e) {
+    return fail(nullptr, MHGP_E_INTERNAL, e.what(), false);
+  }
+}
+
+int mhgp_knn_partition(mhgp_ctx *ctx, const int64_t *query_num, const int64_t *query_den, int64_t queries, int k,
+                       const uint32_t *excluded, const int64_t *excluded_offsets, uint32_t shell_cap,
+                       uint32_t *choice_ids, uint32_t *strict_ids, uint32_t *strict_count, uint32_t *shell_ids,
+                       uint32_t *shell_count, uint32_t *cutoff_id, double *cutoff_sq) {
+  return guarded(ctx ? &ctx->c : nullptr, [&] {
+    mhgp::knn_partition(ctx->c, query_num, query_den, queries, k, excluded, excluded_offsets, shell_cap, choice_ids,
+                        strict_ids, strict_count, shell_ids, shell_count, cutoff_id, cutoff_sq);
+    return MHGP_OK;
+  });
+}
+
 int mhgp_emst(mhgp_ctx *ctx, uint32_t *edges_out) {
   return guarded(ctx ? &ctx->c : nullptr, [&] {
     mhgp::require_cloud(ctx->c);
@@ -176,6 +211,13 @@ int mhgp_supports_radii(mhgp_ctx *ctx, double *radius_out) {
   });
 }
 
+int mhgp_supports_diagnostics(mhgp_ctx *ctx, uint32_t *shell_count, int64_t *witness, int64_t *exterior_count) {
+  return guarded(ctx ? &ctx->c : nullptr, [&] {
+    mhgp::supports_diagnostics(ctx->c, shell_count, witness, exterior_count);
+    return MHGP_OK;
+  });
+}
+
 int mhgp_tower(mhgp_ctx *ctx, int k_max, mhgp_tower_info *info) {
   return guarded(ctx ? &ctx->c : nullptr, [&] {
     if (!info) throw std::invalid_argument("null info");
@@ -188,6 +230,21 @@ int mhgp_tower_fetch(mhgp_ctx *ctx, int32_t *order, int32_t *kind, int64_t *leve
                      double *level, int64_t *child_offset, int64_t *children) {
   return guarded(ctx ? &ctx->c : nullptr, [&] {
     mhgp::tower_fetch(ctx->c, order, kind, level_class, level, child_offset, children);
+    return MHGP_OK;
+  });
+}
+
+int mhgp_tower_cut(mhgp_ctx *ctx, int order, int64_t level_class, int closed, int64_t *roots, int64_t *root_count,
+                   int64_t *root_of_point) {
+  return guarded(ctx ? &ctx->c : nullptr, [&] {
+    mhgp::tower_cut(ctx->c, order, level_class, closed, roots, root_count, root_of_point);
+    return MHGP_OK;
+  });
+}
+
+int mhgp_tower_vertical(mhgp_ctx *ctx, int64_t *target) {
+  return guarded(ctx ? &ctx->c : nullptr, [&] {
+    mhgp::tower_vertical(ctx->c, target);
     return MHGP_OK;
   });
 }

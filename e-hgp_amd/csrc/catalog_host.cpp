@@ -32,6 +32,7 @@ struct ExactJob {
     if (!S.analyze(ex, ids, m) || !S.well) return;
     uint32_t interior[kMaxInterior];
     int nint = 0, extra = 0;
+    int64_t witness = -1;  // least extra-shell point
     for (uint32_t s = 0; s < len; ++s) {
       int64_t x = region[s];
       bool on = false;
@@ -45,6 +46,7 @@ struct ExactJob {
         if (nint + m > s_max) return;
       } else if (side == 0) {
         extra++;
+        if (witness < 0 || x < witness) witness = x;
       }
     }
     EventRec e;
@@ -53,6 +55,9 @@ struct ExactJob {
     e.meta = (uint32_t)m | ((uint32_t)(nint + m) << 8) | ((uint32_t)nint << 16) |
              ((extra ? 1u : 0u) << 24) | ((uint32_t)std::min(extra, 127) << 25);
     for (int k = 0; k < kMaxInterior; ++k) e.interior[k] = k < nint ? interior[k] : kInvalid;
+    // an extra-shell record keeps its least extra-shell point in the last
+    // interior slot, which a support of 2..4 points never fills (nint <= 9)
+    if (extra) e.interior[kMaxInterior - 1] = (uint32_t)witness;
     // in the device's frame: coordinates there are scaled by 2^scale_exp
     e.level = ratio_to_double(S.lnum, S.lden, 2 * (ex.e0() + scale_exp));
     e.level_err = 1e-12f;
@@ -120,7 +125,7 @@ void supports(Context &c, int k_max, mhgp_catalog_info *info) {
   info->event_count = B.count - diagnostics;
   info->diagnostic_count = diagnostics;
   info->s_max = smax;
-  info->status = 0;
+  // status (conditional when anchors were left open) comes from the device pass
   B.info = *info;
   B.valid = true;
 }
@@ -143,8 +148,9 @@ void supports_fetch(Context &c, uint32_t *ids, uint32_t *meta, uint32_t *interio
     if (ids)
       for (int k = 0; k < 4; ++k) ids[4 * i + k] = e.ids[k];
     if (meta) meta[i] = e.meta;
+    const int nint = (int)((e.meta >> 16) & 0xFFu);
     if (interior)
-      for (int k = 0; k < width; ++k) interior[width * i + k] = k < kMaxInterior ? e.interior[k] : kInvalid;
+      for (int k = 0; k < width; ++k) interior[width * i + k] = k < nint ? e.interior[k] : kInvalid;
     if (level) level[i] = std::ldexp(e.level, -2 * c.scale_exp);
     if (level_class) level_class[i] = cls[i];
   }
@@ -166,6 +172,45 @@ void supports_radii(Context &c, double *radius_out) {
   // back to the input unit (an exact power of two); rounded up past the ulp
   // the scaling cannot lose
   for (int64_t i = 0; i < n; ++i) radius_out[i] = std::ldexp(radius_out[i], -c.scale_exp);
+}
+
+}  // namespace mhgp
+
+namespace mhgp {
+
+void supports_diagnostics(Context &c, uint32_t *shell_count, int64_t *witness, int64_t *exterior_count) {
+  if (!c.catalog || !c.catalog->valid) throw StateError("no catalogue has been built");
+  CatalogBuffers &B = *c.catalog;
+  const int64_t n = B.count;
+  std::vector<EventRec> ev(n);
+  std::vector<uint32_t> order(n);
+  if (n) {
+    copy_wait(ev.data(), B.events.p, n * sizeof(EventRec), hipMemcpyDeviceToHost, c.stream);
+    copy_wait(order.data(), B.order.p, n * 4, hipMemcpyDeviceToHost, c.stream);
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    const EventRec &e = ev[order[i]];
+    const int64_t m = e.meta & 0xFFu, rank = (e.meta >> 8) & 0xFFu;
+    const bool deg = (e.meta >> 24) & 1u;
+    int64_t extra = deg ? (int64_t)(e.meta >> 25) : 0;
+    if (extra >= 127) {
+      // the count saturates in the record: recount the sphere exactly
+      const ExactCloud &exc = exact_cloud(c);
+      int64_t ids[4];
+      for (int k = 0; k < m; ++k) ids[k] = e.ids[k];
+      ExactSupport S;
+      if (!S.analyze(exc, ids, (int)m)) throw std::logic_error("a catalogue support is affinely dependent");
+      extra = 0;
+      for (int64_t x = 0; x < c.n; ++x) {
+        bool on = false;
+        for (int k = 0; k < m; ++k) on |= ids[k] == x;
+        if (!on && S.side(exc, x) == 0) extra++;
+      }
+    }
+    if (shell_count) shell_count[i] = (uint32_t)(m + extra);
+    if (witness) witness[i] = deg ? (int64_t)e.interior[kMaxInterior - 1] : -1;
+    if (exterior_count) exterior_count[i] = c.n - rank - extra;
+  }
 }
 
 }  // namespace mhgp

@@ -19,6 +19,7 @@ struct TowerWork;        // tower_gpu.hpp
 
 struct Context {
   int device = 0;
+  int64_t capacity_bytes = 0;  // device memory bound of the context (0: none)
   // device lanes of the tower (streams with their own workspaces); a server
   // slot uses one, since the server already overlaps whole requests
   int tower_lane_count = 2;

@@ -37,6 +37,25 @@ int64_t device_bytes(const Context &c) {
   return b;
 }
 
+void release_device_state(Context &c) {
+  MHGP_HIP(hipStreamSynchronize(c.stream));
+  c.tower_lanes.clear();
+  c.tower_work.reset();
+  c.tower.reset();
+  c.host.reset();
+  c.emst.reset();
+  c.catalog.reset();
+  for (DevBuf<double> *b : {&c.src, &c.X, &c.Y, &c.Z, &c.bounds, &c.LX, &c.LY, &c.LZ}) b->release();
+  for (DevBuf<uint64_t> *b : {&c.key0, &c.key1, &c.key2, &c.codes, &c.path}) b->release();
+  for (DevBuf<uint32_t> *b : {&c.source_index, &c.perm, &c.status, &c.bound_ids, &c.leaf_id, &c.leaf_pos,
+                              &c.parent_int, &c.parent_leaf, &c.wit, &c.counter})
+    b->release();
+  c.nodes.release();
+  c.wide.release();
+  c.n = 0;
+  c.lbvh_ready = false;
+}
+
 static void after_upload(Context &c, int64_t n) {
   c.n = n;
   c.lbvh_ready = false;

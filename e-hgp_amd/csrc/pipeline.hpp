@@ -28,6 +28,9 @@ struct UnsupportedError : std::runtime_error {
 };
 
 int64_t device_bytes(const Context &c);
+// frees every device buffer of the context (its cloud, catalogue and tower
+// are gone; the stream stays)
+void release_device_state(Context &c);
 void upload_host(Context &c, const double *xyz, int64_t n);
 void upload_device(Context &c, const double *d_xyz, int64_t n);
 void require_cloud(const Context &c);
@@ -41,6 +44,11 @@ void closed_ball(Context &c, const uint32_t *ids, const uint8_t *sizes, int64_t 
 void knn_all(Context &c, int k, uint32_t *d_out, uint8_t *d_flags);
 // k nearest neighbours to host memory, near ties settled exactly on the host
 void knn_host(Context &c, int k, uint32_t *ids_out);
+// exact top-k partitions of rational query points
+void knn_partition(Context &c, const int64_t *num, const int64_t *den, int64_t Q, int k, const uint32_t *excl,
+                   const int64_t *eoff, uint32_t shell_cap, uint32_t *choice, uint32_t *strict,
+                   uint32_t *strict_count, uint32_t *shell, uint32_t *shell_count, uint32_t *cutoff_id,
+                   double *cutoff_sq);
 
 // emst.hip
 void emst_host(Context &c, uint32_t *edges_out);
@@ -51,11 +59,16 @@ void supports_fetch(Context &c, uint32_t *ids, uint32_t *meta, uint32_t *interio
                     int64_t *level_class);
 // per-PointId enumeration radius of the last catalogue (input unit)
 void supports_radii(Context &c, double *radius_out);
+// extra-shell diagnostics of the last catalogue's records (canonical order)
+void supports_diagnostics(Context &c, uint32_t *shell_count, int64_t *witness, int64_t *exterior_count);
 
 // tower.cpp
 void tower(Context &c, int k_max, mhgp_tower_info *info);
 void tower_fetch(Context &c, int32_t *order, int32_t *kind, int64_t *level_class, double *level,
                  int64_t *child_offset, int64_t *children);
+void tower_vertical(Context &c, int64_t *target);
+void tower_cut(Context &c, int order, int64_t cls, int closed, int64_t *roots, int64_t *count,
+               int64_t *root_of_point);
 
 // exact host mirror of the canonical cloud, loaded lazily
 const class ExactCloud &exact_cloud(Context &c);

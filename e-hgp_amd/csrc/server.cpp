@@ -167,14 +167,19 @@ void Server::worker_main(int slot) {
       emst_host(c, edges.data());
       mhgp_catalog_info ci{};
       supports(c, r->k_max, &ci);
-      mhgp_tower_info ti{};
-      tower(c, r->k_max, &ti);
-      r->tower = std::move(*c.tower);  // the context's tower is rebuilt by its next request
-      c.tower->valid = false;
       r->result.event_count = ci.event_count;
-      r->result.node_count = ti.node_count;
-      r->result.child_count = ti.child_count;
-      r->result.status = ci.status | ti.status;
+      if (ci.status != 0) {
+        // budget stop in the catalogue: reported as conditional, with no tower
+        r->result.status = 1;
+      } else {
+        mhgp_tower_info ti{};
+        tower(c, r->k_max, &ti);
+        r->tower = std::move(*c.tower);  // the context's tower is rebuilt by its next request
+        c.tower->valid = false;
+        r->result.node_count = ti.node_count;
+        r->result.child_count = ti.child_count;
+        r->result.status = ti.status;
+      }
     } catch (const DuplicateError &e) {
       status = MHGP_E_DUPLICATE;
     } catch (const NonFiniteError &e) {

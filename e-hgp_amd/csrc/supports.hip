@@ -65,7 +65,9 @@ enum : uint32_t { ST_PENDING = 0, ST_DONE = 1, ST_WIDEN = 2, ST_BIG = 3, ST_HUGE
 
 // Regions of the global-memory path are enumerated face by face in
 // O(region^3) work; beyond this size one anchor would keep a workgroup busy
-// for many seconds, and the catalogue reports a capacity error instead.
+// for many seconds.  Such an anchor is not enumerated: the catalogue is
+// returned complete for every other anchor with status conditional (a
+// budget stop), and the anchor and its region size are reported.
 constexpr uint32_t kBigRegionMax = 2048;
 
 struct Params {
@@ -646,6 +648,21 @@ __device__ bool member_mask(const Params &P, const Work &w, uint32_t i, float rm
 
 constexpr int kCoverCap = 10;  // s_max - 1 <= 10
 
+// Cube-map cell (6 faces x 8 x 8) of a direction, row-major on each face.
+__device__ inline uint32_t direction_cell(double x, double y, double z) {
+  const float fx = (float)x, fy = (float)y, fz = (float)z;
+  const float ax = fabsf(fx), ay = fabsf(fy), az = fabsf(fz);
+  int face;
+  float a, b, m;
+  if (ax >= ay && ax >= az) { face = fx >= 0.f ? 0 : 1; m = ax; a = fy; b = fz; }
+  else if (ay >= az) { face = fy >= 0.f ? 2 : 3; m = ay; a = fz; b = fx; }
+  else { face = fz >= 0.f ? 4 : 5; m = az; a = fx; b = fy; }
+  const float inv = m > 0.f ? 1.0f / m : 0.f;
+  const int i = min(7, max(0, (int)((a * inv + 1.0f) * 4.0f)));
+  const int j = min(7, max(0, (int)((b * inv + 1.0f) * 4.0f)));
+  return (uint32_t)(face * 64 + i * 8 + j);
+}
+
 // BIG: arrays in global slabs, regions beyond LDS; the face phase works in
 // place instead of through the wave queues.
 template <bool BIG>
@@ -816,7 +833,10 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     return;
   }
   if (P.big && M > kBigRegionMax) {
-    if (t == 0) P.status[u] = ST_HUGE;
+    if (t == 0) {
+      P.need[u] = (double)M;  // the region's size, reported with the conditional status
+      P.status[u] = ST_HUGE;
+    }
     return;
   }
   if (P.probe) {
@@ -847,10 +867,23 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     }
     __syncthreads();
   }
+  // candidate vertices (ids above the anchor) ordered by the direction cell
+  // of their offset: a wave takes faces (kidx[i], kidx[j]) with one j and
+  // consecutive i, so neighbouring lanes get faces whose pencil capsules
+  // overlap and take the heavy branches of the scans together.  The order
+  // does not change which faces exist.  ridx is free once the points are
+  // compacted; it holds cell << 16 | slot until the rank sort.
   for (uint32_t s = t; s < M; s += blockDim.x)
-    if (w.gid[s] > u) w.kidx[atomicAdd(&sK, 1u)] = s;
+    if (w.gid[s] > u) w.ridx[atomicAdd(&sK, 1u)] = (direction_cell(w.gx[s], w.gy[s], w.gz[s]) << 16) | s;
   __syncthreads();
   const uint32_t Kc = sK;
+  for (uint32_t k = t; k < Kc; k += blockDim.x) {
+    const uint32_t key = w.ridx[k];
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < Kc; ++j) rank += w.ridx[j] < key;
+    w.kidx[rank] = key & 0xFFFFu;
+  }
+  __syncthreads();
   // 3. pairs (u, v), v > u
   for (uint32_t k = t; k < Kc; k += blockDim.x) process_pair(P, w, u, M, w.kidx[k], &sregion);
   // 4. faces (u, v, w), u < v < w, whose members share a cell.  Each wave
@@ -1267,18 +1300,30 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
   overflow.ensure(1);
   total.ensure(1);
   int64_t widened = 0;
+  int64_t open_anchors = 0, largest_open = 0;
   uint64_t ctr[8];
   // The record queues have a fixed capacity per launch; records past it are
   // dropped on the device and counted.  A catalogue that outgrows them runs
   // again with queues sized from the counts (the widened gather radii are
   // kept), unless the device cannot hold them.
   for (int attempt = 0;; ++attempt) {
+  if (c.capacity_bytes > 0) {
+    // the record queues are the catalogue's large, data-sized allocation
+    const int64_t more = (int64_t)(event_cap - std::min<uint64_t>(event_cap, B.events.cap)) * (int64_t)sizeof(EventRec) +
+                         (int64_t)(fb_cap - std::min<uint64_t>(fb_cap, B.fallbacks.cap)) * (int64_t)sizeof(FallbackRec) +
+                         (int64_t)(reg_cap - std::min<uint64_t>(reg_cap, B.fb_region.cap)) * 4;
+    if (device_bytes(c) + more > c.capacity_bytes)
+      throw CapacityError("the catalogue's record queues (" + std::to_string(event_cap) + " records) exceed the " +
+                          "context's device capacity of " + std::to_string(c.capacity_bytes) + " bytes");
+  }
   B.events.ensure(event_cap);
   B.fallbacks.ensure(fb_cap);
   B.fb_region.ensure(reg_cap);
   MHGP_HIP(hipMemsetAsync(B.counters.p, 0, 8 * 8, s));
   MHGP_HIP(hipMemsetAsync(B.status.p, 0, n * 4, s));
   MHGP_HIP(hipMemsetAsync(overflow.p, 0, 4, s));
+  open_anchors = 0;
+  largest_open = 0;
   std::vector<uint32_t> pending = anchors;
   std::vector<uint32_t> big;
   for (int iter = 0; !pending.empty(); ++iter) {
@@ -1409,8 +1454,9 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
             again.push_back(u);
             widened++;
           } else if (st[u] == ST_HUGE) {
-            throw CapacityError("an anchor's candidate region exceeds " + std::to_string(kBigRegionMax) +
-                                " points (dense anisotropic cluster); the device catalogue does not enumerate it");
+            // budget stop: the anchor stays open, the catalogue is conditional
+            open_anchors++;
+            largest_open = std::max<int64_t>(largest_open, (int64_t)need[u]);
           } else if (st[u] == ST_PROBED) {
             full.push_back(u);
           } else {
@@ -1480,6 +1526,9 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
   tm.mark("catalogue download");
   info->candidate_count = (int64_t)ctr[3];
   info->anchors_widened = widened;
+  info->anchors_open = open_anchors;
+  info->largest_open_region = largest_open;
+  info->status = open_anchors ? 1 : 0;
 }
 
 }  // namespace mhgp

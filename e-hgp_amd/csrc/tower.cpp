@@ -267,8 +267,22 @@ void contract_order(int k, int64_t n, TowerOrderData &D, Forest &F) {
   std::vector<Cell> cell(M);
   for (int64_t x = 0; x < M; ++x) cell[x] = {(uint32_t)x, -1};
   F.reserve(k == 1 ? (size_t)n + D.saddles / 8 : D.saddles / 8);
+  F.acq_node.assign(M, -1);
+  F.acq_cls.assign(M, -1);
   if (k == 1)
-    for (int64_t p = 0; p < n; ++p) cell[p].owner = (int32_t)F.add(1, 0, -1, 0.0);
+    for (int64_t p = 0; p < n; ++p) {
+      cell[p].owner = (int32_t)F.add(1, 0, -1, 0.0);
+      F.acq_node[p] = cell[p].owner;
+      F.rep_elem[p] = (int32_t)p;
+    }
+  // a class without an owner is a single element (classes only merge in a
+  // group, which gives them an owner): record its first owner
+  auto acquire = [&](uint32_t root, int64_t node, int64_t cls) {
+    if (cell[root].owner < 0) {
+      F.acq_node[root] = (int32_t)node;
+      F.acq_cls[root] = cls;
+    }
+  };
   std::unordered_map<int64_t, int64_t> bp;
   std::unordered_map<int64_t, uint32_t> efirst;
   for (uint32_t i = 0; i < D.saddles;) {
@@ -301,11 +315,16 @@ void contract_order(int k, int64_t n, TowerOrderData &D, Forest &F) {
         if (nr == 0) node = F.add(k, 1, D.cls[i], D.level[i]);
         else if (nr == 1) node = roots[0];
         else node = F.add(k, 2, D.cls[i], D.level[i], roots, nr);
+        if (nr != 1) F.rep_elem[node] = (int32_t)members[0];
+        for (int q = 0; q < nm; ++q) acquire(members[q], node, D.cls[i]);
         for (int q = 1; q < nm; ++q) cell[members[q]].parent = members[0];
         cell[members[0]].owner = (int32_t)node;
       } else if (cell[members[0]].owner < 0) {
         // all arms in one latent minimum cannot happen in general position
-        cell[members[0]].owner = (int32_t)F.add(k, 1, D.cls[i], D.level[i]);
+        const int64_t node = F.add(k, 1, D.cls[i], D.level[i]);
+        F.rep_elem[node] = (int32_t)members[0];
+        acquire(members[0], node, D.cls[i]);
+        cell[members[0]].owner = (int32_t)node;
       }
       i = j;
       continue;
@@ -350,6 +369,8 @@ void contract_order(int k, int64_t n, TowerOrderData &D, Forest &F) {
       if (roots.empty()) node = F.add(k, 1, D.cls[i], D.level[i]);
       else if (roots.size() == 1) node = roots[0];
       else node = F.add(k, 2, D.cls[i], D.level[i], roots);
+      if (roots.size() != 1) F.rep_elem[node] = (int32_t)members[0];
+      for (int64_t r : members) acquire((uint32_t)r, node, D.cls[i]);
       for (size_t q = 1; q < members.size(); ++q)
         cell[cell_find(cell, (uint32_t)members[q])].parent = cell_find(cell, (uint32_t)members[0]);
       cell[cell_find(cell, (uint32_t)members[0])].owner = (int32_t)node;
@@ -360,7 +381,37 @@ void contract_order(int k, int64_t n, TowerOrderData &D, Forest &F) {
     std::fprintf(stderr, "[mhgp] contract order %2d: %u saddles, %u arms, %u minima, %zu nodes, %.1f ms\n", k,
                  D.saddles, D.arms, D.minima, F.order.size(),
                  std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+  F.raw_of_dense = std::move(D.raw_of_dense);
   D = TowerOrderData{};  // release the order's arrays
+}
+
+// Vertical map of order k + 1 onto order k (reference: the profile vertical
+// map family of reference/morsehgp3d_oracle/hierarchy.py:656-780 and the
+// direct vertical journal, direct_morse_vertical_journal.cpp:822).  A node N
+// of order k + 1 at level L maps to the root, at the closed cut L, of the
+// order-k component that holds the k-subsets of the facets of N's component.
+// N's class at L contains a minimum m of order k + 1 (the representative of
+// the group that created N); m's closed label is a saddle of order k whose
+// first arm (a k-subset of it) descends to an order-k minimum c, touched no
+// later than m's level <= L.  So the target is the owner at L of c's class:
+// the node that first owned the class of c's contraction element, raised
+// through its ancestors while their level is <= L.
+static void vertical_order(const Forest &S, const Forest &T, const std::vector<uint32_t> &elem,
+                           std::vector<int64_t> &target) {
+  const size_t m = S.order.size(), tm = T.order.size();
+  std::vector<int64_t> parent(tm, -1);
+  for (size_t x = 0; x < tm; ++x)
+    for (int64_t j = T.child_offset[x]; j < T.child_offset[x + 1]; ++j) parent[T.children[j]] = (int64_t)x;
+  target.assign(m, -1);
+  for (size_t i = 0; i < m; ++i) {
+    const uint32_t e = elem[i];
+    const int64_t L = S.level_class[i];
+    if (e == kInvalid || e >= T.acq_node.size() || T.acq_node[e] < 0 || T.acq_cls[e] > L)
+      throw std::runtime_error("vertical map: the order below holds no owned class at the node's level");
+    int64_t x = T.acq_node[e];
+    while (parent[x] >= 0 && T.level_class[parent[x]] <= L) x = parent[x];
+    target[i] = x;
+  }
 }
 
 void tower(Context &c, int k_max, mhgp_tower_info *info) {
@@ -370,6 +421,10 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
   // The direct construction needs every relevant sphere to have shell == support;
   // the reference refuses the journal otherwise (decision
   // no_journal_relevant_extra_shell_diagnostics, direct_morse_event_journal.cpp:587-592).
+  if (B.info.status != 0)
+    throw UnsupportedError("the catalogue is conditional (" + std::to_string(B.info.anchors_open) +
+                           " anchors with candidate regions of up to " + std::to_string(B.info.largest_open_region) +
+                           " points were not enumerated): the tower needs the complete catalogue");
   if (B.info.diagnostic_count > 0)
     throw UnsupportedError("the catalogue has " + std::to_string(B.info.diagnostic_count) +
                            " relevant extra-shell diagnostics: the direct Morse tower is not defined");
@@ -444,6 +499,10 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
             std::sort(arm.begin(), arm.end());
             D.carrier[a] = (uint32_t)desc->descend(arm, minima);
           }
+        // vertical maps: the saddles' first-arm carriers are the elements
+        std::vector<uint32_t> elems(D.saddles);
+        for (uint32_t s2 = 0; s2 < D.saddles; ++s2) elems[s2] = D.carrier[D.arm_base[s2]];
+        tower_vm_store(c, W.stream, D.vm_first_pos, elems);
       }
       std::lock_guard<std::mutex> g(mu);
       workers.emplace_back([&, k] {
@@ -491,6 +550,39 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
   for (auto &e : errors)
     if (e) std::rethrow_exception(e);
   tm.mark("tower forests (host)");
+  // vertical maps: one gather of every node's element, then per order a walk
+  // up the order below
+  {
+    std::vector<uint32_t> pos;
+    std::vector<size_t> first(K + 1, 0);
+    for (int o = 2; o <= K; ++o) {
+      const Forest &S = forests[o - 1];
+      first[o] = pos.size();
+      const uint32_t base = c.tower_work->rank_lo[o];
+      for (size_t i = 0; i < S.order.size(); ++i) {
+        const int32_t d = S.rep_elem[i];
+        if (d < 0) throw std::runtime_error("vertical map: a node without a representative element");
+        pos.push_back(base + (S.raw_of_dense.empty() ? (uint32_t)d : S.raw_of_dense[d]));
+      }
+    }
+    std::vector<uint32_t> elem;
+    tower_vm_gather(c, pos, elem);
+    std::vector<std::thread> th;
+    std::vector<std::exception_ptr> verr(K + 1);
+    for (int o = 2; o <= K; ++o)
+      th.emplace_back([&, o] {
+        try {
+          const std::vector<uint32_t> e(elem.begin() + first[o], elem.begin() + first[o] + forests[o - 1].order.size());
+          vertical_order(forests[o - 1], forests[o - 2], e, forests[o - 1].vtarget);
+        } catch (...) {
+          verr[o] = std::current_exception();
+        }
+      });
+    for (auto &t : th) t.join();
+    for (auto &e : verr)
+      if (e) std::rethrow_exception(e);
+  }
+  tm.mark("tower vertical maps (host)");
   // the orders stay separate; fetches concatenate them once
   for (const Forest &F : forests) {
     T.node_count += (int64_t)F.order.size();
@@ -513,6 +605,54 @@ void tower_fetch(Context &c, int32_t *order, int32_t *kind, int64_t *level_class
                  int64_t *child_offset, int64_t *children) {
   if (!c.tower || !c.tower->valid) throw StateError("no tower has been built");
   c.tower->copy_out(order, kind, level_class, level, child_offset, children);
+}
+
+// Components of one order at a cut: the nodes at or below the level (strictly
+// below for a strict cut) whose parent lies above it, as global node ids
+// (reference: K1CompactForest::cut_strict / cut_closed,
+// morsehgp3d/include/morsehgp3d/hierarchy/k1_forest.hpp:85-91, for every order).
+void tower_cut(Context &c, int order, int64_t cls, int closed, int64_t *roots, int64_t *count,
+               int64_t *root_of_point) {
+  if (!c.tower || !c.tower->valid) throw StateError("no tower has been built");
+  const TowerState &T = *c.tower;
+  if (order < 1 || order > T.orders) throw std::invalid_argument("order outside the tower");
+  if (!count) throw std::invalid_argument("null count");
+  if (root_of_point && order != 1) throw std::invalid_argument("point labels exist for order one only");
+  int64_t base = 0;
+  for (int o = 1; o < order; ++o) base += (int64_t)T.forests[o - 1].order.size();
+  const Forest &F = T.forests[order - 1];
+  const size_t m = F.order.size();
+  std::vector<int64_t> parent(m, -1);
+  for (size_t x = 0; x < m; ++x)
+    for (int64_t j = F.child_offset[x]; j < F.child_offset[x + 1]; ++j) parent[F.children[j]] = (int64_t)x;
+  auto in = [&](int64_t x) { return closed ? F.level_class[x] <= cls : F.level_class[x] < cls; };
+  int64_t nr = 0;
+  for (size_t x = 0; x < m; ++x)
+    if (in((int64_t)x) && (parent[x] < 0 || !in(parent[x]))) {
+      if (roots) roots[nr] = base + (int64_t)x;
+      nr++;
+    }
+  *count = nr;
+  if (root_of_point) {
+    // order one: node p is the birth of point p
+    std::vector<int64_t> top(m, -2);
+    for (int64_t p = 0; p < c.n; ++p) {
+      if (!in(p)) { root_of_point[p] = -1; continue; }
+      int64_t x = p;
+      std::vector<int64_t> path;
+      while (top[x] == -2 && parent[x] >= 0 && in(parent[x])) { path.push_back(x); x = parent[x]; }
+      const int64_t r = top[x] >= 0 ? top[x] : x;
+      top[x] = r;
+      for (int64_t y : path) top[y] = r;
+      root_of_point[p] = base + r;
+    }
+  }
+}
+
+void tower_vertical(Context &c, int64_t *target) {
+  if (!c.tower || !c.tower->valid) throw StateError("no tower has been built");
+  if (!target) throw std::invalid_argument("null output buffer");
+  c.tower->copy_vertical(target);
 }
 
 }  // namespace mhgp

@@ -812,7 +812,52 @@ __global__ void k_compact_saddles(int64_t S, const uint32_t *__restrict__ kpos, 
   for (uint32_t a = b; a < e; ++a) o_car[nb + (a - b)] = rep ? dense[rep[carrier[a]]] - 1u : carrier[a];
 }
 
+// Vertical maps: the order-k contraction element (dense representative id;
+// the minimum itself for order one or when the host descends) of the first
+// arm's carrier of every order-k saddle, stored at the saddle's rank-index
+// position, which is also its position as a minimum of order k + 1.
+__global__ void k_vm_elem(int64_t S, const uint32_t *__restrict__ base, const uint32_t *__restrict__ carrier,
+                          const uint32_t *__restrict__ rep, const uint32_t *__restrict__ dense, uint32_t *out) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const uint32_t c = carrier[base[s]];
+  out[s] = (rep && c != kInvalid) ? dense[rep[c]] - 1u : c;
+}
+
+// Raw minimum of every dense representative id (the inverse of dense - 1).
+__global__ void k_raw_of_dense(int64_t M, const uint32_t *__restrict__ dense, uint32_t *out) {
+  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  const uint32_t prev = m ? dense[m - 1] : 0u;
+  if (dense[m] != prev) out[prev] = (uint32_t)m;
+}
+
+__global__ void k_gather_u32(const uint32_t *__restrict__ src, const uint32_t *__restrict__ idx, int64_t n,
+                             uint32_t *out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = src[idx[i]];
+}
+
 }  // namespace
+
+void tower_vm_gather(Context &c, const std::vector<uint32_t> &pos, std::vector<uint32_t> &out) {
+  TowerWork &R = *c.tower_work;
+  const int64_t q = (int64_t)pos.size();
+  out.assign(q, kInvalid);
+  if (!q) return;
+  hipStream_t s = c.stream;
+  R.vm_idx.ensure(q);
+  R.vm_out.ensure(q);
+  MHGP_HIP(hipMemcpyAsync(R.vm_idx.p, pos.data(), q * 4, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_gather_u32, dim3(blocks_for(q, 256)), dim3(256), 0, s, R.vm_elem.p, R.vm_idx.p, q, R.vm_out.p);
+  MHGP_LAUNCH_CHECK();
+  copy_wait(out.data(), R.vm_out.p, q * 4, hipMemcpyDeviceToHost, s);
+}
+
+void tower_vm_store(Context &c, hipStream_t s, uint32_t first_pos, const std::vector<uint32_t> &elems) {
+  if (elems.empty()) return;
+  copy_wait(c.tower_work->vm_elem.p + first_pos, elems.data(), elems.size() * 4, hipMemcpyHostToDevice, s);
+}
 
 void tower_rank_index(Context &c) {
   CatalogBuffers &B = *c.catalog;
@@ -825,6 +870,7 @@ void tower_rank_index(Context &c) {
   if (n == 0) return;
   W.rank_keys.ensure(n);
   W.rank_vals.ensure(n);
+  W.vm_elem.ensure(n);
   W.rank_bounds.ensure(512);
   hipLaunchKernelGGL(k_rank_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.events.p, B.order.p, n,
                      W.rank_keys.p, W.rank_vals.p);
@@ -1079,7 +1125,14 @@ void tower_order_device(Context &c, TowerWork &W, const TowerWork &R, int k, Tow
       copy_wait(&Mdense, dense.p + minima_n - 1, 4, hipMemcpyDeviceToHost, s);
       d_rep = rep.p;
       d_dense = dense.p;
+      // dense id -> raw minimum, for the vertical maps out of this order
+      W.raw_dense.ensure(Mdense + 1);
+      hipLaunchKernelGGL(k_raw_of_dense, gm, tb, 0, s, minima_n, dense.p, W.raw_dense.p);
+      out.raw_of_dense.resize(Mdense);
+      if (Mdense) copy_wait(out.raw_of_dense.data(), W.raw_dense.p, (size_t)Mdense * 4, hipMemcpyDeviceToHost, s);
     }
+    hipLaunchKernelGGL(k_vm_elem, gs, tb, 0, s, (int64_t)S, base.p, carrier.p, d_rep, d_dense,
+                       R.vm_elem.p + R.rank_lo[k + 1]);
     W.scanner.inclusive(keep.p, S, s);
     W.scanner.inclusive(kept.p, S, s);
     MHGP_HIP(hipMemcpyAsync(&S2, keep.p + S - 1, 4, hipMemcpyDeviceToHost, s));
@@ -1134,6 +1187,7 @@ void tower_order_device(Context &c, TowerWork &W, const TowerWork &R, int k, Tow
                  std::chrono::duration<double, std::milli>(ph[6] - ph[5]).count(),
                  std::chrono::duration<double, std::milli>(ph[7] - ph[6]).count());
   const bool need_labels = any_invalid != 0;  // then the saddles were not filtered
+  out.vm_first_pos = R.rank_lo[k + 1];
   if (need_labels) {
     // the host descends unresolved arms; it needs the saddle labels, their
     // supports and the minima labels

@@ -24,6 +24,8 @@ struct TowerOrderData {
   std::vector<uint32_t> sup;            // 4 support ids per saddle, only when the host must descend
   std::vector<uint32_t> minima_labels;  // only when the host must descend
   std::vector<uint32_t> saddle_labels;  // 11 per saddle, only when the host must descend
+  std::vector<uint32_t> raw_of_dense;   // raw minimum of each dense contraction element (empty: identity)
+  uint32_t vm_first_pos = 0;            // rank-index position of the first saddle
   int64_t steps = 0;
 };
 
@@ -60,13 +62,17 @@ struct TowerWork {
   DevBuf<uint32_t> comp, comp2, best, hook, msf, mcount, changed, err, live_n;
   DevBuf<uint32_t> Lu[2], Lv[2], Li[2];
   DevBuf<uint32_t> attach, rep, dense, slab, ssup;
+  // vertical maps: per record (rank-index position) the contraction element
+  // of its first arm's carrier as a saddle of the order below (first lane
+  // only), the gather scratch, and the dense -> raw minimum map of a lane
+  DevBuf<uint32_t> vm_elem, vm_idx, vm_out, raw_dense;
   int64_t bytes() const {
     int64_t b = 0;
     for (const DevBuf<uint32_t> *x : {&rank_vals, &rank_bounds, &tvals, &labels, &sizes, &carrier, &scls,
                                       &base, &arm_saddle, &queue, &first, &keep, &kept, &flag1, &c_cls, &c_base,
                                       &c_car, &eflag, &eu, &ev, &es, &comp, &comp2, &best, &hook, &msf, &mcount,
                                       &changed, &err, &live_n, &Lu[0], &Lu[1], &Lv[0], &Lv[1], &Li[0], &Li[1],
-                                      &attach, &rep, &dense, &slab, &ssup, &alias_arm, &alias_rep, &dF[0], &dF[1], &dH[0], &dH[1], &dO[0], &dO[1]})
+                                      &attach, &rep, &dense, &slab, &ssup, &vm_elem, &vm_idx, &vm_out, &raw_dense, &alias_arm, &alias_rep, &dF[0], &dF[1], &dH[0], &dH[1], &dO[0], &dO[1]})
       b += (int64_t)x->bytes();
     return b + (int64_t)(rank_keys.bytes() + qkey.bytes() + facet_slots.bytes() + alias_count.bytes() + dcount.bytes() + tkeys.bytes() + steps.bytes() + qcount.bytes() + slev.bytes() + c_lev.bytes());
   }
@@ -77,5 +83,9 @@ void tower_rank_index(Context &c);
 // Resolves the arms of order k and reduces its saddles (after
 // tower_rank_index) on the lane W; R holds the rank index.
 void tower_order_device(Context &c, TowerWork &W, const TowerWork &R, int k, TowerOrderData &out);
+// Vertical maps: contraction elements at the given rank-index positions
+// (first lane's stream), and host-settled elements written back.
+void tower_vm_gather(Context &c, const std::vector<uint32_t> &pos, std::vector<uint32_t> &out);
+void tower_vm_store(Context &c, hipStream_t s, uint32_t first_pos, const std::vector<uint32_t> &elems);
 
 }  // namespace mhgp

@@ -17,8 +17,20 @@ struct Forest {
   std::vector<double> level;
   std::vector<int64_t> child_offset{0};
   std::vector<int64_t> children;
+  // Vertical maps.  Contraction elements are the order's minima (points at
+  // order one, dense representatives otherwise): per element, the node that
+  // first owned its class and that node's level class there; per node, an
+  // element of its class at its own level; element -> raw minimum index
+  // (empty: identity); per node, its vertical target in the order below
+  // (node id local to that order, -1 at order one).
+  std::vector<int32_t> acq_node;
+  std::vector<int64_t> acq_cls;
+  std::vector<int32_t> rep_elem;
+  std::vector<uint32_t> raw_of_dense;
+  std::vector<int64_t> vtarget;
 
   int64_t add(int32_t o, int32_t k, int64_t cls, double lev, const int64_t *kids = nullptr, size_t nk = 0) {
+    rep_elem.push_back(-1);
     order.push_back(o);
     kind.push_back(k);
     level_class.push_back(cls);
@@ -31,6 +43,7 @@ struct Forest {
     return add(o, k, cls, lev, kids.data(), kids.size());
   }
   void reserve(size_t nodes) {
+    rep_elem.reserve(nodes);
     order.reserve(nodes);
     kind.reserve(nodes);
     level_class.reserve(nodes);
@@ -72,6 +85,17 @@ struct TowerState {
       cb += (int64_t)F.children.size();
     }
     for (auto &t : th) t.join();
+  }
+
+  // Vertical targets as global node ids (orders ascending), -1 at order one.
+  void copy_vertical(int64_t *target) const {
+    int64_t nb = 0, below = 0;
+    for (const Forest &F : forests) {
+      for (size_t i = 0; i < F.order.size(); ++i)
+        target[nb + (int64_t)i] = (F.vtarget.empty() || F.vtarget[i] < 0) ? -1 : below + F.vtarget[i];
+      below = nb;
+      nb += (int64_t)F.order.size();
+    }
   }
 };
 

@@ -32,7 +32,8 @@ class CatalogInfo(ctypes.Structure):
     _fields_ = [("event_count", ctypes.c_int64), ("diagnostic_count", ctypes.c_int64),
                 ("candidate_count", ctypes.c_int64), ("fallback_count", ctypes.c_int64),
                 ("anchors_widened", ctypes.c_int64), ("s_max", ctypes.c_int32),
-                ("status", ctypes.c_int32)]
+                ("status", ctypes.c_int32), ("anchors_open", ctypes.c_int64),
+                ("largest_open_region", ctypes.c_int64)]
 
 
 class TowerInfo(ctypes.Structure):
@@ -64,7 +65,7 @@ def lib() -> ctypes.CDLL:
         L = ctypes.CDLL(LIB_PATH)
         p, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
         L.mhgp_last_error.restype = ctypes.c_char_p
-        L.mhgp_ctx_create.argtypes = [i32, ctypes.POINTER(p)]
+        L.mhgp_ctx_create.argtypes = [i32, i64, ctypes.POINTER(p)]
         L.mhgp_ctx_destroy.argtypes = [p]
         L.mhgp_ctx_device_bytes.argtypes = [p]
         L.mhgp_ctx_device_bytes.restype = i64
@@ -75,12 +76,17 @@ def lib() -> ctypes.CDLL:
         L.mhgp_canonical_points.argtypes = [p, p, p]
         L.mhgp_lbvh_snapshot.argtypes = [p, p, p, p]
         L.mhgp_knn.argtypes = [p, i32, p]
+        L.mhgp_lbvh_certify.argtypes = [p, i64, p, p, p]
         L.mhgp_emst.argtypes = [p, p]
+        L.mhgp_knn_partition.argtypes = [p, p, p, i64, i32, p, p, ctypes.c_uint32, p, p, p, p, p, p, p]
         L.mhgp_supports.argtypes = [p, i32, ctypes.POINTER(CatalogInfo)]
         L.mhgp_supports_fetch.argtypes = [p, p, p, p, p, p]
         L.mhgp_supports_radii.argtypes = [p, p]
+        L.mhgp_supports_diagnostics.argtypes = [p, p, p, p]
         L.mhgp_tower.argtypes = [p, i32, ctypes.POINTER(TowerInfo)]
         L.mhgp_tower_fetch.argtypes = [p, p, p, p, p, p, p]
+        L.mhgp_tower_vertical.argtypes = [p, p]
+        L.mhgp_tower_cut.argtypes = [p, i32, i64, i32, p, p, p]
         L.mhgp_ctx_stats.argtypes = [p, ctypes.POINTER(Stats), i32]
         L.mhgp_closed_ball.argtypes = [p, p, p, i64, ctypes.c_uint32, p, p, p]
         L.mhgp_server_create.argtypes = [i32, i32, i64, ctypes.POINTER(p)]
@@ -97,11 +103,21 @@ def lib() -> ctypes.CDLL:
 EXPORTED = [
     "mhgp_last_error", "mhgp_ctx_create", "mhgp_ctx_destroy", "mhgp_ctx_device_bytes",
     "mhgp_cloud_upload", "mhgp_cloud_upload_device", "mhgp_point_count",
-    "mhgp_canonical_points", "mhgp_lbvh_snapshot", "mhgp_knn", "mhgp_emst", "mhgp_supports",
-    "mhgp_supports_fetch", "mhgp_supports_radii", "mhgp_tower", "mhgp_tower_fetch", "mhgp_ctx_stats",
+    "mhgp_canonical_points", "mhgp_lbvh_snapshot", "mhgp_lbvh_certify", "mhgp_knn", "mhgp_knn_partition", "mhgp_emst", "mhgp_supports",
+    "mhgp_supports_fetch", "mhgp_supports_radii", "mhgp_supports_diagnostics", "mhgp_tower", "mhgp_tower_fetch", "mhgp_tower_vertical", "mhgp_tower_cut", "mhgp_ctx_stats",
     "mhgp_closed_ball", "mhgp_server_create", "mhgp_server_submit", "mhgp_server_wait", "mhgp_server_fetch",
     "mhgp_server_release", "mhgp_server_destroy",
 ]
+
+
+def lbvh_certify(canonical_xyz, leaves, nodes, counters) -> None:
+    """Exact host re-certification of a snapshot (raises MhgpError with the
+    first violation); needs no GPU."""
+    xyz = np.ascontiguousarray(np.asarray(canonical_xyz, dtype=np.float64).reshape(-1, 3))
+    lv = np.ascontiguousarray(np.asarray(leaves, dtype=np.uint64))
+    nd = np.ascontiguousarray(np.asarray(nodes, dtype=np.uint64))
+    ct = np.ascontiguousarray(np.asarray(counters, dtype=np.uint64))
+    _check(lib().mhgp_lbvh_certify(_ptr(xyz), len(xyz), _ptr(lv), _ptr(nd), _ptr(ct)))
 
 
 def _check(rc: int) -> None:
@@ -128,9 +144,9 @@ class Catalog:
 class Context:
     """One cloud on one GPU (owns a HIP stream and the device state)."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, capacity_bytes: int = 0):
         h = ctypes.c_void_p()
-        _check(lib().mhgp_ctx_create(device, ctypes.byref(h)))
+        _check(lib().mhgp_ctx_create(device, capacity_bytes, ctypes.byref(h)))
         self._h = h
 
     def close(self) -> None:
@@ -189,6 +205,33 @@ class Context:
         _check(lib().mhgp_knn(self._h, k, _ptr(out)))
         return out
 
+    def knn_partition(self, num, den, k: int, exclusions=None, shell_cap: int = 64):
+        """Exact top-k partitions of rational queries num[q] / den[q]
+        (integer numerators, positive integer denominators).  Returns a list
+        of dicts: cutoff_sq, cutoff_id, strict (ids by exact distance), shell
+        (complete ids ascending), shell_count, choice (ascending)."""
+        num = np.ascontiguousarray(np.asarray(num, dtype=np.int64).reshape(-1, 3))
+        den = np.ascontiguousarray(np.asarray(den, dtype=np.int64).reshape(-1))
+        q = len(num)
+        ex = exclusions or [[] for _ in range(q)]
+        off = np.zeros(q + 1, dtype=np.int64)
+        off[1:] = np.cumsum([len(e) for e in ex])
+        ids = np.ascontiguousarray(np.asarray([x for e in ex for x in e] or [0], dtype=np.uint32))
+        choice = np.zeros((max(q, 1), k), dtype=np.uint32)
+        strict = np.zeros((max(q, 1), k), dtype=np.uint32)
+        ns = np.zeros(max(q, 1), dtype=np.uint32)
+        shell = np.zeros((max(q, 1), max(shell_cap, 1)), dtype=np.uint32)
+        nsh = np.zeros(max(q, 1), dtype=np.uint32)
+        cid = np.zeros(max(q, 1), dtype=np.uint32)
+        csq = np.zeros(max(q, 1), dtype=np.float64)
+        _check(lib().mhgp_knn_partition(self._h, _ptr(num), _ptr(den), q, k, _ptr(ids), _ptr(off), shell_cap,
+                                        _ptr(choice), _ptr(strict), _ptr(ns), _ptr(shell), _ptr(nsh), _ptr(cid),
+                                        _ptr(csq)))
+        return [{"cutoff_sq": float(csq[i]), "cutoff_id": int(cid[i]),
+                 "strict": [int(x) for x in strict[i, : ns[i]]], "shell_count": int(nsh[i]),
+                 "shell": [int(x) for x in shell[i, : min(int(nsh[i]), shell_cap)]],
+                 "choice": [int(x) for x in choice[i]]} for i in range(q)]
+
     def emst(self) -> np.ndarray:
         out = np.zeros((max(self.n - 1, 0), 2), dtype=np.uint32)
         _check(lib().mhgp_emst(self._h, _ptr(out)))
@@ -246,11 +289,30 @@ class Context:
                                          _ptr(out["level"]), _ptr(out["level_class"])))
         return out
 
+    def diagnostics(self, count: int):
+        """Shell count, extra-shell witness (-1: none) and exterior count of
+        each of the last catalogue's `count` records (fetch order)."""
+        sc = np.zeros(max(count, 1), dtype=np.uint32)
+        wit = np.zeros(max(count, 1), dtype=np.int64)
+        ext = np.zeros(max(count, 1), dtype=np.int64)
+        _check(lib().mhgp_supports_diagnostics(self._h, _ptr(sc), _ptr(wit), _ptr(ext)))
+        return sc[:count], wit[:count], ext[:count]
+
     def radii(self) -> np.ndarray:
         """Per-PointId enumeration radius of the last catalogue."""
         out = np.zeros(max(self.n, 1), dtype=np.float64)
         _check(lib().mhgp_supports_radii(self._h, _ptr(out)))
         return out[: self.n]
+
+    def tower_cut(self, order: int, level_class: int, closed: bool = True, node_count: int = 0):
+        """Roots of `order`'s components at the cut, and for order one the
+        root of every PointId (-1: not yet born)."""
+        roots = np.zeros(max(node_count, 1), dtype=np.int64)
+        cnt = ctypes.c_int64()
+        labels = np.zeros(max(self.n, 1), dtype=np.int64) if order == 1 else None
+        _check(lib().mhgp_tower_cut(self._h, order, level_class, int(closed), _ptr(roots) if node_count else None,
+                                    ctypes.byref(cnt), _ptr(labels) if labels is not None else None))
+        return roots[: cnt.value] if node_count else None, labels[: self.n] if labels is not None else None
 
     def tower(self, k_max: int, fetch: bool = True):
         info = TowerInfo()
@@ -266,8 +328,10 @@ class Context:
         kids = np.zeros(max(info.child_count, 1), dtype=np.int64)
         _check(lib().mhgp_tower_fetch(self._h, _ptr(order), _ptr(kind), _ptr(cls), _ptr(level),
                                       _ptr(off), _ptr(kids)))
+        vert = np.zeros(max(m, 1), dtype=np.int64)
+        _check(lib().mhgp_tower_vertical(self._h, _ptr(vert)))
         return {"order": order, "kind": kind, "level_class": cls, "level": level,
-                "child_offset": off, "children": kids[: info.child_count], "info": info}
+                "child_offset": off, "children": kids[: info.child_count], "vertical": vert[:m], "info": info}
 
 
 def _tower_arrays(node_count: int, child_count: int):

@@ -43,8 +43,15 @@ typedef struct mhgp_ctx mhgp_ctx;
 /* Message of the last failure on this thread ("" when none). */
 const char *mhgp_last_error(void);
 
-/* Creates a context on HIP device `device` with its own stream. */
-int mhgp_ctx_create(int device, mhgp_ctx **out);
+/* Creates a context on HIP device `device` with its own stream.
+ * capacity_bytes > 0 bounds the device memory the context may hold (the
+ * capacity of the reference's launcher contexts, e.g.
+ * MortonLbvhBuildContext's device capacity, morton_lbvh_build.hpp:41-118):
+ * the catalogue refuses record queues that would exceed it, and any call
+ * that leaves the context holding more returns MHGP_E_CAPACITY after
+ * releasing the context's device buffers (the context stays usable; upload
+ * again).  0: no bound beyond the device's own memory. */
+int mhgp_ctx_create(int device, int64_t capacity_bytes, mhgp_ctx **out);
 /* Releases every device buffer of the context. */
 void mhgp_ctx_destroy(mhgp_ctx *ctx);
 /* Device bytes currently held by the context. */
@@ -73,12 +80,46 @@ int mhgp_canonical_points(mhgp_ctx *ctx, double *xyz_out, uint32_t *source_index
  * morton_collision_group_count, maximum_morton_collision_size. */
 int mhgp_lbvh_snapshot(mhgp_ctx *ctx, uint64_t *leaves, uint64_t *nodes, uint64_t *counters);
 
+/* Exact re-certification of an LBVH snapshot against a canonical cloud
+ * (xyz: n x 3 binary64 in PointId order, as mhgp_canonical_points returns
+ * it), on the host, with no context or device: the role of the reference's
+ * MortonLbvhIndex::import_certified_snapshot (morsehgp3d/src/cpu/spatial/
+ * lbvh.cpp:562-829).  Every code is re-quantised exactly, the leaf order,
+ * every split, postorder index, leaf range and witness, and the five
+ * counters are checked.  MHGP_OK when the snapshot is exactly the certified
+ * LBVH; MHGP_E_ARGUMENT naming the first violation otherwise. */
+int mhgp_lbvh_certify(const double *xyz, int64_t n, const uint64_t *leaves, const uint64_t *nodes,
+                      const uint64_t *counters);
+
 /* k nearest other points of every point, 1 <= k <= 32, n*k PointIds ordered
  * by exact squared distance then PointId.  Replaces the top-k proposals of
  * propose_phase14_facet_top_k_candidates_on_gpu
  * (phase14_facet_top_k_proposal_internal.hpp:183) for data-point queries;
  * ties are settled exactly, no proposal is left to the caller. */
 int mhgp_knn(mhgp_ctx *ctx, int k, uint32_t *ids_out);
+
+/* Exact top-k partitions of rational query points: reference lbvh_top_k
+ * (morsehgp3d/include/morsehgp3d/spatial/lbvh.hpp:424, lbvh.cpp:1098-1198)
+ * and its TopKPartition (spatial/query.hpp:109-169; brute_force.cpp:320-370).
+ * Query q is (query_num[3q..3q+2]) / query_den[q] (den > 0, one common
+ * denominator per query); its eligible points are every PointId except
+ * excluded[excluded_offsets[q] .. excluded_offsets[q+1]) (excluded_offsets
+ * has queries + 1 entries starting at 0; both may be NULL for none).
+ * requested rank k in 1..32, at most the eligible count.  Per query:
+ *  cutoff_sq[q]: the exact k-th smallest squared distance (FP64 value; the
+ *    exact rational is |p - q|^2 for p = cutoff_id[q]);
+ *  cutoff_id[q]: the least PointId on the cutoff shell;
+ *  strict_ids[k*q ..]: the points strictly closer than the cutoff, by exact
+ *    (squared distance, PointId), strict_count[q] < k of them, UINT32_MAX padded;
+ *  shell_ids[shell_cap*q ..]: the complete cutoff shell ascending (complete
+ *    when shell_count[q] <= shell_cap), shell_count[q] its size;
+ *  choice_ids[k*q ..]: the canonical choice, the strict points plus the
+ *    smallest shell ids up to k, ascending.
+ * The device proposes a superset of the shell; the host decides exactly. */
+int mhgp_knn_partition(mhgp_ctx *ctx, const int64_t *query_num, const int64_t *query_den, int64_t queries, int k,
+                       const uint32_t *excluded, const int64_t *excluded_offsets, uint32_t shell_cap,
+                       uint32_t *choice_ids, uint32_t *strict_ids, uint32_t *strict_count, uint32_t *shell_ids,
+                       uint32_t *shell_count, uint32_t *cutoff_id, double *cutoff_sq);
 
 /* Exact Euclidean minimum spanning tree under kappa = (d^2, u, v): n-1 edges
  * (u < v) in kappa order.  Replaces propose_k1_boruvka_candidates_on_gpu
@@ -116,7 +157,10 @@ typedef struct {
   int64_t fallback_count;    /* predicates settled by the host exact path */
   int64_t anchors_widened;   /* anchors whose gather radius had to grow */
   int32_t s_max;
-  int32_t status;            /* 0 exact, 1 conditional */
+  int32_t status;            /* 0 exact; 1 conditional: anchors_open anchors were not enumerated */
+  int64_t anchors_open;      /* anchors whose candidate region exceeds the enumeration bound (their
+                                supports are missing; every other anchor's are complete and exact) */
+  int64_t largest_open_region;  /* largest candidate region among them (points) */
 } mhgp_catalog_info;
 
 int mhgp_supports(mhgp_ctx *ctx, int k_max, mhgp_catalog_info *info);
@@ -131,6 +175,18 @@ int mhgp_supports(mhgp_ctx *ctx, int k_max, mhgp_catalog_info *info);
  * level_class: dense rank of the exact squared radius (equal classes share it). */
 int mhgp_supports_fetch(mhgp_ctx *ctx, uint32_t *ids, uint32_t *meta, uint32_t *interior,
                         double *level, int64_t *level_class);
+
+/* Shell diagnostics of the last catalogue's records, in the order of
+ * mhgp_supports_fetch (reference: the extra-shell fields of the pair and
+ * higher records, pair_support_stream.hpp:272-284):
+ *  shell_count: points on the sphere, the support included
+ *    (observed closed rank = interior count + shell_count; minimum possible
+ *    closed rank = interior count + support size = the record's rank);
+ *  witness: the least PointId on the sphere outside the support
+ *    (canonical extra-shell witness), -1 for a regular event;
+ *  exterior_count: points strictly outside the sphere.
+ * Any argument may be NULL. */
+int mhgp_supports_diagnostics(mhgp_ctx *ctx, uint32_t *shell_count, int64_t *witness, int64_t *exterior_count);
 
 /* Enumeration radius of every PointId u in the last catalogue (n doubles,
  * input unit): every point of every critical sphere whose least PointId is u
@@ -160,6 +216,30 @@ int mhgp_tower(mhgp_ctx *ctx, int k_max, mhgp_tower_info *info);
  * CSR child list (child_offset has node_count + 1 entries). */
 int mhgp_tower_fetch(mhgp_ctx *ctx, int32_t *order, int32_t *kind, int64_t *level_class,
                      double *level, int64_t *child_offset, int64_t *children);
+
+/* Components of order `order` of the last tower at a cut of level class
+ * `level_class` (as in mhgp_supports_fetch; -1 is level zero): closed != 0
+ * keeps every node at or below the level, closed == 0 only those strictly
+ * below (the order-one leaves, born at zero, are then absent at level zero).
+ * roots (capacity node_count, may be NULL): the components' root nodes
+ * ascending, root_count their number.  root_of_point (order one only, n
+ * entries, may be NULL): each PointId's root, -1 when not yet born.
+ * Reference: K1CompactForest::cut_strict / cut_closed
+ * (morsehgp3d/include/morsehgp3d/hierarchy/k1_forest.hpp:85-91; k1_forest.cpp:665),
+ * here for every order. */
+int mhgp_tower_cut(mhgp_ctx *ctx, int order, int64_t level_class, int closed, int64_t *roots, int64_t *root_count,
+                   int64_t *root_of_point);
+
+/* Vertical maps of the last tower: for every node (numbered as in
+ * mhgp_tower_fetch) of order k + 1 >= 2, the node of order k onto which its
+ * component maps at the node's own level (closed cut): the root, at that
+ * level, of the order-k component holding the k-subsets of the node's
+ * facets; -1 for order-one nodes.  Reference: the profile vertical map
+ * family (reference/morsehgp3d_oracle/hierarchy.py:656-780, serialised as
+ * VerticalMap assignments, schemas/morsehgp3d-contract-v2.schema.json:904)
+ * and build_exact_direct_morse_vertical_journal
+ * (direct_morse_vertical_journal.cpp:822). */
+int mhgp_tower_vertical(mhgp_ctx *ctx, int64_t *target);
 
 /* Device-time accounting of the context's hot kernels since the last reset,
  * measured with HIP events on the context stream.  No reference counterpart:

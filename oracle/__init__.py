@@ -268,3 +268,28 @@ def emst_graph(canonical_points, cand_edges) -> np.ndarray:
     if k < 0:
         raise OracleError(f"emst_graph failed with code {k}")
     return out[:k]
+
+
+def knn_partition(canonical_points, num, den: int, k: int, exclusions=()) -> dict:
+    """Exact top-k partition of the rational query num / den (pure Python,
+    exact rationals; small clouds), following the reference TopKPartition
+    (morsehgp3d/src/cpu/spatial/brute_force.cpp:320-370): cutoff = exact k-th
+    smallest squared distance over the eligible points, strict = the points
+    below it by (distance, id), shell = every eligible point at it, choice =
+    strict plus the smallest shell ids up to k, ascending."""
+    a = _pts(canonical_points)
+    q = [Fraction(int(x), int(den)) for x in num]
+    ex = set(int(x) for x in exclusions)
+    d = []
+    for i, p in enumerate(a):
+        if i in ex:
+            continue
+        d.append((sum((Fraction(float(p[t])) - q[t]) ** 2 for t in range(3)), i))
+    if not 1 <= k <= len(d):
+        raise OracleError("requested rank out of range")
+    d.sort()
+    cut = d[k - 1][0]
+    strict = [i for v, i in d if v < cut]
+    shell = sorted(i for v, i in d if v == cut)
+    choice = sorted(strict + shell[: k - len(strict)])
+    return {"cutoff_sq": cut, "strict": strict, "shell": shell, "choice": choice}

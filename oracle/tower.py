@@ -70,13 +70,24 @@ def _new_node(nodes, order, kind, level, children, covered=None):
     return len(nodes) - 1
 
 
-def gamma_forests(points, k_max: int, profile: str = "hgp_reduced") -> dict:
-    """Exhaustive Gamma_k forests for orders 1..min(k_max, n)."""
+def gamma_forests(points, k_max: int, profile: str = "hgp_reduced", vertical: bool = False):
+    """Exhaustive Gamma_k forests for orders 1..min(k_max, n).  With
+    vertical=True also returns the vertical maps: for every node of order
+    k + 1 >= 2 (index local to its order) the node of order k onto which its
+    component maps at the node's own level, closed cut (reference
+    reference/morsehgp3d_oracle/hierarchy.py build_vertical_map_family and
+    serialization.py _serialize_vertical_maps: the image of a facet F of
+    Gamma_{k+1} is the Gamma_k component of its k-subsets, which the coface
+    F joins at its own level)."""
     n = len(points)
     catalog(points, k_max, max_support=1)  # loads the exact cloud
     geo = _Geometry(n)
     forests = {}
+    history = {}   # order -> facet -> [(level, root)] in level order
+    witness = {}   # order -> node -> a facet of its component at its level
     for k in range(1, min(k_max, n) + 1):
+        hist = history.setdefault(k, {})
+        wit = witness.setdefault(k, {})
         facets = {f: geo.miniball(f)[0] for f in combinations(range(n), k)}
         cofaces = {}
         if k < n:
@@ -135,8 +146,33 @@ def gamma_forests(points, k_max: int, profile: str = "hgp_reduced") -> dict:
                 comp_facets[root] = allf
                 for f in allf:
                     root_of[f] = root
+                    hist.setdefault(f, []).append((level, root))
+                if root not in wit:
+                    wit[root] = min(allf)
         forests[k] = nodes
-    return forests
+    if not vertical:
+        return forests
+
+    def root_at(k, f, level):
+        out = None
+        for lev, root in history[k].get(f, []):
+            if lev <= level:
+                out = root
+        return out
+
+    maps = {}
+    for k in range(1, min(k_max, n)):
+        targets = []
+        for i, nd in enumerate(forests[k + 1]):
+            face = witness[k + 1][i]
+            t = root_at(k, tuple(face[:-1]), nd["level"])
+            if t is None:
+                raise DescentError("a vertical image outside the order-k forest")
+            # every k-subset of the facet lands in the same component
+            assert all(root_at(k, tuple(x for x in face if x != r), nd["level"]) == t for r in face)
+            targets.append(t)
+        maps[k + 1] = targets
+    return forests, maps
 
 
 def _descend(geo, label, minima, stats):

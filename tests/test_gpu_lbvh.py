@@ -81,3 +81,18 @@ def test_lattice_50k_snapshot_invariants():
     assert np.array_equal(nodes, ref.nodes)
     # published snapshot size of the reference at n = 50 000: 176 n - 80 bytes
     assert leaves.nbytes + nodes.nbytes == 176 * 50_000 - 80
+
+
+def test_device_snapshot_is_certified_exactly():
+    """The device snapshot passes the exact host re-certification (the
+    reference's import_certified_snapshot role) at 50 000 points and on a
+    cloud with equal-code runs."""
+    for label, pts in (("lattice50k", workloads.jittered_lattice(50_000, side=37, seed=0)),
+                       ("collisions", np.concatenate([workloads.uniform_cloud(200, seed=3),
+                                                      0.5 + np.arange(40)[:, None] * 1e-13 * np.ones((1, 3))]))):
+        print(f"certify device snapshot {label}", flush=True)
+        with mhgp.Context(0) as ctx:
+            ctx.upload(pts)
+            xyz, _ = ctx.canonical_points()
+            leaves, nodes, counters = ctx.lbvh_snapshot()
+        mhgp.lbvh_certify(xyz, leaves, nodes, counters)

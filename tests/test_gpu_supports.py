@@ -28,9 +28,15 @@ def _compare(pts, k_max, label):
     with mhgp.Context(0) as ctx:
         ctx.upload(pts)
         cat = ctx.supports(k_max)
+        shell, witness, exterior = ctx.diagnostics(len(cat.size))
     mine = _product_records(cat)
     ref_set = sorted((r.support, r.interior, r.rank, r.degenerate) for r in ref)
     assert sorted(mine) == ref_set, label
+    # shell diagnostics: shell size, least extra-shell point, exterior count
+    diag = {(r.support, r.interior): (r.shell_count, r.shell_witness, len(canon) - r.rank - (r.shell_count - len(r.support)))
+            for r in ref}
+    for i, (sup, inter, _, _) in enumerate(mine):
+        assert (int(shell[i]), int(witness[i]), int(exterior[i])) == diag[(sup, inter)], (label, sup)
     # exact level classes agree with the oracle's exact rational levels
     lev = {(r.support, r.interior): r.level for r in ref}
     exact = [lev[(s, i)] for (s, i, _, _) in mine]
@@ -98,20 +104,58 @@ def test_catalog_euler_characteristic_lattice_5k():
         assert _euler(cat, len(pts), k) == 1, k
 
 
-def test_capacity_error_is_clean_and_context_survives():
-    """Anchors whose candidate region exceeds the global-memory path's bound
-    report MHGP_E_CAPACITY (no fault, no poisoning); the same context then
-    serves an ordinary cloud."""
+@pytest.mark.parametrize("seed,n,bits", [(5, 100, 2), (6, 110, 3), (7, 110, 12)])
+def test_catalog_snapped_mixture_small(seed, n, bits):
+    """Snapped anisotropic mixtures with noise (the large-cloud stress
+    distribution at oracle size): coarse grids force exact cospherical and
+    coplanar ties, extra-shell diagnostics and host settlement."""
+    pts = workloads.gaussian_mixture(n, blobs=3, noise_fraction=0.15, box=2.0, snap_bits=bits, seed=seed)
+    cat = _compare(pts, 10, f"snapped{n}-b{bits}")
+    assert cat.info.status == 0
+    if bits <= 3:
+        assert cat.info.diagnostic_count > 0 or cat.info.fallback_count > 0
+
+
+def test_big_regions_give_a_conditional_catalogue():
+    """Anchors whose candidate region exceeds the enumeration bound are left
+    open: the catalogue reports status conditional with their count and
+    largest region (never exact), the tower is refused, and the same context
+    then serves an ordinary cloud exactly."""
     pts = workloads.gaussian_mixture(50_000, seed=0)
-    print("capacity case: snapped mixture n=50000 k=10", flush=True)
+    print("conditional case: snapped mixture n=50000 k=10", flush=True)
     with mhgp.Context(0) as ctx:
         ctx.upload(pts)
+        info = ctx.supports(10, fetch=False)
+        assert info.status == 1
+        assert info.anchors_open > 0 and info.largest_open_region > 2048
         with pytest.raises(mhgp.MhgpError) as err:
-            ctx.supports(10, fetch=False)
-        assert err.value.code == mhgp.E_CAPACITY, err.value
+            ctx.tower(10)
+        assert err.value.code == mhgp.E_UNSUPPORTED
         small = workloads.uniform_cloud(400, seed=9)
         ctx.upload(small)
         cat = ctx.supports(4)
+        assert cat.info.status == 0 and cat.info.anchors_open == 0
+    with mhgp.Context(0) as fresh:
+        fresh.upload(small)
+        ref = fresh.supports(4)
+    assert np.array_equal(cat.ids, ref.ids) and np.array_equal(cat.level_class, ref.level_class)
+
+
+def test_context_capacity_refuses_and_recovers():
+    """A context created with a device capacity refuses a catalogue whose
+    record queues would exceed it (MHGP_E_CAPACITY, no poisoning); after an
+    upload the same context serves a small cloud within the bound."""
+    pts = workloads.jittered_lattice(20_000, side=28, sigma=0.1, seed=3)
+    print("capacity: 20k lattice under a 64 MiB context", flush=True)
+    with mhgp.Context(0, capacity_bytes=64 << 20) as ctx:
+        ctx.upload(pts)
+        with pytest.raises(mhgp.MhgpError) as err:
+            ctx.supports(10, fetch=False)
+        assert err.value.code == mhgp.E_CAPACITY
+        small = workloads.uniform_cloud(300, seed=2)
+        ctx.upload(small)
+        cat = ctx.supports(4)
+        assert ctx.device_bytes() <= 64 << 20
     with mhgp.Context(0) as fresh:
         fresh.upload(small)
         ref = fresh.supports(4)

@@ -49,6 +49,60 @@ def _signatures(t, levels):
     return out
 
 
+def _node_sigs(nodes):
+    """Per node (local index) its signature (order, kind, level, children)."""
+    memo = {}
+
+    def sig(i):
+        if i not in memo:
+            nd = nodes[i]
+            memo[i] = (nd["order"], nd["kind"], nd["level"], tuple(sorted(sig(c) for c in nd["children"])))
+        return memo[i]
+
+    return [sig(i) for i in range(len(nodes))]
+
+
+def _product_vertical(t, levels):
+    """Vertical assignments of the product as (source, target) signatures,
+    with the invariants every vertical map satisfies: the target's level is at
+    most the source's, and a child's target lies below its parent's target."""
+    kinds = {0: "birth", 1: "birth", 2: "merge"}
+    off = t["child_offset"]
+    n = len(t["order"])
+    order = [int(x) for x in t["order"]]
+    local, per = {}, {}
+    for i in range(n):
+        per.setdefault(order[i], []).append(i)
+    nodes = {}
+    for k, idx in per.items():
+        remap = {old: new for new, old in enumerate(idx)}
+        for old in idx:
+            local[old] = (k, remap[old])
+        nodes[k] = [{"order": k, "kind": kinds[int(t["kind"][old])], "level": levels[int(t["level_class"][old])],
+                     "children": tuple(remap[int(c)] for c in t["children"][off[old]:off[old + 1]])} for old in idx]
+    sigs = {k: _node_sigs(v) for k, v in nodes.items()}
+    parent = {}
+    for i in range(n):
+        for c in t["children"][off[i]:off[i + 1]]:
+            parent[int(c)] = i
+    vert = t["vertical"]
+    pairs = []
+    for i in range(n):
+        if order[i] == 1:
+            assert vert[i] == -1
+            continue
+        tg = int(vert[i])
+        assert order[tg] == order[i] - 1
+        assert t["level_class"][tg] <= t["level_class"][i]
+        if i in parent:  # naturality: the parent's target is an ancestor-or-self of the child's
+            x, up = tg, int(vert[parent[i]])
+            while x != up and x in parent:
+                x = parent[x]
+            assert x == up
+        pairs.append((sigs[order[i]][local[i][1]], sigs[order[tg]][local[tg][1]]))
+    return sorted(pairs)
+
+
 def _check(pts, k_max, label, gamma=False):
     print(f"tower case {label} n={len(pts)} k={k_max}", flush=True)
     canon = oracle.canonicalize(pts)
@@ -58,9 +112,19 @@ def _check(pts, k_max, label, gamma=False):
         t = ctx.tower(k_max)
     levels = _exact_level_of_class(cat, canon, k_max)
     mine = _signatures(t, levels)
-    ref = otower.gamma_forests(canon, k_max, "hgp_reduced") if gamma else otower.direct_forests(canon, k_max)
+    pairs = _product_vertical(t, levels)
+    if gamma:
+        ref, maps = otower.gamma_forests(canon, k_max, "hgp_reduced", vertical=True)
+    else:
+        ref = otower.direct_forests(canon, k_max)
     for k in range(1, min(k_max, len(pts)) + 1):
         assert mine.get(k, []) == otower.signatures(ref[k]), (label, k)
+    if gamma:
+        want = []
+        for k, targets in maps.items():
+            ss, ts = _node_sigs(ref[k]), _node_sigs(ref[k - 1])
+            want += [(ss[i], ts[tg]) for i, tg in enumerate(targets)]
+        assert pairs == sorted(want), (label, "vertical maps")
 
 
 def test_tower_matches_gamma_small():
@@ -97,3 +161,74 @@ def test_tower_shared_levels_translated_copies():
 def test_tower_lattice90():
     """Enough minima per order that most saddles only grow a component."""
     _check(workloads.jittered_lattice(90, side=5, seed=44), 5, "lattice90")
+
+
+def test_vertical_growth_target_regression():
+    """tests/fixtures/regressions/vertical_q1_growth_target.json: the order-3
+    component at level 4 maps onto the order-2 root born at level 1 covering
+    {0, 1, 2}, evaluated in its state at level 4 (it has grown by then)."""
+    import json, os
+    from fractions import Fraction
+    with open(os.path.join(os.path.dirname(__file__), "golden", "regressions.json")) as fh:
+        fx = json.load(fh)["vertical_q1_growth_target"]
+    w = fx["vertical_witness"]
+    pts = np.asarray(fx["points"], dtype=float)
+    _check(pts, fx["k_max"], "vertical-q1-growth", gamma=True)
+    canon = oracle.canonicalize(pts)
+    with mhgp.Context(0) as ctx:
+        ctx.upload(pts)
+        cat = ctx.supports(fx["k_max"])
+        t = ctx.tower(fx["k_max"])
+    levels = _exact_level_of_class(cat, canon, fx["k_max"])
+    at = Fraction(w["at_squared_level"]["numerator"], w["at_squared_level"]["denominator"])
+    born = Fraction(w["target_birth_squared_level"]["numerator"], w["target_birth_squared_level"]["denominator"])
+    src = [i for i in range(len(t["order"])) if t["order"][i] == w["source_order"]
+           and levels[int(t["level_class"][i])] == at]
+    assert src, "no source node at the witnessed level"
+    tgt = int(t["vertical"][src[0]])
+    assert t["order"][tgt] == w["target_order"] and levels[int(t["level_class"][tgt])] == born
+
+
+def test_order_one_cuts_equal_emst_components():
+    """Strict and closed cuts of the order-one forest: the components equal
+    the union-find of the exact EMST edges of level (d^2 / 4) below or at the
+    cut (reference K1CompactForest::cut_strict / cut_closed)."""
+    from fractions import Fraction
+    pts = workloads.uniform_cloud(300, seed=52)
+    canon = oracle.canonicalize(pts)
+    print("tower cuts: uniform n=300 k=3", flush=True)
+    with mhgp.Context(0) as ctx:
+        ctx.upload(pts)
+        cat = ctx.supports(3)
+        t = ctx.tower(3)
+        levels = _exact_level_of_class(cat, canon, 3)
+        edges = oracle.emst(canon)
+        d2 = [sum((Fraction(float(canon[u][a])) - Fraction(float(canon[v][a]))) ** 2 for a in range(3)) / 4
+              for u, v in edges]
+        classes = sorted(set(int(c) for c in t["level_class"][t["order"] == 1]))
+        for cls in [-1] + classes[:: max(1, len(classes) // 12)]:
+            lev = levels[cls]
+            for closed in (True, False):
+                roots, lab = ctx.tower_cut(1, cls, closed, node_count=len(t["order"]))
+                parent = list(range(len(canon)))
+
+                def find(x):
+                    while parent[x] != x:
+                        parent[x] = parent[parent[x]]
+                        x = parent[x]
+                    return x
+
+                for (u, v), w in zip(edges, d2):
+                    if (w <= lev) if closed else (w < lev):
+                        parent[find(int(u))] = find(int(v))
+                if not closed and lev == 0:
+                    assert (lab == -1).all() and len(roots) == 0
+                    continue
+                groups = {}
+                for p in range(len(canon)):
+                    groups.setdefault(find(p), set()).add(p)
+                mine = {}
+                for p in range(len(canon)):
+                    mine.setdefault(int(lab[p]), set()).add(p)
+                assert sorted(map(sorted, groups.values())) == sorted(map(sorted, mine.values())), (cls, closed)
+                assert sorted(mine) == sorted(int(r) for r in roots)

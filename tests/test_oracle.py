@@ -1,6 +1,7 @@
 """Pins the CPU oracle against the reference's own fixtures and closed forms."""
 
 import json
+import struct
 import os
 from fractions import Fraction
 
@@ -10,6 +11,8 @@ import pytest
 import oracle
 from helpers import workloads
 from oracle import tower
+
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -220,3 +223,33 @@ def test_emst_graph_equals_complete_graph_kruskal():
     u, v = np.triu_indices(60, 1)
     keep = ~((u == full[0, 0]) & (v == full[0, 1]))
     assert not np.array_equal(oracle.emst_graph(pts, np.stack([u[keep], v[keep]], 1)), full)
+
+
+def test_knn_partition_oracle_pins():
+    """The exact top-k partition oracle against the reference's
+    gpu_fp64_tie_split fixture (two points one ulp apart, a rational query
+    half way: an exact tie that FP64 splits) and closed forms."""
+    with open(os.path.join(HERE, "golden", "regressions.json"), encoding="utf-8") as fh:
+        fx = json.load(fh)["gpu_fp64_tie_split"]
+    pts = np.array([[float.fromhex(x) if x.startswith("0x") else struct.unpack(">d", bytes.fromhex(x))[0]
+                     for x in p] for p in fx["points_binary64"]])
+    qd = fx["query"]
+    num = [int(qd["x_numerator"]), int(qd["y_numerator"]), int(qd["z_numerator"])]
+    r = oracle.knn_partition(pts, num, int(qd["denominator"]), fx["requested_rank"])
+    cut = fx["expected_exact_cutoff"]
+    assert r["cutoff_sq"] == Fraction(int(cut["numerator"]), int(cut["denominator"]))
+    assert r["shell"] == fx["expected_exact_shell_ids"] and r["strict"] == []
+    assert r["choice"] == [0]
+    # the centre of a unit cube of an integer grid: eight corners at 3/4
+    g = np.stack(np.meshgrid(np.arange(3), np.arange(3), np.arange(3), indexing="ij"), -1).reshape(-1, 3)
+    c = oracle.canonicalize(g.astype(float))
+    r = oracle.knn_partition(c, [1, 1, 1], 2, 3)
+    assert r["cutoff_sq"] == Fraction(3, 4) and len(r["shell"]) == 8 and r["strict"] == []
+    assert r["choice"] == r["shell"][:3]
+    # a grid point itself: strictly below it the point, then six at distance 1
+    r = oracle.knn_partition(c, [1, 1, 1], 1, 4)
+    assert r["cutoff_sq"] == 1 and len(r["strict"]) == 1 and len(r["shell"]) == 6
+    # excluding the point moves every neighbour one rank up
+    pid = r["strict"][0]
+    r2 = oracle.knn_partition(c, [1, 1, 1], 1, 4, exclusions=[pid])
+    assert r2["cutoff_sq"] == 1 and r2["strict"] == [] and r2["shell"] == r["shell"]
