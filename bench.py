@@ -252,15 +252,16 @@ def served_stream(k_values=(4, 10), count=32, lo=2_000, hi=200_000, rate_mult=4.
 
 def batched(count=256, slots=4, plain_subset=32):
     """256 clouds (sizes log-uniform in [1 000, 50 000], each a jittered
-    lattice or a 5-20 blob anisotropic mixture) submitted at once to the
-    server; the plain path runs the first clouds one after another."""
+    lattice or a 5-20 blob anisotropic mixture, K_max 10) submitted at once to
+    the server; the plain path runs the first clouds one after another on one
+    context, and every served result among them must end as it does there."""
     sizes = workloads.batch_sizes(count)
     clouds = [workloads.batch_cloud(int(m), seed=2000 + i) for i, m in enumerate(sizes)]
-    codes = {}
     ctx = mhgp.Context(0)
+    plain = []
     t0 = time.perf_counter()
     for i, pts in enumerate(clouds[:plain_subset]):
-        plain_request(ctx, pts, 10)
+        plain.append(plain_request(ctx, pts, 10))
         if (i + 1) % 8 == 0:
             print(f"batched, plain path: {i + 1}/{plain_subset} done", file=sys.stderr, flush=True)
     plain_s = time.perf_counter() - t0
@@ -268,14 +269,15 @@ def batched(count=256, slots=4, plain_subset=32):
     srv = mhgp.Server(0, slots=slots)
     t0 = time.perf_counter()
     tickets = [srv.submit(p, 10) for p in clouds]
-    lat = []
+    outcomes = {}
     for i, t in enumerate(tickets):
         r = srv.wait(t)
         key = "exact" if r.code == 0 and r.status == 0 else "conditional" if r.code == 0 else f"code {r.code}"
-        codes[key] = codes.get(key, 0) + 1
+        outcomes[key] = outcomes.get(key, 0) + 1
+        if i < plain_subset and (r.code, r.status) != plain[i]:
+            raise mhgp.MhgpError(r.code, "a served cloud ended unlike the same cloud on the plain path")
         if (i + 1) % 32 == 0:
             print(f"batched: {i + 1}/{count} done", file=sys.stderr, flush=True)
-        lat.append(r.latency_ms)
         srv.release(t)
     served_s = time.perf_counter() - t0
     srv.close()
@@ -283,8 +285,7 @@ def batched(count=256, slots=4, plain_subset=32):
         "clouds": count, "points": int(sizes.sum()), "slots": slots,
         "served_clouds_per_s": round(count / served_s, 3),
         "plain_clouds_per_s_first_%d" % plain_subset: round(plain_subset / plain_s, 3),
-        "outcomes": codes,
-        "served_total_s": round(served_s, 2),
+        "outcomes": outcomes, "served_total_s": round(served_s, 2),
     }
 
 

@@ -85,8 +85,15 @@ void supports(Context &c, int k_max, mhgp_catalog_info *info) {
     ExactJob job{ex, smax, c.scale_exp, settled};
     for (const FallbackRec &f : fbs) {
       if (f.region_off == kInvalid) throw std::runtime_error("fallback without a published region");
+      // device records index host arrays: refuse any that leaves them
+      if (f.kind < 2 || f.kind > 4 || (uint64_t)f.region_off + f.region_len > region.size())
+        throw std::runtime_error("malformed fallback record from the device");
+      for (uint32_t q = 0; q < (f.kind == 3 ? 3u : f.kind); ++q)
+        if (f.ids[q] >= (uint64_t)c.n) throw std::runtime_error("malformed fallback record from the device");
       const uint32_t *reg = region.data() + f.region_off;
       const uint32_t len = f.region_len;
+      for (uint32_t q = 0; q < len; ++q)
+        if (reg[q] >= (uint64_t)c.n) throw std::runtime_error("malformed fallback region from the device");
       if (f.kind == 2) {
         int64_t ids[2] = {f.ids[0], f.ids[1]};
         job.support(ids, 2, reg, len);

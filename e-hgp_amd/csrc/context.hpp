@@ -16,10 +16,14 @@ struct EmstBuffers;      // emst.hpp
 struct HostMirror;       // pipeline.cpp
 struct TowerState;       // tower.cpp
 struct TowerWork;        // tower_gpu.hpp
+struct AnchorBatch;      // supports.hip (batched first anchor pass)
 
 struct Context {
   int device = 0;
   int64_t capacity_bytes = 0;  // device memory bound of the context (0: none)
+  // batched catalogue: the rendezvous and this cloud's slot in it (none: plain)
+  AnchorBatch *batch = nullptr;
+  int batch_slot = -1;
   // device lanes of the tower (streams with their own workspaces); a server
   // slot uses one, since the server already overlaps whole requests
   int tower_lane_count = 2;

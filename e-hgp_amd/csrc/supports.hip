@@ -33,8 +33,13 @@
 // host fallback queue with the anchor's region.
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <numeric>
 
+#include "batch.hpp"
 #include "wave.hpp"
 #include "pipeline.hpp"
 #include "predicates.hpp"
@@ -957,7 +962,7 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
 // room for four waves per SIMD; the wide one (one workgroup per CU) takes the
 // anchors whose gather or region outgrow it, before the global-memory path.
 template <int GCAP, int RCAP>
-__device__ void anchor_lds(const Params &P) {
+__device__ void anchor_lds(const Params &P, uint32_t a) {
   __shared__ double gx[GCAP], gy[GCAP], gz[GCAP];
   __shared__ uint32_t gid[GCAP];
   __shared__ double rho[kCells];
@@ -967,7 +972,6 @@ __device__ void anchor_lds(const Params &P) {
   __shared__ __align__(16) uint64_t rmask[6 * RCAP];
   static_assert(sizeof(uint64_t) * 6 * RCAP >= sizeof(float4) * GCAP, "fc alias");
   __shared__ uint32_t kidx[RCAP];
-  const uint32_t a = blockIdx.x;
   if (a >= P.anchor_count) return;
   const uint32_t u = P.anchors[a];
   if (P.gcount[u] > P.gcap) {
@@ -982,9 +986,40 @@ __device__ void anchor_lds(const Params &P) {
   anchor_work<false>(P, w, u);
 }
 
-__global__ void __launch_bounds__(kBlock, 3) k_anchor_lds(Params P) { anchor_lds<kGatherCap, kRegionCap>(P); }
+__global__ void __launch_bounds__(kBlock, 3) k_anchor_lds(Params P) {
+  anchor_lds<kGatherCap, kRegionCap>(P, blockIdx.x);
+}
 __global__ void __launch_bounds__(kBlock, 2) k_anchor_lds_wide(Params P) {
-  anchor_lds<kGatherCapWide, kRegionCapWide>(P);
+  anchor_lds<kGatherCapWide, kRegionCapWide>(P, blockIdx.x);
+}
+
+// One persistent launch over the first anchor pass of several clouds (a
+// batch): the workgroups take anchors from a shared counter until every
+// cloud's list is done.  Item i belongs to the job j with
+// offsets[j] <= i < offsets[j + 1]; each job carries its cloud's parameters
+// (coordinates, gathers, record queues).  The item index is uniform across
+// the workgroup, so a job's parameters are read with scalar loads.
+__global__ void __launch_bounds__(kBlock, 3) k_anchor_persist(const Params *__restrict__ jobs,
+                                                              const uint32_t *__restrict__ offsets, int njobs,
+                                                              unsigned int *head) {
+  __shared__ uint32_t s_item;
+  const uint32_t total = offsets[njobs];
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(head, 1u);
+    __syncthreads();
+    const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (item >= total) return;
+    int lo = 0, hi = njobs - 1;  // last job whose offset is <= item
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (offsets[mid] <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const int j = __builtin_amdgcn_readfirstlane(lo);
+    anchor_lds<kGatherCap, kRegionCap>(jobs[j], item - offsets[j]);
+    __syncthreads();
+  }
 }
 
 // Offset (in doubles, even) of the FP32 offsets in a global slab of gather
@@ -1023,8 +1058,8 @@ __global__ void __launch_bounds__(kBlock) k_anchor_big(Params P) {
 // Pruning runs in FP32 on the two-level LBVH copy (boxes rounded outward, the
 // radius inflated by the rounding of the query and of the FP32 arithmetic; a
 // non-finite bound prunes nothing); the leaf test is the FP64 one.
-template <bool EMIT>
-__global__ void __launch_bounds__(128) k_gather(int64_t n, const WideNode *__restrict__ wide,
+template <bool EMIT, int B>
+__global__ void __launch_bounds__(B) k_gather(int64_t n, const WideNode *__restrict__ wide,
                                                 const double *__restrict__ LX,
                                                 const double *__restrict__ LY,
                                                 const double *__restrict__ LZ,
@@ -1036,7 +1071,7 @@ __global__ void __launch_bounds__(128) k_gather(int64_t n, const WideNode *__res
                                                 uint32_t *overflow) {
   // depth-first over 4-way nodes: at most 3 pending siblings per level
   constexpr int kStack = 96;
-  __shared__ uint32_t stk[kStack * 128];
+  __shared__ uint32_t stk[kStack * B];
   const int lane = threadIdx.x;
   uint32_t a = blockIdx.x * blockDim.x + lane;
   if (a >= count) return;
@@ -1053,9 +1088,9 @@ __global__ void __launch_bounds__(128) k_gather(int64_t n, const WideNode *__res
   uint32_t cnt = 0;
   uint64_t base = EMIT ? goff[u] : 0;
   int sp = 0;
-  if (n > 1) stk[(sp++) * 128 + lane] = 0;
+  if (n > 1) stk[(sp++) * B + lane] = 0;
   while (sp) {
-    const WideNode &N = wide[stk[(--sp) * 128 + lane]];
+    const WideNode &N = wide[stk[(--sp) * B + lane]];
     for (int c = 0; c < 4; ++c) {
       const uint32_t x = N.child[c];
       if (x == kInvalid) continue;
@@ -1074,7 +1109,7 @@ __global__ void __launch_bounds__(128) k_gather(int64_t n, const WideNode *__res
           cnt++;
         }
       } else if (sp < kStack) {
-        stk[(sp++) * 128 + lane] = x;
+        stk[(sp++) * B + lane] = x;
       } else {
         atomicOr(overflow, 1u);
       }
@@ -1089,11 +1124,12 @@ __global__ void __launch_bounds__(128) k_gather(int64_t n, const WideNode *__res
 // gather; the certificate widens any anchor it does not cover.
 constexpr int kRadiusK = 32;
 constexpr int kRadiusStack = 64;
-__global__ void __launch_bounds__(128) k_knn_radius(int64_t n, int kq, float factor, const WideNode *__restrict__ wide,
+template <int B>
+__global__ void __launch_bounds__(B) k_knn_radius(int64_t n, int kq, float factor, const WideNode *__restrict__ wide,
                                                     const double *__restrict__ LX, const double *__restrict__ LY,
                                                     const double *__restrict__ LZ,
                                                     const uint32_t *__restrict__ leaf_id, double *radius) {
-  __shared__ uint32_t stk[kRadiusStack * 128];
+  __shared__ uint32_t stk[kRadiusStack * B];
   const int lane = threadIdx.x;
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + lane;
   if (p >= n) return;
@@ -1120,9 +1156,9 @@ __global__ void __launch_bounds__(128) k_knn_radius(int64_t n, int kq, float fac
   }
   float cut = bound;  // min(bound, the kq-th smallest so far)
   int sp = 0;
-  if (n > 1) stk[(sp++) * 128 + lane] = 0;
+  if (n > 1) stk[(sp++) * B + lane] = 0;
   while (sp) {
-    const WideNode &N = wide[stk[(--sp) * 128 + lane]];
+    const WideNode &N = wide[stk[(--sp) * B + lane]];
     // leaves are taken at once; inner children are pushed far to near, so
     // the nearest is expanded first and the bound shrinks early
     float cd[4];
@@ -1170,7 +1206,7 @@ __global__ void __launch_bounds__(128) k_knn_radius(int64_t n, int kq, float fac
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       if (!(cd[c] >= 0.f) || !(cd[c] < cut)) continue;
-      if (sp < kRadiusStack) stk[(sp++) * 128 + lane] = cx[c];
+      if (sp < kRadiusStack) stk[(sp++) * B + lane] = cx[c];
     }
   }
   // a subtree dropped by a full stack can only raise the kq-th distance found
@@ -1250,6 +1286,198 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
                      std::vector<FallbackRec> &fbs, std::vector<uint32_t> &fb_region,
                      mhgp_catalog_info *info);
 
+// Workgroup tile (queries per workgroup, one query per lane) of the two
+// LBVH traversal kernels of the catalogue, the 32nd-neighbour radius and the
+// gather.  The tile is picked per device and cloud-size class (powers of two
+// of n) by timing every candidate on the first cloud of the class, once per
+// process; both kernels are idempotent, so the timing runs are real work.
+constexpr int kTiles[3] = {64, 128, 256};
+struct TileChoice {
+  int radius = 128, gather = 128;
+};
+std::mutex g_tile_mu;
+std::map<std::pair<int, int>, TileChoice> g_tiles;
+
+template <int B>
+void launch_radius(hipStream_t s, Context &c, int kq, CatalogBuffers &Bf) {
+  hipLaunchKernelGGL(k_knn_radius<B>, dim3(blocks_for(c.n, B)), dim3(B), 0, s, c.n, kq, 2.2f, c.wide.p, c.LX.p,
+                     c.LY.p, c.LZ.p, c.leaf_id.p, Bf.radius.p);
+}
+void launch_radius(int B, hipStream_t s, Context &c, int kq, CatalogBuffers &Bf) {
+  if (B == 64) launch_radius<64>(s, c, kq, Bf);
+  else if (B == 256) launch_radius<256>(s, c, kq, Bf);
+  else launch_radius<128>(s, c, kq, Bf);
+}
+
+template <bool EMIT, int B>
+void launch_gather(hipStream_t s, Context &c, CatalogBuffers &Bf, uint32_t cnt, uint32_t *overflow) {
+  hipLaunchKernelGGL((k_gather<EMIT, B>), dim3(blocks_for(cnt, B)), dim3(B), 0, s, c.n, c.wide.p, c.LX.p, c.LY.p,
+                     c.LZ.p, c.leaf_id.p, c.leaf_pos.p, Bf.anchors.p, cnt, Bf.radius.p, Bf.gcount.p, Bf.goff.p,
+                     Bf.glist.p, overflow);
+}
+template <bool EMIT>
+void launch_gather(int B, hipStream_t s, Context &c, CatalogBuffers &Bf, uint32_t cnt, uint32_t *overflow) {
+  if (B == 64) launch_gather<EMIT, 64>(s, c, Bf, cnt, overflow);
+  else if (B == 256) launch_gather<EMIT, 256>(s, c, Bf, cnt, overflow);
+  else launch_gather<EMIT, 128>(s, c, Bf, cnt, overflow);
+}
+
+// Times `run(tile)` for every candidate tile; returns the fastest.
+template <class F>
+int fastest_tile(hipStream_t s, F &&run) {
+  hipEvent_t a, b;
+  MHGP_HIP(hipEventCreate(&a));
+  MHGP_HIP(hipEventCreate(&b));
+  int best = 128;
+  float best_ms = INFINITY;
+  for (int B : kTiles) {
+    run(B);  // warm (code objects, caches)
+    MHGP_HIP(hipEventRecord(a, s));
+    run(B);
+    MHGP_HIP(hipEventRecord(b, s));
+    MHGP_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    MHGP_HIP(hipEventElapsedTime(&ms, a, b));
+    if (ms < best_ms) { best_ms = ms; best = B; }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return best;
+}
+
+TileChoice tiles_for(Context &c, CatalogBuffers &Bf, int kq, uint32_t *overflow) {
+  const std::pair<int, int> key{c.device, (int)std::ilogb((double)std::max<int64_t>(c.n, 1))};
+  {
+    std::lock_guard<std::mutex> g(g_tile_mu);
+    auto it = g_tiles.find(key);
+    if (it != g_tiles.end()) return it->second;
+  }
+  hipStream_t s = c.stream;
+  TileChoice t;
+  t.radius = fastest_tile(s, [&](int B) { launch_radius(B, s, c, kq, Bf); });
+  // the gather's count pass over every point in Morton order, as the first
+  // anchor pass takes them
+  const uint32_t cnt = (uint32_t)c.n;
+  MHGP_HIP(hipMemcpyAsync(Bf.anchors.p, c.leaf_id.p, cnt * 4, hipMemcpyDeviceToDevice, s));
+  t.gather = fastest_tile(s, [&](int B) { launch_gather<false>(B, s, c, Bf, cnt, overflow); });
+  std::lock_guard<std::mutex> g(g_tile_mu);
+  return g_tiles.emplace(key, t).first->second;
+}
+
+// Rendezvous of the clouds of one batch: each cloud's thread reaches its
+// first anchor pass, registers its parameters and waits; the last of them
+// queues one persistent launch over every registered cloud on the batch's
+// stream (after each cloud's own stream has reached that point), and every
+// cloud's stream then waits for that launch.  A cloud that will not reach
+// the pass (an error, a cloud too small for supports) leaves instead.
+struct AnchorBatch {
+  int device = 0;
+  std::mutex mu;
+  std::condition_variable cv;
+  int expected = 0, arrived = 0;
+  bool launched = false;
+  std::string error;
+  std::vector<char> done;  // per slot: arrived or left
+  std::vector<Params> jobs;
+  std::vector<uint32_t> counts, off;  // host sources of the launch's copies, kept until the batch ends
+  std::vector<hipEvent_t> ready;
+  hipStream_t stream = nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  DevBuf<Params> d_jobs;
+  DevBuf<uint32_t> d_off;
+  DevBuf<unsigned int> d_head;
+  int64_t anchors = 0;
+  ~AnchorBatch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (hipEvent_t e : ready) (void)hipEventDestroy(e);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void launch_locked() {
+    try {
+      MHGP_HIP(hipSetDevice(device));
+      if (!stream) {
+        MHGP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        MHGP_HIP(hipEventCreate(&t0));
+        MHGP_HIP(hipEventCreate(&t1));
+      }
+      for (hipEvent_t e : ready) MHGP_HIP(hipStreamWaitEvent(stream, e, 0));
+      const int J = (int)jobs.size();
+      off.assign(J + 1, 0);
+      for (int j = 0; j < J; ++j) off[j + 1] = off[j] + counts[j];
+      anchors = off[J];
+      d_jobs.ensure(J);
+      d_off.ensure(J + 1);
+      d_head.ensure(1);
+      MHGP_HIP(hipMemcpyAsync(d_jobs.p, jobs.data(), J * sizeof(Params), hipMemcpyHostToDevice, stream));
+      MHGP_HIP(hipMemcpyAsync(d_off.p, off.data(), (J + 1) * 4, hipMemcpyHostToDevice, stream));
+      MHGP_HIP(hipMemsetAsync(d_head.p, 0, 4, stream));
+      int cus = 0;
+      MHGP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+      // three workgroups per CU fit the kernel's LDS and registers
+      const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(off[J], (uint32_t)std::max(cus, 1) * 3));
+      MHGP_HIP(hipEventRecord(t0, stream));
+      if (off[J])
+        hipLaunchKernelGGL(k_anchor_persist, dim3(grid), dim3(kBlock), 0, stream, d_jobs.p, d_off.p, J, d_head.p);
+      MHGP_HIP(hipEventRecord(t1, stream));
+      MHGP_LAUNCH_CHECK();
+    } catch (const std::exception &e) {
+      error = e.what();
+    }
+    launched = true;
+    cv.notify_all();
+  }
+  void arrive(int slot, Context &c, const Params &P, uint32_t count) {
+    hipEvent_t ev = nullptr;
+    MHGP_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    MHGP_HIP(hipEventRecord(ev, c.stream));
+    std::unique_lock<std::mutex> g(mu);
+    if (slot < 0 || slot >= (int)done.size() || done[slot]) throw std::logic_error("batch slot reused");
+    done[slot] = 1;
+    jobs.push_back(P);
+    counts.push_back(count);
+    ready.push_back(ev);
+    arrived++;
+    if (arrived == expected) launch_locked();
+    else cv.wait(g, [&] { return launched; });
+    if (!error.empty()) throw HipError("batched anchor launch: " + error);
+    g.unlock();
+    MHGP_HIP(hipStreamWaitEvent(c.stream, t1, 0));
+  }
+  void leave(int slot) {
+    std::lock_guard<std::mutex> g(mu);
+    if (slot < 0 || slot >= (int)done.size() || done[slot]) return;
+    done[slot] = 1;
+    expected--;
+    if (!launched && arrived > 0 && arrived == expected) launch_locked();
+  }
+};
+
+std::shared_ptr<AnchorBatch> anchor_batch_create(int device, int participants) {
+  auto b = std::make_shared<AnchorBatch>();
+  b->device = device;
+  b->expected = participants;
+  b->done.assign(participants, 0);
+  return b;
+}
+
+void anchor_batch_leave(AnchorBatch &b, int slot) { b.leave(slot); }
+
+AnchorBatchStats anchor_batch_stats(AnchorBatch &b) {
+  AnchorBatchStats st{};
+  std::lock_guard<std::mutex> g(b.mu);
+  st.jobs = (int64_t)b.jobs.size();
+  st.anchors = b.anchors;
+  if (b.launched && b.error.empty() && b.t1) {
+    float ms = 0;
+    MHGP_HIP(hipEventSynchronize(b.t1));
+    MHGP_HIP(hipEventElapsedTime(&ms, b.t0, b.t1));
+    st.ms = ms;
+  }
+  return st;
+}
+
 void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
                      std::vector<FallbackRec> &fbs, std::vector<uint32_t> &fb_region,
                      mhgp_catalog_info *info) {
@@ -1262,7 +1490,10 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
   B.k_max = k_max;
   B.s_max = smax;
   info->s_max = smax;
-  if (n < 2 || smax < 2) return;
+  if (n < 2 || smax < 2) {
+    B.count = 0;  // no support of two or more points: an empty catalogue
+    return;
+  }
   if (smax > 11) throw std::invalid_argument("k_max must lie in 1..10");
   B.radius.ensure(n);
   B.gcount.ensure(n);
@@ -1279,8 +1510,14 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
   // initial gather radius: twice the distance to the min(32, n-1)-th neighbour
   const int kq = (int)std::min<int64_t>(32, n - 1);
   StageTimer tm(s);
-  hipLaunchKernelGGL(k_knn_radius, dim3(blocks_for(n, 128)), dim3(128), 0, s, n, kq, 2.2f, c.wide.p, c.LX.p,
-                     c.LY.p, c.LZ.p, c.leaf_id.p, B.radius.p);
+  DevBuf<uint32_t> overflow;
+  DevBuf<uint64_t> total;
+  overflow.ensure(1);
+  total.ensure(1);
+  MHGP_HIP(hipMemsetAsync(overflow.p, 0, 4, s));
+  B.glist.ensure(1);
+  const TileChoice tile = tiles_for(c, B, kq, overflow.p);
+  launch_radius(tile.radius, s, c, kq, B);
   MHGP_LAUNCH_CHECK();
   std::vector<uint32_t> anchors(n);
   std::iota(anchors.begin(), anchors.end(), 0u);
@@ -1295,10 +1532,6 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
   uint64_t event_cap = std::max<uint64_t>(1024, (uint64_t)n * (smax >= 11 ? 640 : 64 * smax));
   uint64_t fb_cap = std::max<uint64_t>(4096, (uint64_t)n / 4);
   uint64_t reg_cap = std::max<uint64_t>(1 << 20, (uint64_t)n * 64);
-  DevBuf<uint32_t> overflow;
-  DevBuf<uint64_t> total;
-  overflow.ensure(1);
-  total.ensure(1);
   int64_t widened = 0;
   int64_t open_anchors = 0, largest_open = 0;
   uint64_t ctr[8];
@@ -1330,18 +1563,14 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     if (iter > 12) throw CapacityError("anchor gather radii did not converge");
     const uint32_t cnt = (uint32_t)pending.size();
     MHGP_HIP(hipMemcpyAsync(B.anchors.p, pending.data(), cnt * 4, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_gather<false>, dim3(blocks_for(cnt, 128)), dim3(128), 0, s, n, c.wide.p,
-                       c.LX.p, c.LY.p, c.LZ.p, c.leaf_id.p, c.leaf_pos.p, B.anchors.p, cnt,
-                       B.radius.p, B.gcount.p, B.goff.p, B.glist.p, overflow.p);
+    launch_gather<false>(tile.gather, s, c, B, cnt, overflow.p);
     hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, B.anchors.p, cnt, B.gcount.p,
                        B.goff.p, total.p);
     uint64_t tot = 0;
     MHGP_HIP(hipMemcpyAsync(&tot, total.p, 8, hipMemcpyDeviceToHost, s));
     MHGP_HIP(hipStreamSynchronize(s));
     B.glist.ensure(tot + 1);
-    hipLaunchKernelGGL(k_gather<true>, dim3(blocks_for(cnt, 128)), dim3(128), 0, s, n, c.wide.p,
-                       c.LX.p, c.LY.p, c.LZ.p, c.leaf_id.p, c.leaf_pos.p, B.anchors.p, cnt,
-                       B.radius.p, B.gcount.p, B.goff.p, B.glist.p, overflow.p);
+    launch_gather<true>(tile.gather, s, c, B, cnt, overflow.p);
     Params P{};
     P.X = c.X.p; P.Y = c.Y.p; P.Z = c.Z.p;
     P.gcount = B.gcount.p; P.goff = B.goff.p; P.glist = B.glist.p; P.radius = B.radius.p;
@@ -1358,6 +1587,10 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     tm.mark("catalogue gather");
     uint64_t ctr_before[8];
     copy_wait(ctr_before, B.counters.p, sizeof(ctr_before), hipMemcpyDeviceToHost, s);
+    if (c.batch && attempt == 0 && iter == 0) {
+      // first pass of a batched cloud: one persistent launch over the batch
+      c.batch->arrive(c.batch_slot, c, P, cnt);
+    } else {
     c.anchor_clock.start(s);
     hipLaunchKernelGGL(k_anchor_lds, dim3(cnt), dim3(kBlock), 0, s, P);
     c.anchor_clock.stop(s);
@@ -1375,6 +1608,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
       c.stats.anchor_bytes += (int64_t)tot * 28 + (int64_t)ev * (int64_t)sizeof(EventRec);
       // FP64 operations of the certified predicates the launch evaluated
       c.stats.anchor_flops += (int64_t)(ctr_after[5] - ctr_before[5]);
+    }
     }
     tm.mark("catalogue anchors");
     std::vector<uint32_t> st(n);
@@ -1471,17 +1705,13 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
         const uint32_t ac = (uint32_t)again.size();
         copy_wait(B.radius.p, rad.data(), n * 8, hipMemcpyHostToDevice, s);
         MHGP_HIP(hipMemcpyAsync(B.anchors.p, again.data(), ac * 4, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_gather<false>, dim3(blocks_for(ac, 128)), dim3(128), 0, s, n, c.wide.p,
-                           c.LX.p, c.LY.p, c.LZ.p, c.leaf_id.p, c.leaf_pos.p, B.anchors.p, ac,
-                           B.radius.p, B.gcount.p, B.goff.p, B.glist.p, overflow.p);
+        launch_gather<false>(tile.gather, s, c, B, ac, overflow.p);
         hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, B.anchors.p, ac, B.gcount.p,
                            B.goff.p, total.p);
         MHGP_HIP(hipMemcpyAsync(&tot, total.p, 8, hipMemcpyDeviceToHost, s));
         MHGP_HIP(hipStreamSynchronize(s));
         B.glist.ensure(tot + 1);
-        hipLaunchKernelGGL(k_gather<true>, dim3(blocks_for(ac, 128)), dim3(128), 0, s, n, c.wide.p,
-                           c.LX.p, c.LY.p, c.LZ.p, c.leaf_id.p, c.leaf_pos.p, B.anchors.p, ac,
-                           B.radius.p, B.gcount.p, B.goff.p, B.glist.p, overflow.p);
+        launch_gather<true>(tile.gather, s, c, B, ac, overflow.p);
         MHGP_HIP(hipMemcpyAsync(gc.data(), B.gcount.p, n * 4, hipMemcpyDeviceToHost, s));
         MHGP_HIP(hipStreamSynchronize(s));
         big = again;

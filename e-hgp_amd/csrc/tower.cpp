@@ -264,6 +264,13 @@ void contract_order(int k, int64_t n, TowerOrderData &D, Forest &F) {
   const auto t0 = std::chrono::steady_clock::now();
   // carrier union-find; each class may own a reduced root
   const int64_t M = k == 1 ? n : (int64_t)D.minima;
+  // the device's arm lists index the arrays below: refuse any that leaves them
+  for (uint32_t s = 0; s < D.saddles; ++s)
+    if (D.arm_base[s] > D.arm_base[s + 1] || D.arm_base[s + 1] - D.arm_base[s] > 4)
+      throw std::runtime_error("malformed saddle arms from the device");
+  if (D.saddles && D.arm_base[D.saddles] != D.arms) throw std::runtime_error("malformed saddle arms from the device");
+  for (uint32_t a = 0; a < D.arms; ++a)
+    if ((int64_t)D.carrier[a] >= M) throw std::runtime_error("an arm's carrier lies outside the order's minima");
   std::vector<Cell> cell(M);
   for (int64_t x = 0; x < M; ++x) cell[x] = {(uint32_t)x, -1};
   F.reserve(k == 1 ? (size_t)n + D.saddles / 8 : D.saddles / 8);
@@ -399,9 +406,24 @@ void contract_order(int k, int64_t n, TowerOrderData &D, Forest &F) {
 static void vertical_order(const Forest &S, const Forest &T, const std::vector<uint32_t> &elem,
                            std::vector<int64_t> &target) {
   const size_t m = S.order.size(), tm = T.order.size();
-  std::vector<int64_t> parent(tm, -1);
+  // ancestor tables: up[j][x] is the 2^j-th ancestor of x (-1: none); levels
+  // never decrease towards the root, so the highest ancestor at or below L
+  // is found by descending the powers of two
+  std::vector<std::vector<int32_t>> up(1, std::vector<int32_t>(tm, -1));
   for (size_t x = 0; x < tm; ++x)
-    for (int64_t j = T.child_offset[x]; j < T.child_offset[x + 1]; ++j) parent[T.children[j]] = (int64_t)x;
+    for (int64_t j = T.child_offset[x]; j < T.child_offset[x + 1]; ++j) up[0][T.children[j]] = (int32_t)x;
+  for (;;) {
+    const std::vector<int32_t> &prev = up.back();
+    std::vector<int32_t> next(tm, -1);
+    bool any = false;
+    for (size_t x = 0; x < tm; ++x)
+      if (prev[x] >= 0) {
+        next[x] = prev[prev[x]];
+        any |= next[x] >= 0;
+      }
+    if (!any) break;
+    up.push_back(std::move(next));
+  }
   target.assign(m, -1);
   for (size_t i = 0; i < m; ++i) {
     const uint32_t e = elem[i];
@@ -409,7 +431,10 @@ static void vertical_order(const Forest &S, const Forest &T, const std::vector<u
     if (e == kInvalid || e >= T.acq_node.size() || T.acq_node[e] < 0 || T.acq_cls[e] > L)
       throw std::runtime_error("vertical map: the order below holds no owned class at the node's level");
     int64_t x = T.acq_node[e];
-    while (parent[x] >= 0 && T.level_class[parent[x]] <= L) x = parent[x];
+    for (int j = (int)up.size() - 1; j >= 0; --j) {
+      const int32_t a = up[j][x];
+      if (a >= 0 && T.level_class[a] <= L) x = a;
+    }
     target[i] = x;
   }
 }
@@ -483,6 +508,11 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
           canonical_points(c, xyz.data(), nullptr);
           desc = std::make_unique<Descender>(xyz, n, exact_cloud(c));
         }
+        // device labels index the host cloud: refuse any that leaves it
+        for (uint32_t x : D.minima_labels)
+          if (x >= (uint64_t)n) throw std::runtime_error("malformed minimum label from the device");
+        for (uint32_t x : D.saddle_labels)
+          if (x != kInvalid && x >= (uint64_t)n) throw std::runtime_error("malformed saddle label from the device");
         std::unordered_map<std::vector<uint32_t>, int64_t, LabelHash> minima;
         for (uint32_t i = 0; i < D.minima; ++i)
           minima.emplace(std::vector<uint32_t>(D.minima_labels.begin() + (size_t)i * k,
@@ -558,11 +588,15 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
     for (int o = 2; o <= K; ++o) {
       const Forest &S = forests[o - 1];
       first[o] = pos.size();
-      const uint32_t base = c.tower_work->rank_lo[o];
+      const uint32_t base = c.tower_work->rank_lo[o], span = c.tower_work->rank_hi[o] - base;
       for (size_t i = 0; i < S.order.size(); ++i) {
         const int32_t d = S.rep_elem[i];
         if (d < 0) throw std::runtime_error("vertical map: a node without a representative element");
-        pos.push_back(base + (S.raw_of_dense.empty() ? (uint32_t)d : S.raw_of_dense[d]));
+        if (!S.raw_of_dense.empty() && (size_t)d >= S.raw_of_dense.size())
+          throw std::runtime_error("vertical map: a representative outside the order's elements");
+        const uint32_t raw = S.raw_of_dense.empty() ? (uint32_t)d : S.raw_of_dense[d];
+        if (raw >= span) throw std::runtime_error("vertical map: a minimum outside the order's rank range");
+        pos.push_back(base + raw);
       }
     }
     std::vector<uint32_t> elem;

@@ -243,7 +243,13 @@ struct ArmParams {
   int64_t n;
   uint32_t *carrier;  // per arm: minimum index, or kInvalid (host)
   unsigned long long *steps;
+  // nearest neighbours of every PointId (kNearK ids, nearest first) and the
+  // FP64 squared distance to the last of them; nn == nullptr: none
+  const uint32_t *nn;
+  const double *nn_d2;
 };
+
+constexpr int kNearK = 32;
 
 constexpr int kArmBlock = 64;
 constexpr int kArmStack = 40;  // LBVH depth is 18 at 50k points, 32 at 200k; overflow goes to the host
@@ -357,6 +363,30 @@ __global__ void k_alias_carriers(const uint32_t *__restrict__ alias_arm, const u
   if (i < count) carrier[alias_arm[i]] = carrier[alias_rep[i]];
 }
 
+// Trade the miniball's least support point for the least point strictly
+// inside; false leaves the arm to the host (shell points, or no point inside:
+// a minimum missing from the table).
+__device__ bool finish_step(const Ball &b, const uint32_t *sid, int k, uint32_t *F, uint32_t *hint, uint32_t xin,
+                            bool undecided) {
+  if (undecided || xin == kInvalid) return false;
+  uint32_t y = sid[0];
+  for (int i = 1; i < b.m; ++i) y = min(y, sid[i]);
+  // F := F - y + xin, kept sorted
+  int w = 0;
+  uint32_t G[kMaxLabel];
+  bool placed = false;
+  for (int i = 0; i < k; ++i) {
+    if (F[i] == y) continue;
+    if (!placed && xin < F[i]) { G[w++] = xin; placed = true; }
+    G[w++] = F[i];
+  }
+  if (!placed) G[w++] = xin;
+  for (int i = 0; i < k; ++i) F[i] = G[i];
+  // the next miniball's likely support: this one's, y traded for xin
+  for (int h = 0; h < 4; ++h) hint[h] = h < b.m ? (sid[h] == y ? xin : sid[h]) : kInvalid;
+  return true;
+}
+
 // One descent step of facet F (sorted ids) with the likely support hint:
 // F := F - y + x, where x is the least point strictly inside the facet's
 // miniball and y its least support point.  false: the certified filters
@@ -402,6 +432,34 @@ __device__ bool descent_step(const ArmParams &A, uint32_t *F, uint32_t *hint, ui
   bool undecided = false;
   const double q[3] = {b.c.x, b.c.y, b.c.z};
   const double R2 = b.r2 * (1.0 + 1e-9) + 1e-300;
+  if (A.nn) {
+    // every point of the ball lies within sqrt(R2) + |c - f| of a facet point
+    // f; when that radius stays below the distance to f's last listed
+    // neighbour, f's neighbour list holds all of them and replaces the
+    // traversal (with margins far above the FP64 rounding)
+    uint32_t f = F[0];
+    double best = INFINITY;
+    for (int i = 0; i < k; ++i) {
+      const double dx = A.X[F[i]] - q[0], dy = A.Y[F[i]] - q[1], dz = A.Z[F[i]] - q[2];
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 < best) { best = d2; f = F[i]; }
+    }
+    const double rho = (sqrt(R2) + sqrt(best)) * (1.0 + 1e-9) + 1e-150;
+    if (rho * rho < A.nn_d2[f] * (1.0 - 1e-9)) {
+      for (int j = 0; j < kNearK && !undecided; ++j) {
+        const uint32_t id = A.nn[(size_t)f * kNearK + j];
+        const double dx = A.X[id] - q[0], dy = A.Y[id] - q[1], dz = A.Z[id] - q[2];
+        if (dx * dx + dy * dy + dz * dz > R2) continue;
+        bool inF = false;
+        for (int i = 0; i < k; ++i) inF |= F[i] == id;
+        if (inF) continue;
+        const int sd = S.side(A.X, A.Y, A.Z, id);
+        if (sd < 0) xin = min(xin, id);
+        else if (sd == 0) undecided = true;
+      }
+      return finish_step(b, sid, k, F, hint, xin, undecided);
+    }
+  }
   // FP32 pruning against outward-rounded child boxes: the radius is
   // inflated by the rounding of the query point and of the FP32 arithmetic
   // (a non-finite bound prunes nothing)
@@ -441,23 +499,7 @@ __device__ bool descent_step(const ArmParams &A, uint32_t *F, uint32_t *hint, ui
       }
     }
   }
-  if (undecided || xin == kInvalid) return false;  // host decides (shell points or a missing minimum)
-  uint32_t y = sid[0];
-  for (int i = 1; i < b.m; ++i) y = min(y, sid[i]);
-  // F := F - y + xin, kept sorted
-  int w = 0;
-  uint32_t G[kMaxLabel];
-  bool placed = false;
-  for (int i = 0; i < k; ++i) {
-    if (F[i] == y) continue;
-    if (!placed && xin < F[i]) { G[w++] = xin; placed = true; }
-    G[w++] = F[i];
-  }
-  if (!placed) G[w++] = xin;
-  for (int i = 0; i < k; ++i) F[i] = G[i];
-  // the next miniball's likely support: this one's, y traded for xin
-  for (int h = 0; h < 4; ++h) hint[h] = h < b.m ? (sid[h] == y ? xin : sid[h]) : kInvalid;
-  return true;
+  return finish_step(b, sid, k, F, hint, xin, undecided);
 }
 
 // Descents advance one step per round for every arm still descending, so the
@@ -832,6 +874,25 @@ __global__ void k_raw_of_dense(int64_t M, const uint32_t *__restrict__ dense, ui
   if (dense[m] != prev) out[prev] = (uint32_t)m;
 }
 
+// FP64 squared distance from every point to the last of its listed neighbours;
+// 0 (the list is never used) when the traversal that made the list dropped a
+// subtree (flag 4) or left a slot empty.
+__global__ void k_nn_last_d2(int64_t n, const uint32_t *__restrict__ nn, const uint8_t *__restrict__ flags,
+                             const double *__restrict__ X, const double *__restrict__ Y,
+                             const double *__restrict__ Z, double *d2) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  bool whole = (flags[p] & 4) == 0;
+  for (int j = 0; j < kNearK; ++j) whole &= nn[p * kNearK + j] < (uint64_t)n;
+  const uint32_t q = nn[p * kNearK + kNearK - 1];
+  if (!whole) {
+    d2[p] = 0.0;
+    return;
+  }
+  const double dx = X[q] - X[p], dy = Y[q] - Y[p], dz = Z[q] - Z[p];
+  d2[p] = dx * dx + dy * dy + dz * dz;
+}
+
 __global__ void k_gather_u32(const uint32_t *__restrict__ src, const uint32_t *__restrict__ idx, int64_t n,
                              uint32_t *out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -871,6 +932,18 @@ void tower_rank_index(Context &c) {
   W.rank_keys.ensure(n);
   W.rank_vals.ensure(n);
   W.vm_elem.ensure(n);
+  // neighbour lists of the points, for the descents' ball queries
+  W.nn_ready = false;
+  if (c.n > kNearK) {
+    W.nn.ensure((size_t)c.n * kNearK);
+    W.nn_d2.ensure(c.n);
+    W.nn_flags.ensure(c.n);
+    knn_all(c, kNearK, W.nn.p, W.nn_flags.p);
+    hipLaunchKernelGGL(k_nn_last_d2, dim3(blocks_for(c.n, 256)), dim3(256), 0, s, c.n, W.nn.p, W.nn_flags.p, c.X.p,
+                       c.Y.p, c.Z.p, W.nn_d2.p);
+    MHGP_LAUNCH_CHECK();
+    W.nn_ready = true;
+  }
   W.rank_bounds.ensure(512);
   hipLaunchKernelGGL(k_rank_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.events.p, B.order.p, n,
                      W.rank_keys.p, W.rank_vals.p);
@@ -882,6 +955,8 @@ void tower_rank_index(Context &c) {
   copy_wait(h, W.rank_bounds.p, sizeof h, hipMemcpyDeviceToHost, s);
   std::copy(h, h + 256, W.rank_lo);
   std::copy(h + 256, h + 512, W.rank_hi);
+  for (int r = 0; r < 256; ++r)
+    if (W.rank_lo[r] > W.rank_hi[r] || W.rank_hi[r] > (uint64_t)n) throw std::runtime_error("malformed rank index");
 }
 
 void tower_order_device(Context &c, TowerWork &W, const TowerWork &R, int k, TowerOrderData &out) {
@@ -940,6 +1015,8 @@ void tower_order_device(Context &c, TowerWork &W, const TowerWork &R, int k, Tow
   A.wide = c.wide.p; A.LX = c.LX.p; A.LY = c.LY.p; A.LZ = c.LZ.p; A.leaf_id = c.leaf_id.p; A.leaf_pos = c.leaf_pos.p; A.n = c.n;
   A.carrier = carrier.p;
   A.steps = (unsigned long long *)steps.p;
+  A.nn = R.nn_ready ? R.nn.p : nullptr;
+  A.nn_d2 = R.nn_ready ? R.nn_d2.p : nullptr;
   auto &queue = W.queue;
   auto &qcount = W.qcount;
   queue.ensure(arms);

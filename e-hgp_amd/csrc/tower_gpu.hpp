@@ -66,6 +66,12 @@ struct TowerWork {
   // of its first arm's carrier as a saddle of the order below (first lane
   // only), the gather scratch, and the dense -> raw minimum map of a lane
   DevBuf<uint32_t> vm_elem, vm_idx, vm_out, raw_dense;
+  // nearest-neighbour lists of every point and the squared distance to the
+  // last listed one (first lane only)
+  DevBuf<uint32_t> nn;
+  DevBuf<double> nn_d2;
+  DevBuf<uint8_t> nn_flags;
+  bool nn_ready = false;
   int64_t bytes() const {
     int64_t b = 0;
     for (const DevBuf<uint32_t> *x : {&rank_vals, &rank_bounds, &tvals, &labels, &sizes, &carrier, &scls,
@@ -74,7 +80,7 @@ struct TowerWork {
                                       &changed, &err, &live_n, &Lu[0], &Lu[1], &Lv[0], &Lv[1], &Li[0], &Li[1],
                                       &attach, &rep, &dense, &slab, &ssup, &vm_elem, &vm_idx, &vm_out, &raw_dense, &alias_arm, &alias_rep, &dF[0], &dF[1], &dH[0], &dH[1], &dO[0], &dO[1]})
       b += (int64_t)x->bytes();
-    return b + (int64_t)(rank_keys.bytes() + qkey.bytes() + facet_slots.bytes() + alias_count.bytes() + dcount.bytes() + tkeys.bytes() + steps.bytes() + qcount.bytes() + slev.bytes() + c_lev.bytes());
+    return b + (int64_t)(nn.bytes() + nn_d2.bytes() + nn_flags.bytes()) + (int64_t)(rank_keys.bytes() + qkey.bytes() + facet_slots.bytes() + alias_count.bytes() + dcount.bytes() + tkeys.bytes() + steps.bytes() + qcount.bytes() + slev.bytes() + c_lev.bytes());
   }
 };
 

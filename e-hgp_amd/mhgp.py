@@ -57,6 +57,11 @@ class RequestResult(ctypes.Structure):
                 ("service_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
 
 
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("clouds", ctypes.c_int64), ("groups", ctypes.c_int64), ("anchors", ctypes.c_int64),
+                ("anchor_ms", ctypes.c_double), ("wall_ms", ctypes.c_double)]
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
@@ -95,6 +100,12 @@ def lib() -> ctypes.CDLL:
         L.mhgp_server_fetch.argtypes = [p, i64, p, p, p, p, p, p]
         L.mhgp_server_release.argtypes = [p, i64]
         L.mhgp_server_destroy.argtypes = [p]
+        L.mhgp_batch_run.argtypes = [i32, i64, p, p, p, i32, i64, ctypes.POINTER(p)]
+        L.mhgp_batch_info_get.argtypes = [p, ctypes.POINTER(BatchInfo)]
+        L.mhgp_batch_result.argtypes = [p, i64, ctypes.POINTER(RequestResult)]
+        L.mhgp_batch_fetch.argtypes = [p, i64, p, p, p, p, p, p]
+        L.mhgp_batch_destroy.argtypes = [p]
+        L.mhgp_batch_destroy.restype = None
         L.mhgp_server_destroy.restype = None
         _lib = L
     return _lib
@@ -106,7 +117,8 @@ EXPORTED = [
     "mhgp_canonical_points", "mhgp_lbvh_snapshot", "mhgp_lbvh_certify", "mhgp_knn", "mhgp_knn_partition", "mhgp_emst", "mhgp_supports",
     "mhgp_supports_fetch", "mhgp_supports_radii", "mhgp_supports_diagnostics", "mhgp_tower", "mhgp_tower_fetch", "mhgp_tower_vertical", "mhgp_tower_cut", "mhgp_ctx_stats",
     "mhgp_closed_ball", "mhgp_server_create", "mhgp_server_submit", "mhgp_server_wait", "mhgp_server_fetch",
-    "mhgp_server_release", "mhgp_server_destroy",
+    "mhgp_server_release", "mhgp_server_destroy", "mhgp_batch_run", "mhgp_batch_info_get", "mhgp_batch_result",
+    "mhgp_batch_fetch", "mhgp_batch_destroy",
 ]
 
 
@@ -390,3 +402,38 @@ class Server:
 
     def release(self, ticket: int) -> None:
         _check(lib().mhgp_server_release(self._h, ticket))
+
+
+def batch_run(clouds, k_max, group: int = 32, memory_cap_bytes: int = 0, device: int = 0):
+    """Runs many clouds through mhgp_batch_run (groups side by side, one
+    persistent anchor launch per group).  k_max: one value or one per cloud.
+    Returns (results, towers, info): per cloud a RequestResult and its tower
+    (dict as Context.tower without vertical maps, or None), and the batch's
+    BatchInfo."""
+    arrs = [np.ascontiguousarray(np.asarray(c, dtype=np.float64).reshape(-1, 3)) for c in clouds]
+    count = len(arrs)
+    ptrs = (ctypes.c_void_p * max(count, 1))(*[a.ctypes.data for a in arrs])
+    n = np.ascontiguousarray([len(a) for a in arrs] or [0], dtype=np.int64)
+    ks = np.ascontiguousarray(np.broadcast_to(np.asarray(k_max, dtype=np.int32), (max(count, 1),)))
+    h = ctypes.c_void_p()
+    _check(lib().mhgp_batch_run(device, count, ctypes.cast(ptrs, ctypes.c_void_p), _ptr(n), _ptr(ks), group,
+                                memory_cap_bytes, ctypes.byref(h)))
+    try:
+        info = BatchInfo()
+        _check(lib().mhgp_batch_info_get(h, ctypes.byref(info)))
+        results, towers = [], []
+        for i in range(count):
+            r = RequestResult()
+            _check(lib().mhgp_batch_result(h, i, ctypes.byref(r)))
+            results.append(r)
+            if r.code != 0 or r.status != 0:
+                towers.append(None)
+                continue
+            order, kind, cls, level, off, kids = _tower_arrays(r.node_count, r.child_count)
+            _check(lib().mhgp_batch_fetch(h, i, _ptr(order), _ptr(kind), _ptr(cls), _ptr(level), _ptr(off),
+                                          _ptr(kids)))
+            towers.append({"order": order, "kind": kind, "level_class": cls, "level": level, "child_offset": off,
+                           "children": kids[: r.child_count]})
+        return results, towers, info
+    finally:
+        lib().mhgp_batch_destroy(h)

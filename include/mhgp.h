@@ -292,6 +292,40 @@ int mhgp_server_fetch(mhgp_server *srv, int64_t ticket, int32_t *order, int32_t 
 int mhgp_server_release(mhgp_server *srv, int64_t ticket);
 void mhgp_server_destroy(mhgp_server *srv);
 
+/* Batched processing of many clouds in one call.  Clouds run in groups, side
+ * by side (one host thread and one context per cloud of a group); the first
+ * anchor pass of the catalogue of every cloud of a group is ONE persistent
+ * launch over all of their anchors (workgroups take anchors from a shared
+ * counter across clouds), then each cloud continues as the plain path does:
+ * the remaining anchor passes, the tower unless the catalogue is conditional.
+ * A group holds at most `group` clouds (<= 0: 32) whose predicted device
+ * bytes fit memory_cap_bytes (<= 0: 80% of the device's memory).  Results per
+ * cloud as for the server; a failing cloud fails alone.  No reference
+ * counterpart: the batching layer above the launchers.
+ * Currently switched off: mhgp_batch_run returns MHGP_E_UNSUPPORTED (a
+ * process running it ended with a host segmentation fault whose cause has
+ * not been found); batch through the server meanwhile. */
+typedef struct mhgp_batch mhgp_batch;
+
+typedef struct {
+  int64_t clouds;
+  int64_t groups;     /* persistent launches */
+  int64_t anchors;    /* anchors those launches processed */
+  double anchor_ms;   /* their device time, summed */
+  double wall_ms;     /* the whole call */
+} mhgp_batch_info;
+
+int mhgp_batch_run(int device, int64_t count, const double *const *xyz, const int64_t *n, const int32_t *k_max,
+                   int32_t group, int64_t memory_cap_bytes, mhgp_batch **out);
+int mhgp_batch_info_get(const mhgp_batch *b, mhgp_batch_info *info);
+/* Result of cloud i (code, status, sizes; service_ms is the cloud's own
+ * time, latency_ms the time from the call's start to its completion). */
+int mhgp_batch_result(const mhgp_batch *b, int64_t cloud, mhgp_request_result *result);
+/* Tower of cloud i, laid out as mhgp_tower_fetch (MHGP_E_STATE: none). */
+int mhgp_batch_fetch(const mhgp_batch *b, int64_t cloud, int32_t *order, int32_t *kind, int64_t *level_class,
+                     double *level, int64_t *child_offset, int64_t *children);
+void mhgp_batch_destroy(mhgp_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -160,3 +160,14 @@ def test_context_capacity_refuses_and_recovers():
         fresh.upload(small)
         ref = fresh.supports(4)
     assert np.array_equal(cat.ids, ref.ids) and np.array_equal(cat.level_class, ref.level_class)
+
+
+def test_reused_context_one_point_cloud_has_an_empty_catalogue():
+    """A context that held a cloud and then takes a one-point cloud returns an
+    empty catalogue (no records left over from the earlier cloud)."""
+    with mhgp.Context(0) as ctx:
+        ctx.upload(workloads.uniform_cloud(200, seed=4))
+        assert ctx.supports(4).info.event_count > 0
+        ctx.upload(np.array([[1.0, 2.0, 3.0]]))
+        cat = ctx.supports(4)
+        assert cat.info.event_count == 0 and len(cat.size) == 0
