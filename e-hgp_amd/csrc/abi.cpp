@@ -257,3 +257,9 @@ extern "C" int mhgp_ctx_stats(mhgp_ctx *ctx, mhgp_stats *stats, int reset) {
   if (reset) ctx->c.stats = mhgp_stats{};
   return MHGP_OK;
 }
+
+extern "C" int mhgp_ctx_count_flops(mhgp_ctx *ctx, int on) {
+  if (!ctx) return fail(nullptr, MHGP_E_ARGUMENT, "null context", false);
+  ctx->c.count_flops = on != 0;
+  return MHGP_OK;
+}

@@ -88,6 +88,10 @@ __global__ void __launch_bounds__(64) k_closed_ball(BallParams B) {
   const double c[3] = {o.x + off.x, o.y + off.y, o.z + off.z};
   const double R2 = r2 * (1.0 + 1e-6) + 1e-300;  // every point beyond is certainly outside
   uint32_t nin = 0, nsh = (uint32_t)m;
+  // the interior row keeps the cap smallest ids (their largest and its slot
+  // tracked), so a truncated list does not depend on the traversal order
+  uint32_t *row = B.interior + (size_t)q * B.cap;
+  uint32_t rmax = 0, rmax_at = 0;
   uint32_t stack[kBallStack];
   int sp = 0;
   stack[sp++] = 0;
@@ -107,7 +111,15 @@ __global__ void __launch_bounds__(64) k_closed_ball(BallParams B) {
         const V3 p = sub(pt(B, id), o);
         const int sd = m == 2 ? ps.side(p) : (m == 3 ? ts.side(p) : tt.side(p));
         if (sd < 0) {
-          if (nin < B.cap) B.interior[(size_t)q * B.cap + nin] = id;
+          if (nin < B.cap) {
+            row[nin] = id;
+            if (id >= rmax) { rmax = id; rmax_at = nin; }
+          } else if (B.cap && id < rmax) {
+            row[rmax_at] = id;
+            rmax = 0;
+            for (uint32_t k = 0; k < B.cap; ++k)
+              if (row[k] >= rmax) { rmax = row[k]; rmax_at = k; }
+          }
           nin++;
         } else if (sd == 0) {
           const unsigned long long at = atomicAdd(B.nund, 1ull);
@@ -126,6 +138,19 @@ __global__ void __launch_bounds__(64) k_closed_ball(BallParams B) {
   }
   B.nin[q] = nin;
   B.nshell[q] = nsh;
+}
+
+// Adds id to a row holding the `cap` smallest interior ids seen so far
+// (`k` of them in use): appended while there is room, else it replaces the
+// row's largest id when smaller.
+void keep_smallest(uint32_t *row, uint32_t k, uint32_t cap, uint32_t id) {
+  if (k < cap) {
+    row[k] = id;
+    return;
+  }
+  if (!cap) return;
+  uint32_t *mx = std::max_element(row, row + cap);
+  if (id < *mx) *mx = id;
 }
 
 }  // namespace
@@ -194,7 +219,7 @@ void closed_ball(Context &c, const uint32_t *ids, const uint8_t *sizes, int64_t 
     return *ex;
   };
   auto add_inside = [&](int64_t q, uint32_t id) {
-    if (nin[q] < cap) inter[(size_t)q * cap + nin[q]] = id;
+    keep_smallest(inter.data() + (size_t)q * cap, std::min(nin[q], cap), cap, id);
     nin[q]++;
   };
   if (!und.empty()) {

@@ -29,6 +29,7 @@ struct Context {
   int tower_lane_count = 2;
   hipStream_t stream = nullptr;
   bool poisoned = false;
+  bool count_flops = false;  // FP64 operation count of the anchor kernel (stats.anchor_flops)
   std::string last_error;
 
   int64_t n = 0;

@@ -8,6 +8,7 @@
 // ordered exactly on the host by (exact level, rank, support ids) and
 // scattered back.  A device scan turns the class breaks into dense classes.
 #include <algorithm>
+#include <cstdio>
 #include <exception>
 #include <numeric>
 #include <thread>
@@ -282,6 +283,9 @@ void order_catalogue(Context &c, CatalogBuffers &B) {
                        B.order.p, B.cls.p);
     MHGP_HIP(hipStreamSynchronize(s));
   }
+  if (tm.on)
+    std::fprintf(stderr, "[mhgp] level runs: %llu open, %llu exact on the host\n", (unsigned long long)runs,
+                 (unsigned long long)nh);
   tm.mark("catalogue level runs (host exact)");
   c.scanner.inclusive(B.cls.p, n, s);
   hipLaunchKernelGGL(k_class_from_scan, dim3(blocks_for(n, 256)), dim3(256), 0, s, B.cls.p, n);

@@ -100,6 +100,7 @@ struct Params {
   double *scratch;
   uint64_t slab_doubles;
   uint32_t gcap, rcap;
+  int count_flops;               // 1: count the FP64 operations of the predicates (counters[5])
 };
 
 struct Work {
@@ -111,7 +112,7 @@ struct Work {
   uint32_t *kidx;
   float4 *fc;       // FP32 offset and distance per gathered point (certificate and masks)
   uint32_t *cand;   // candidate supports classified (LDS, flushed once per anchor)
-  unsigned long long *flops;  // FP64 operations of the certified predicates (LDS, flushed once per anchor)
+  unsigned long long *flops;  // FP64 operations of the certified predicates (LDS, flushed once per anchor; null: not counted)
 };
 
 // FP64 operations (add, subtract, multiply; a fused multiply-add counts two)
@@ -125,8 +126,10 @@ constexpr uint32_t kFlopSweepKey = 24;   // sweep_key
 constexpr uint32_t kFlopTriInit = 66;    // TriSphere::init
 constexpr uint32_t kFlopTetInit = 150;   // TetSphere::init
 
+// Counting is off unless the launch asks for it (flops == nullptr): the
+// timed path pays for one uniform branch per call.
 __device__ inline void add_flops(const Work &w, uint64_t v) {
-  if (v) atomicAdd(w.flops, (unsigned long long)v);
+  if (w.flops && v) atomicAdd(w.flops, (unsigned long long)v);
 }
 
 __device__ inline V3 gpt(const Work &w, uint32_t i) { return {w.gx[i], w.gy[i], w.gz[i]}; }
@@ -653,21 +656,6 @@ __device__ bool member_mask(const Params &P, const Work &w, uint32_t i, float rm
 
 constexpr int kCoverCap = 10;  // s_max - 1 <= 10
 
-// Cube-map cell (6 faces x 8 x 8) of a direction, row-major on each face.
-__device__ inline uint32_t direction_cell(double x, double y, double z) {
-  const float fx = (float)x, fy = (float)y, fz = (float)z;
-  const float ax = fabsf(fx), ay = fabsf(fy), az = fabsf(fz);
-  int face;
-  float a, b, m;
-  if (ax >= ay && ax >= az) { face = fx >= 0.f ? 0 : 1; m = ax; a = fy; b = fz; }
-  else if (ay >= az) { face = fy >= 0.f ? 2 : 3; m = ay; a = fz; b = fx; }
-  else { face = fz >= 0.f ? 4 : 5; m = az; a = fx; b = fy; }
-  const float inv = m > 0.f ? 1.0f / m : 0.f;
-  const int i = min(7, max(0, (int)((a * inv + 1.0f) * 4.0f)));
-  const int j = min(7, max(0, (int)((b * inv + 1.0f) * 4.0f)));
-  return (uint32_t)(face * 64 + i * 8 + j);
-}
-
 // BIG: arrays in global slabs, regions beyond LDS; the face phase works in
 // place instead of through the wave queues.
 template <bool BIG>
@@ -872,23 +860,13 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     }
     __syncthreads();
   }
-  // candidate vertices (ids above the anchor) ordered by the direction cell
-  // of their offset: a wave takes faces (kidx[i], kidx[j]) with one j and
-  // consecutive i, so neighbouring lanes get faces whose pencil capsules
-  // overlap and take the heavy branches of the scans together.  The order
-  // does not change which faces exist.  ridx is free once the points are
-  // compacted; it holds cell << 16 | slot until the rank sort.
+  // candidate vertices (ids above the anchor); their order only decides
+  // which lane takes which face (a rank sort by direction cell, to give
+  // neighbouring lanes overlapping pencil capsules, measured slower)
   for (uint32_t s = t; s < M; s += blockDim.x)
-    if (w.gid[s] > u) w.ridx[atomicAdd(&sK, 1u)] = (direction_cell(w.gx[s], w.gy[s], w.gz[s]) << 16) | s;
+    if (w.gid[s] > u) w.kidx[atomicAdd(&sK, 1u)] = s;
   __syncthreads();
   const uint32_t Kc = sK;
-  for (uint32_t k = t; k < Kc; k += blockDim.x) {
-    const uint32_t key = w.ridx[k];
-    uint32_t rank = 0;
-    for (uint32_t j = 0; j < Kc; ++j) rank += w.ridx[j] < key;
-    w.kidx[rank] = key & 0xFFFFu;
-  }
-  __syncthreads();
   // 3. pairs (u, v), v > u
   for (uint32_t k = t; k < Kc; k += blockDim.x) process_pair(P, w, u, M, w.kidx[k], &sregion);
   // 4. faces (u, v, w), u < v < w, whose members share a cell.  Each wave
@@ -953,7 +931,7 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   __syncthreads();
   if (t == 0) {
     if (*w.cand) atomicAdd(&P.counters[3], (unsigned long long)*w.cand);
-    if (*w.flops) atomicAdd(&P.counters[5], *w.flops);
+    if (w.flops && *w.flops) atomicAdd(&P.counters[5], *w.flops);
     P.status[u] = ST_DONE;
   }
 }
@@ -982,7 +960,8 @@ __device__ void anchor_lds(const Params &P, uint32_t a) {
   __shared__ unsigned long long flops;
   if (threadIdx.x == 0) { cand = 0; flops = 0; }
   __syncthreads();
-  Work w{gx, gy, gz, gid, rho, ridx, rmask, kidx, reinterpret_cast<float4 *>(rmask), &cand, &flops};
+  Work w{gx, gy, gz, gid, rho, ridx, rmask, kidx, reinterpret_cast<float4 *>(rmask), &cand,
+         P.count_flops ? &flops : nullptr};
   anchor_work<false>(P, w, u);
 }
 
@@ -1050,7 +1029,7 @@ __global__ void __launch_bounds__(kBlock) k_anchor_big(Params P) {
   if (threadIdx.x == 0) { cand = 0; flops = 0; }
   __syncthreads();
   w.cand = &cand;
-  w.flops = &flops;
+  w.flops = P.count_flops ? &flops : nullptr;
   anchor_work<true>(P, w, u);
 }
 
@@ -1584,6 +1563,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     P.counters = (unsigned long long *)B.counters.p;
     P.anchors = B.anchors.p; P.anchor_count = cnt;
     P.big = 0; P.gcap = kGatherCap; P.rcap = kRegionCap;
+    P.count_flops = c.count_flops ? 1 : 0;
     tm.mark("catalogue gather");
     uint64_t ctr_before[8];
     copy_wait(ctr_before, B.counters.p, sizeof(ctr_before), hipMemcpyDeviceToHost, s);

@@ -406,9 +406,48 @@ void contract_order(int k, int64_t n, TowerOrderData &D, Forest &F) {
 static void vertical_order(const Forest &S, const Forest &T, const std::vector<uint32_t> &elem,
                            std::vector<int64_t> &target) {
   const size_t m = S.order.size(), tm = T.order.size();
-  // ancestor tables: up[j][x] is the 2^j-th ancestor of x (-1: none); levels
-  // never decrease towards the root, so the highest ancestor at or below L
-  // is found by descending the powers of two
+  target.assign(m, -1);
+  auto start = [&](size_t i) -> int64_t {
+    const uint32_t e = elem[i];
+    if (e == kInvalid || e >= T.acq_node.size() || T.acq_node[e] < 0 || T.acq_cls[e] > S.level_class[i])
+      throw std::runtime_error("vertical map: the order below holds no owned class at the node's level");
+    return T.acq_node[e];
+  };
+  // Nodes are created in level order, so node ids ascend with their level
+  // classes; a node's parent has a larger id and a level at least its own.
+  // Sweep the queries by level: before answering level L, every node of the
+  // order below at level <= L absorbs its children's sets, so the set of a
+  // node is headed by its highest ancestor at or below L.
+  bool sorted_t = true;
+  for (size_t x = 1; x < tm && sorted_t; ++x) sorted_t = T.level_class[x] >= T.level_class[x - 1];
+  if (sorted_t) {
+    std::vector<uint32_t> q(m);
+    std::iota(q.begin(), q.end(), 0u);
+    bool sorted_s = true;
+    for (size_t i = 1; i < m && sorted_s; ++i) sorted_s = S.level_class[i] >= S.level_class[i - 1];
+    if (!sorted_s)
+      std::stable_sort(q.begin(), q.end(), [&](uint32_t x, uint32_t y) { return S.level_class[x] < S.level_class[y]; });
+    std::vector<uint32_t> head(tm);
+    std::iota(head.begin(), head.end(), 0u);
+    auto find = [&](uint32_t x) {
+      while (head[x] != x) {
+        head[x] = head[head[x]];
+        x = head[x];
+      }
+      return x;
+    };
+    size_t p = 0;
+    for (uint32_t i : q) {
+      const int64_t L = S.level_class[i];
+      for (; p < tm && T.level_class[p] <= L; ++p)
+        for (int64_t j = T.child_offset[p]; j < T.child_offset[p + 1]; ++j) head[T.children[j]] = (uint32_t)p;
+      target[i] = find((uint32_t)start(i));
+    }
+    return;
+  }
+  // otherwise: ancestor tables, up[j][x] the 2^j-th ancestor of x (-1: none);
+  // levels never decrease towards the root, so the highest ancestor at or
+  // below L is found by descending the powers of two
   std::vector<std::vector<int32_t>> up(1, std::vector<int32_t>(tm, -1));
   for (size_t x = 0; x < tm; ++x)
     for (int64_t j = T.child_offset[x]; j < T.child_offset[x + 1]; ++j) up[0][T.children[j]] = (int32_t)x;
@@ -424,13 +463,9 @@ static void vertical_order(const Forest &S, const Forest &T, const std::vector<u
     if (!any) break;
     up.push_back(std::move(next));
   }
-  target.assign(m, -1);
   for (size_t i = 0; i < m; ++i) {
-    const uint32_t e = elem[i];
     const int64_t L = S.level_class[i];
-    if (e == kInvalid || e >= T.acq_node.size() || T.acq_node[e] < 0 || T.acq_cls[e] > L)
-      throw std::runtime_error("vertical map: the order below holds no owned class at the node's level");
-    int64_t x = T.acq_node[e];
+    int64_t x = start(i);
     for (int j = (int)up.size() - 1; j >= 0; --j) {
       const int32_t a = up[j][x];
       if (a >= 0 && T.level_class[a] <= L) x = a;

@@ -1,6 +1,8 @@
 // Host multiprecision fallback of the MI355X backend (see exact.hpp).
 #include "exact.hpp"
 
+#include <thread>
+
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -176,12 +178,24 @@ void ExactCloud::load(const double *xyz, int64_t n) {
     any = true;
   }
   e0_ = emin;
-  P_.resize(3 * n);
-  for (int64_t i = 0; i < 3 * n; ++i) {
-    int64_t s; int e;
-    decode_binary64(xyz[i], s, e);
-    P_[i] = s ? BigInt::from_dyadic(s, e - emin) : BigInt();
+  xyz_ = xyz;
+  P_.assign(3 * n, BigInt());
+  state_.reset(new std::atomic<uint8_t>[n]);
+  for (int64_t i = 0; i < n; ++i) state_[i].store(0, std::memory_order_relaxed);
+}
+
+void ExactCloud::make(int64_t i) const {
+  uint8_t expect = 0;
+  if (state_[i].compare_exchange_strong(expect, 1, std::memory_order_acq_rel)) {
+    for (int a = 0; a < 3; ++a) {
+      int64_t s; int e;
+      decode_binary64(xyz_[3 * i + a], s, e);
+      P_[3 * i + a] = s ? BigInt::from_dyadic(s, e - e0_) : BigInt();
+    }
+    state_[i].store(2, std::memory_order_release);
+    return;
   }
+  while (state_[i].load(std::memory_order_acquire) != 2) std::this_thread::yield();
 }
 
 BigInt ExactCloud::sq_dist(int64_t i, int64_t j) const {

@@ -8,7 +8,9 @@
 // plays in the reference, morsehgp3d/include/morsehgp3d/exact/integer.hpp:13).
 #pragma once
 
+#include <atomic>
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 namespace mhgp {
@@ -39,22 +41,30 @@ class BigInt {
 };
 
 // Exact cloud in canonical PointId order, coordinates = integer * 2^e0.
+// The integer coordinates of a point are made on its first use (most exact
+// work touches few points); coord() may be called from several threads.
 class ExactCloud {
  public:
   ExactCloud() = default;
-  void load(const double *xyz, int64_t n);  // xyz interleaved, canonical order
+  void load(const double *xyz, int64_t n);  // xyz interleaved, canonical order; kept by reference
   bool loaded() const { return n_ > 0; }
   int64_t size() const { return n_; }
-  const BigInt &coord(int64_t i, int a) const { return P_[3 * i + a]; }
+  const BigInt &coord(int64_t i, int a) const {
+    if (state_[i].load(std::memory_order_acquire) != 2) make(i);
+    return P_[3 * i + a];
+  }
   BigInt sq_dist(int64_t i, int64_t j) const;
   // sign(|q-i|^2 - |q-j|^2)
   int cmp_dist(int64_t q, int64_t i, int64_t j) const;
   int e0() const { return e0_; }
 
  private:
+  void make(int64_t i) const;
   int64_t n_ = 0;
   int e0_ = 0;
-  std::vector<BigInt> P_;
+  const double *xyz_ = nullptr;
+  mutable std::vector<BigInt> P_;
+  mutable std::unique_ptr<std::atomic<uint8_t>[]> state_;  // per point: 0 none, 1 in progress, 2 made
 };
 
 // Exact support analysis: circumsphere with centre in the affine hull of

@@ -93,6 +93,7 @@ def lib() -> ctypes.CDLL:
         L.mhgp_tower_vertical.argtypes = [p, p]
         L.mhgp_tower_cut.argtypes = [p, i32, i64, i32, p, p, p]
         L.mhgp_ctx_stats.argtypes = [p, ctypes.POINTER(Stats), i32]
+        L.mhgp_ctx_count_flops.argtypes = [p, i32]
         L.mhgp_closed_ball.argtypes = [p, p, p, i64, ctypes.c_uint32, p, p, p]
         L.mhgp_server_create.argtypes = [i32, i32, i64, ctypes.POINTER(p)]
         L.mhgp_server_submit.argtypes = [p, p, i64, i32, ctypes.POINTER(i64)]
@@ -115,7 +116,7 @@ EXPORTED = [
     "mhgp_last_error", "mhgp_ctx_create", "mhgp_ctx_destroy", "mhgp_ctx_device_bytes",
     "mhgp_cloud_upload", "mhgp_cloud_upload_device", "mhgp_point_count",
     "mhgp_canonical_points", "mhgp_lbvh_snapshot", "mhgp_lbvh_certify", "mhgp_knn", "mhgp_knn_partition", "mhgp_emst", "mhgp_supports",
-    "mhgp_supports_fetch", "mhgp_supports_radii", "mhgp_supports_diagnostics", "mhgp_tower", "mhgp_tower_fetch", "mhgp_tower_vertical", "mhgp_tower_cut", "mhgp_ctx_stats",
+    "mhgp_supports_fetch", "mhgp_supports_radii", "mhgp_supports_diagnostics", "mhgp_tower", "mhgp_tower_fetch", "mhgp_tower_vertical", "mhgp_tower_cut", "mhgp_ctx_stats", "mhgp_ctx_count_flops",
     "mhgp_closed_ball", "mhgp_server_create", "mhgp_server_submit", "mhgp_server_wait", "mhgp_server_fetch",
     "mhgp_server_release", "mhgp_server_destroy", "mhgp_batch_run", "mhgp_batch_info_get", "mhgp_batch_result",
     "mhgp_batch_fetch", "mhgp_batch_destroy",
@@ -189,6 +190,10 @@ class Context:
         st = Stats()
         _check(lib().mhgp_ctx_stats(self._h, ctypes.byref(st), 1 if reset else 0))
         return st
+
+    def count_flops(self, on: bool) -> None:
+        """FP64 operation count of the anchor kernel (Stats.anchor_flops)."""
+        _check(lib().mhgp_ctx_count_flops(self._h, 1 if on else 0))
 
     def upload(self, points) -> None:
         a = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))

@@ -135,8 +135,9 @@ int mhgp_emst(mhgp_ctx *ctx, uint32_t *edges_out);
  * sphere is their circumsphere with centre in their affine hull.
  * interior_count[q]: points strictly inside; shell_count[q]: points on the
  * sphere, the support included; exterior = n - interior - shell.
- * interior_ids: cap per query, ascending, UINT32_MAX padded; complete when
- * interior_count[q] <= cap.  Exact: undecided points are settled on the host. */
+ * interior_ids: cap per query, the cap smallest interior PointIds in
+ * ascending order, UINT32_MAX padded; complete when interior_count[q] <= cap.
+ * Exact: undecided points are settled on the host. */
 int mhgp_closed_ball(mhgp_ctx *ctx, const uint32_t *support_ids, const uint8_t *support_sizes,
                      int64_t queries, uint32_t cap, uint32_t *interior_ids, uint32_t *interior_count,
                      uint32_t *shell_count);
@@ -255,10 +256,14 @@ typedef struct {
   int64_t gathered_points;   /* neighbour entries gathered for the anchors */
   int64_t events;            /* catalogue records written by the device */
   int64_t anchor_flops;      /* FP64 operations of the certified predicates the anchor kernel evaluated
-                                (per evaluation: the value, its magnitude bound, the certified sign) */
+                                (per evaluation: the value, its magnitude bound, the certified sign);
+                                counted only while mhgp_ctx_count_flops is on */
 } mhgp_stats;
 
 int mhgp_ctx_stats(mhgp_ctx *ctx, mhgp_stats *stats, int reset);
+/* Turns the FP64 operation count of the anchor kernel on (1) or off (0, the
+ * default: the kernel then skips the per-call LDS atomics of the count). */
+int mhgp_ctx_count_flops(mhgp_ctx *ctx, int on);
 
 /* Continuous-batching server: a pool of worker slots (each one context and
  * one stream) that runs upload -> EMST -> catalogue -> tower for submitted

@@ -35,9 +35,9 @@ def _check(pts, supports, cap=12):
         inside, shell = _expected(n, sup)
         assert nin[q] == len(inside), (q, sup)
         assert nsh[q] == shell, (q, sup)
-        if len(inside) <= cap:
-            got = [int(x) for x in inter[q] if x != 0xFFFFFFFF]
-            assert got == inside, (q, sup)
+        # the cap smallest interior ids, whether or not the list is complete
+        got = [int(x) for x in inter[q] if x != 0xFFFFFFFF]
+        assert got == inside[:cap], (q, sup)
 
 
 def _random_supports(rng, n, count):
@@ -79,3 +79,24 @@ def test_closed_ball_exact_shell_points():
     ]
     print("closed ball: 4x4x4 grid, exact shells", flush=True)
     _check(pts, sup)
+
+
+def test_closed_ball_truncated_lists_are_the_smallest_ids():
+    """Interiors far above the cap on an integer grid, with more undecided
+    shell points than the device's list holds (queries then move to the host
+    whole): every list is the cap smallest interior ids, as the oracle's."""
+    g = np.arange(7, dtype=np.float64)
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    rng = np.random.default_rng(7)
+    sups = []
+    while len(sups) < 400:
+        m = 2 + len(sups) % 3
+        sup = [int(x) for x in rng.choice(len(pts), size=m, replace=False)]
+        a = pts[sup[1:]] - pts[sup[0]]
+        if m == 3 and not np.any(np.cross(a[0], a[1]) != 0):
+            continue
+        if m == 4 and abs(np.linalg.det(a)) < 0.5:
+            continue
+        sups.append(sup)
+    print("closed ball: 7x7x7 grid, 400 supports, cap 5", flush=True)
+    _check(pts, sups, cap=5)

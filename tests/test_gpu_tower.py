@@ -194,9 +194,9 @@ def test_order_one_cuts_equal_emst_components():
     the union-find of the exact EMST edges of level (d^2 / 4) below or at the
     cut (reference K1CompactForest::cut_strict / cut_closed)."""
     from fractions import Fraction
-    pts = workloads.uniform_cloud(300, seed=52)
+    pts = workloads.uniform_cloud(80, seed=52)
     canon = oracle.canonicalize(pts)
-    print("tower cuts: uniform n=300 k=3", flush=True)
+    print("tower cuts: uniform n=80 k=3", flush=True)
     with mhgp.Context(0) as ctx:
         ctx.upload(pts)
         cat = ctx.supports(3)
