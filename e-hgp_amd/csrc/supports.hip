@@ -375,7 +375,7 @@ constexpr uint32_t kSideQ = 128;  // >= 2 items for each of a wave's 64 faces
 constexpr uint32_t kTetQ = 256;
 struct WaveQueues {
   uint32_t *side;    // vi | wi << 10 | side << 20 | cin << 21 (region slots, < 1024)
-  double *side_r;    // the face's rmax: the side pass rebuilds exactly the face pass's capsule
+  float *side_r;     // the face's rmax (a float, see face_feasible): the side pass rebuilds exactly the face pass's capsule
   uint32_t *tet;     // vi | wi << 10 | zi << 20
   uint32_t *nside, *ntet;
 };
@@ -572,7 +572,7 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
       const uint32_t at = atomicAdd(q->nside, 1u);
       if (at < kSideQ) {
         q->side[at] = vi | (wi << 10) | ((uint32_t)side << 20) | ((uint32_t)cin_side << 21);
-        q->side_r[at] = rmax;
+        q->side_r[at] = (float)rmax;  // exact: rmax is a float value
       } else {
         atomicAdd(&P.counters[4], 1ull);  // reported by the host as an internal error
       }
@@ -655,6 +655,36 @@ __device__ bool member_mask(const Params &P, const Work &w, uint32_t i, float rm
 }
 
 constexpr int kCoverCap = 10;  // s_max - 1 <= 10
+
+// Face (u, v, w) of region slots si, sj: false when v and w share no cell or
+// when the triangle's circumradius exceeds every shared cell's certificate
+// radius (well-conditioned faces only; FP64 r2 is then accurate to ~1e-10).
+// rmax, the largest radius over the shared cells, is rounded up to a float
+// value, so that the queued side passes can keep it in a float and rebuild
+// exactly the first pass's capsule (a larger bound only prunes less).
+__device__ inline bool face_feasible(const Work &w, uint32_t si, uint32_t sj, double &rmax) {
+  bool share = false;
+  for (int k = 0; k < 6; ++k) share |= (w.rmask[6 * si + k] & w.rmask[6 * sj + k]) != 0;
+  rmax = 0;
+  if (!share) return false;
+  const V3 a = gpt(w, si), b = gpt(w, sj);
+  const double aa = dot(a, a), bb = dot(b, b), ab = dot(a, b);
+  const double D = aa * bb - ab * ab;
+  const V3 amb = sub(a, b);
+  const double r2 = D > 0 ? aa * bb * dot(amb, amb) / (4.0 * D) : INFINITY;
+  for (int k = 0; k < 6; ++k) {
+    uint64_t mm = w.rmask[6 * si + k] & w.rmask[6 * sj + k];
+    while (mm) {
+      const int bit = __ffsll((long long)mm) - 1;
+      mm &= mm - 1;
+      rmax = fmax(rmax, w.rho[64 * k + bit]);
+    }
+  }
+  float rf = (float)rmax;
+  if ((double)rf < rmax) rf = nextafterf(rf, INFINITY);
+  rmax = (double)rf;
+  return !(D > 1e-6 * (aa * bb + fabs(ab) * fabs(ab)) && r2 > rmax * rmax * (1.0 + 1e-6));
+}
 
 // BIG: arrays in global slabs, regions beyond LDS; the face phase works in
 // place instead of through the wave queues.
@@ -874,7 +904,8 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   // queued sides and tetrahedra one per lane (LDS path; the global-memory
   // path works in place).
   __shared__ uint32_t q_side[kBlock / 64][kSideQ];
-  __shared__ double q_side_r[kBlock / 64][kSideQ];
+  __shared__ float q_side_r[kBlock / 64][kSideQ];
+  __shared__ uint32_t q_face[kBlock / 64][128];
   __shared__ uint32_t q_tet[kBlock / 64][kTetQ];
   __shared__ uint32_t q_n[kBlock / 64][2];
   const int wave = t >> 6, lane = t & 63, nwaves = (int)(blockDim.x >> 6);
@@ -883,50 +914,71 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   if (lane == 0) { q_n[wave][0] = 0; q_n[wave][1] = 0; }
   wave_sync();
   const uint64_t F = (uint64_t)Kc * (Kc - 1) / 2;
-  for (uint64_t base = (uint64_t)wave * 64; base < F; base += (uint64_t)nwaves * 64) {
-    const uint64_t fidx = base + lane;
-    if (fidx < F) {
-      // decode the pair index (i < j) of the strictly lower triangle
-      uint64_t j = (uint64_t)((1.0 + sqrt(1.0 + 8.0 * (double)fidx)) * 0.5);
-      while (j * (j - 1) / 2 > fidx) j--;
-      while ((j + 1) * j / 2 <= fidx) j++;
-      const uint64_t i = fidx - j * (j - 1) / 2;
-      const uint32_t si = w.kidx[i], sj = w.kidx[j];
-      bool share = false;
-      for (int k = 0; k < 6; ++k) share |= (w.rmask[6 * si + k] & w.rmask[6 * sj + k]) != 0;
-      if (share) {
-        // a sphere through u, v, w has radius >= r3; skip faces beyond every shared cell
-        const uint32_t vi = si, wi = sj;
-        const V3 a = gpt(w, vi), b = gpt(w, wi);
-        const double aa = dot(a, a), bb = dot(b, b), ab = dot(a, b);
-        const double D = aa * bb - ab * ab;
-        const V3 amb = sub(a, b);
-        const double r2 = D > 0 ? aa * bb * dot(amb, amb) / (4.0 * D) : INFINITY;
-        double rmax = 0;
-        for (int k = 0; k < 6; ++k) {
-          uint64_t mm = w.rmask[6 * si + k] & w.rmask[6 * sj + k];
-          while (mm) {
-            const int bit = __ffsll((long long)mm) - 1;
-            mm &= mm - 1;
-            rmax = fmax(rmax, w.rho[64 * k + bit]);
-          }
-        }
-        // prune only well-conditioned faces (FP64 r2 is then accurate to ~1e-10)
-        if (!(D > 1e-6 * (aa * bb + fabs(ab) * fabs(ab)) && r2 > rmax * rmax * (1.0 + 1e-6)))
-          process_face<!BIG>(P, w, u, M, vi, wi, rmax, &sregion, q);
-      }
+  // candidate face fidx -> region slots of (v, w): the pair (i < j) of the
+  // strictly lower triangle
+  auto face_of = [&](uint64_t fidx, uint32_t &si, uint32_t &sj) {
+    uint64_t j = (uint64_t)((1.0 + sqrt(1.0 + 8.0 * (double)fidx)) * 0.5);
+    while (j * (j - 1) / 2 > fidx) j--;
+    while ((j + 1) * j / 2 <= fidx) j++;
+    const uint64_t i = fidx - j * (j - 1) / 2;
+    si = w.kidx[i];
+    sj = w.kidx[j];
+  };
+  if (BIG) {
+    for (uint64_t base = (uint64_t)wave * 64; base < F; base += (uint64_t)nwaves * 64) {
+      const uint64_t fidx = base + lane;
+      if (fidx >= F) continue;
+      uint32_t si, sj;
+      face_of(fidx, si, sj);
+      double rmax;
+      if (face_feasible(w, si, sj, rmax)) process_face<false>(P, w, u, M, si, sj, rmax, &sregion, q);
     }
-    if (BIG) continue;
-    wave_sync();
-    const uint32_t ns = min(q_n[wave][0], kSideQ);
-    for (uint32_t it = lane; it < ns; it += 64)
-      side_item(P, w, u, M, q_side[wave][it], q_side_r[wave][it], &sregion, q);
-    wave_sync();
-    const uint32_t nt = min(q_n[wave][1], kTetQ);
-    for (uint32_t it = lane; it < nt; it += 64) tet_item(P, w, u, M, q_tet[wave][it], &sregion);
-    wave_sync();
-    if (lane == 0) { q_n[wave][0] = 0; q_n[wave][1] = 0; }
-    wave_sync();
+  } else {
+    // Each wave first collects up to 64 feasible faces in its LDS list (the
+    // faces that fail the shared-cell or radius test cost only the test),
+    // then runs their first passes one per lane, then the queued sides and
+    // tetrahedra.  The lists keep region slots only; a lane recomputes the
+    // face's rmax (same arithmetic, same value).
+    uint32_t nf = 0;  // faces in the wave's list (uniform)
+    uint64_t cbase = (uint64_t)wave * 64;
+    const uint64_t cstride = (uint64_t)nwaves * 64;
+    for (;;) {
+      while (nf < 64 && cbase < F) {
+        const uint64_t fidx = cbase + lane;
+        bool ok = false;
+        uint32_t si = 0, sj = 0;
+        if (fidx < F) {
+          face_of(fidx, si, sj);
+          double rmax;
+          ok = face_feasible(w, si, sj, rmax);
+        }
+        const uint64_t m = __ballot(ok);
+        if (ok) q_face[wave][nf + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = si | (sj << 10);
+        nf += (uint32_t)__popcll(m);
+        cbase += cstride;
+      }
+      if (nf == 0) break;
+      wave_sync();
+      const uint32_t take = min(nf, 64u);
+      nf -= take;
+      if ((uint32_t)lane < take) {
+        const uint32_t fv = q_face[wave][nf + lane];
+        const uint32_t si = fv & 1023u, sj = fv >> 10;
+        double rmax;
+        face_feasible(w, si, sj, rmax);
+        process_face<true>(P, w, u, M, si, sj, rmax, &sregion, q);
+      }
+      wave_sync();
+      const uint32_t ns = min(q_n[wave][0], kSideQ);
+      for (uint32_t it = lane; it < ns; it += 64)
+        side_item(P, w, u, M, q_side[wave][it], (double)q_side_r[wave][it], &sregion, q);
+      wave_sync();
+      const uint32_t nt = min(q_n[wave][1], kTetQ);
+      for (uint32_t it = lane; it < nt; it += 64) tet_item(P, w, u, M, q_tet[wave][it], &sregion);
+      wave_sync();
+      if (lane == 0) { q_n[wave][0] = 0; q_n[wave][1] = 0; }
+      wave_sync();
+    }
   }
   __syncthreads();
   if (t == 0) {
