@@ -196,6 +196,18 @@ int mhgp_closed_ball(mhgp_ctx *ctx, const uint32_t *support_ids, const uint8_t *
   });
 }
 
+int mhgp_closed_ball_noalloc(mhgp_ctx *ctx, const uint32_t *support_ids, const uint8_t *support_sizes,
+                             int64_t queries, uint32_t cap, uint32_t *interior_ids, uint32_t *interior_count,
+                             uint32_t *shell_count) {
+  return guarded(ctx ? &ctx->c : nullptr, [&] {
+    mhgp::closed_ball_noalloc(ctx->c, support_ids, support_sizes, queries, cap, interior_ids, interior_count,
+                              shell_count);
+    return MHGP_OK;
+  });
+}
+
+int64_t mhgp_closed_ball_noalloc_scratch_bytes(void) { return (int64_t)mhgp::closed_ball_scratch_bytes(); }
+
 int mhgp_supports_fetch(mhgp_ctx *ctx, uint32_t *ids, uint32_t *meta, uint32_t *interior,
                         double *level, int64_t *level_class) {
   return guarded(ctx ? &ctx->c : nullptr, [&] {

@@ -10,6 +10,7 @@
 // cannot decide (points exactly or nearly on the sphere, ill-conditioned
 // supports) is settled on the host in exact arithmetic.
 #include <algorithm>
+#include <mutex>
 
 #include "../host/exact.hpp"
 #include "pipeline.hpp"
@@ -140,6 +141,89 @@ __global__ void __launch_bounds__(64) k_closed_ball(BallParams B) {
   B.nshell[q] = nsh;
 }
 
+// Allocation-free variant: the host-query flag of each query is folded into
+// the high bit of its interior count (counts stay below 2^31 since n does),
+// so the host reads both with one copy into the caller's own array.
+__global__ void __launch_bounds__(256) k_fold_flags(uint32_t *nin, const uint32_t *flags, int64_t queries) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < queries && (flags[q] & kHostQuery)) nin[q] |= 0x80000000u;
+}
+
+__global__ void k_zero_u64(unsigned long long *x) { *x = 0ull; }
+
+// The undecided (query, point) pairs as sortable keys query << 32 | point,
+// padded with the largest key up to the power of two the sort works on.
+__global__ void k_und_keys(uint64_t *und, uint64_t count, uint64_t padded) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= padded) return;
+  if (i < count) {
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(und + i);
+    und[i] = ((uint64_t)w[0] << 32) | w[1];
+  } else {
+    und[i] = ~0ull;
+  }
+}
+
+// One compare-exchange stage (size k, distance j) of a bitonic sort, ascending.
+__global__ void k_bitonic_stage(uint64_t *a, uint64_t n, uint64_t j, uint64_t k) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t l = i ^ j;
+  if (i >= n || l <= i) return;
+  const uint64_t x = a[i], y = a[l];
+  const bool up = (i & k) == 0;
+  if (up ? x > y : x < y) {
+    a[i] = y;
+    a[l] = x;
+  }
+}
+
+// Device scratch of mhgp_closed_ball_noalloc: query staging, per-query
+// counts and flags, and the list of undecided (query, point) pairs.  Each
+// device holds its own copy until the process ends; host code reaches it
+// only through the address hipGetSymbolAddress returns on that device.
+constexpr size_t kNoAllocScratchBytes = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+// Address of this device's scratch (the calling thread's current device),
+// looked up once per device.
+unsigned char *scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) throw std::invalid_argument("device index beyond the scratch table");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    MHGP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw HipError("hipGetSymbolAddress returned no address for the closed-ball scratch");
+    g_scratch_ptr[device] = static_cast<unsigned char *>(p);
+  }
+  return g_scratch_ptr[device];
+}
+
+// Byte layout of the scratch for Q queries with `cap` interior ids each;
+// every part starts on a 256-byte boundary.
+struct NoAllocLayout {
+  size_t ids, sizes, interior, nin, nsh, flags, und, nund, end;
+  uint64_t und_cap, und_pow2;  // pairs recorded; room for the padded sort
+  NoAllocLayout(int64_t Q, uint32_t cap) {
+    auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+    und_cap = std::max<uint64_t>(1024, (uint64_t)Q * 4);
+    und_pow2 = 1;
+    while (und_pow2 < und_cap) und_pow2 <<= 1;
+    ids = 0;
+    sizes = up(ids + 16 * (size_t)Q);
+    interior = up(sizes + (size_t)Q);
+    nin = up(interior + 4 * (size_t)Q * std::max<uint32_t>(cap, 1));
+    nsh = up(nin + 4 * (size_t)Q);
+    flags = up(nsh + 4 * (size_t)Q);
+    und = up(flags + 4 * (size_t)Q);
+    nund = up(und + 8 * (size_t)und_pow2);
+    end = nund + 8;
+  }
+};
+
 // Adds id to a row holding the `cap` smallest interior ids seen so far
 // (`k` of them in use): appended while there is room, else it replaces the
 // row's largest id when smaller.
@@ -267,6 +351,149 @@ void closed_ball(Context &c, const uint32_t *ids, const uint8_t *sizes, int64_t 
       std::copy(inter.begin() + (size_t)q * cap, inter.begin() + (size_t)q * cap + k, interior + (size_t)q * cap);
       std::fill(interior + (size_t)q * cap + k, interior + (size_t)(q + 1) * cap, kInvalid);
     }
+}
+
+size_t closed_ball_scratch_bytes() { return kNoAllocScratchBytes; }
+
+// Same results as closed_ball, with every device buffer taken from the
+// file-scope scratch: no device allocation, no hipMalloc or hipFree.  The
+// rare exact settlement of undecided points runs on the host, in host memory
+// (the context's exact mirror of the cloud is built on its first use).
+void closed_ball_noalloc(Context &c, const uint32_t *ids, const uint8_t *sizes, int64_t Q, uint32_t cap,
+                         uint32_t *interior, uint32_t *nin, uint32_t *nshell) {
+  require_cloud(c);
+  if (Q < 0 || (Q > 0 && (!ids || !sizes || !nin || !nshell)) || (cap > 0 && Q > 0 && !interior))
+    throw std::invalid_argument("invalid closed-ball query arrays");
+  if (Q == 0) return;
+  for (int64_t q = 0; q < Q; ++q) {
+    const int m = sizes[q];
+    if (m < 1 || m > 4) throw std::invalid_argument("support sizes must lie in 1..4");
+    for (int i = 0; i < m; ++i) {
+      if (ids[4 * q + i] >= (uint64_t)c.n) throw std::invalid_argument("support PointId out of range");
+      for (int j = 0; j < i; ++j)
+        if (ids[4 * q + i] == ids[4 * q + j]) throw std::invalid_argument("repeated support PointId");
+    }
+  }
+  const NoAllocLayout L(Q, cap);
+  if (L.end > kNoAllocScratchBytes)
+    throw CapacityError("closed-ball queries need " + std::to_string(L.end) + " bytes of device scratch; the " +
+                        "allocation-free entry point has " + std::to_string(kNoAllocScratchBytes) +
+                        " (fewer queries or a smaller interior cap per call)");
+  unsigned char *base = scratch_ptr(c.device);
+  uint32_t *d_ids = reinterpret_cast<uint32_t *>(base + L.ids);
+  uint8_t *d_sizes = base + L.sizes;
+  uint32_t *d_int = reinterpret_cast<uint32_t *>(base + L.interior);
+  uint32_t *d_nin = reinterpret_cast<uint32_t *>(base + L.nin);
+  uint32_t *d_nsh = reinterpret_cast<uint32_t *>(base + L.nsh);
+  uint32_t *d_flags = reinterpret_cast<uint32_t *>(base + L.flags);
+  uint32_t *d_und = reinterpret_cast<uint32_t *>(base + L.und);
+  unsigned long long *d_nund = reinterpret_cast<unsigned long long *>(base + L.nund);
+  hipStream_t s = c.stream;
+  constexpr uint32_t kFlag = 0x80000000u;
+  auto add_inside = [&](int64_t q, uint32_t id) {
+    keep_smallest(interior + (size_t)q * cap, std::min(nin[q] & ~kFlag, cap), cap, id);
+    nin[q]++;
+  };
+  if (c.n > 1) {
+    MHGP_HIP(hipMemcpyAsync(d_ids, ids, 16 * (size_t)Q, hipMemcpyHostToDevice, s));
+    MHGP_HIP(hipMemcpyAsync(d_sizes, sizes, (size_t)Q, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, d_nund);
+    MHGP_LAUNCH_CHECK();
+    BallParams B{};
+    B.ids = d_ids; B.sizes = d_sizes; B.queries = Q; B.cap = cap;
+    B.X = c.X.p; B.Y = c.Y.p; B.Z = c.Z.p;
+    B.nodes = c.nodes.p; B.LX = c.LX.p; B.LY = c.LY.p; B.LZ = c.LZ.p; B.leaf_id = c.leaf_id.p; B.n = c.n;
+    B.interior = d_int; B.nin = d_nin; B.nshell = d_nsh; B.flags = d_flags;
+    B.und = d_und; B.nund = d_nund; B.und_cap = L.und_cap;
+    hipLaunchKernelGGL(k_closed_ball, dim3(blocks_for(Q, 64)), dim3(64), 0, s, B);
+    MHGP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fold_flags, dim3(blocks_for(Q, 256)), dim3(256), 0, s, d_nin, d_flags, Q);
+    MHGP_LAUNCH_CHECK();
+    uint64_t nund = 0;
+    MHGP_HIP(hipMemcpyAsync(nin, d_nin, 4 * (size_t)Q, hipMemcpyDeviceToHost, s));
+    MHGP_HIP(hipMemcpyAsync(nshell, d_nsh, 4 * (size_t)Q, hipMemcpyDeviceToHost, s));
+    if (cap) MHGP_HIP(hipMemcpyAsync(interior, d_int, 4 * (size_t)Q * cap, hipMemcpyDeviceToHost, s));
+    MHGP_HIP(hipMemcpyAsync(&nund, d_nund, 8, hipMemcpyDeviceToHost, s));
+    MHGP_HIP(hipStreamSynchronize(s));
+    nund = std::min<uint64_t>(nund, L.und_cap);
+    // settle the pairs in (query, point) order, as mhgp_closed_ball does: it
+    // keeps, of a query's undecided inside points, the smallest ids when the
+    // interior list overflows.  More pairs than one host chunk are sorted on
+    // the device, in the scratch, by a bitonic network.
+    constexpr uint64_t kChunk = 1024;
+    const bool device_sorted = nund > kChunk;
+    if (device_sorted) {
+      uint64_t padded = 1;
+      while (padded < nund) padded <<= 1;
+      uint64_t *keys = reinterpret_cast<uint64_t *>(d_und);
+      hipLaunchKernelGGL(k_und_keys, dim3(blocks_for((int64_t)padded, 256)), dim3(256), 0, s, keys, nund, padded);
+      for (uint64_t k = 2; k <= padded; k <<= 1)
+        for (uint64_t j = k >> 1; j > 0; j >>= 1)
+          hipLaunchKernelGGL(k_bitonic_stage, dim3(blocks_for((int64_t)padded, 256)), dim3(256), 0, s, keys, padded,
+                             j, k);
+      MHGP_LAUNCH_CHECK();
+    }
+    // undecided points of the device queries, settled exactly in chunks
+    // copied through a fixed host array
+    uint64_t pairs[kChunk];
+    ExactSupport S;
+    int64_t last = -1;
+    for (uint64_t at = 0; at < nund; at += kChunk) {
+      const uint64_t cnt = std::min(kChunk, nund - at);
+      copy_wait(pairs, d_und + 2 * at, 8 * cnt, hipMemcpyDeviceToHost, s);
+      if (!device_sorted) {
+        // one chunk holds every pair: (query, point) keys sorted here
+        for (uint64_t i = 0; i < cnt; ++i) {
+          const uint32_t *w = reinterpret_cast<const uint32_t *>(&pairs[i]);
+          pairs[i] = ((uint64_t)w[0] << 32) | w[1];
+        }
+        std::sort(pairs, pairs + cnt);
+      }
+      for (uint64_t i = 0; i < cnt; ++i) {
+        const int64_t q = (int64_t)(pairs[i] >> 32);
+        const uint32_t x = (uint32_t)pairs[i];
+        if (nin[q] & kFlag) continue;
+        if (q != last) {
+          int64_t sid[4];
+          for (int k = 0; k < sizes[q]; ++k) sid[k] = ids[4 * q + k];
+          if (!S.analyze(exact_cloud(c), sid, sizes[q])) throw std::logic_error("a device query had a dependent support");
+          last = q;
+        }
+        const int sd = S.side(exact_cloud(c), x);
+        if (sd < 0) add_inside(q, x);
+        else if (sd == 0) nshell[q]++;
+      }
+    }
+  } else {
+    // one point: every query is that point alone, on its own zero sphere
+    for (int64_t q = 0; q < Q; ++q) { nin[q] = 0; nshell[q] = sizes[q]; }
+  }
+  // queries the device handed over whole
+  for (int64_t q = 0; q < Q; ++q) {
+    if (!(nin[q] & kFlag)) continue;
+    nin[q] = kFlag;
+    nshell[q] = sizes[q];
+    int64_t sid[4];
+    for (int i = 0; i < sizes[q]; ++i) sid[i] = ids[4 * q + i];
+    ExactSupport S;
+    if (!S.analyze(exact_cloud(c), sid, sizes[q]))
+      throw std::invalid_argument("the support of a closed-ball query is affinely dependent");
+    for (int64_t x = 0; x < c.n; ++x) {
+      bool on = false;
+      for (int i = 0; i < sizes[q]; ++i) on |= sid[i] == x;
+      if (on) continue;
+      const int sd = S.side(exact_cloud(c), x);
+      if (sd < 0) add_inside(q, (uint32_t)x);
+      else if (sd == 0) nshell[q]++;
+    }
+  }
+  for (int64_t q = 0; q < Q; ++q) {
+    nin[q] &= ~kFlag;
+    if (!cap) continue;
+    const uint32_t k = std::min(nin[q], cap);
+    std::sort(interior + (size_t)q * cap, interior + (size_t)q * cap + k);
+    std::fill(interior + (size_t)q * cap + k, interior + (size_t)(q + 1) * cap, kInvalid);
+  }
 }
 
 }  // namespace mhgp

@@ -39,6 +39,10 @@ void canonical_points(Context &c, double *xyz_out, uint32_t *source_index_out);
 // ball.hip
 void closed_ball(Context &c, const uint32_t *ids, const uint8_t *sizes, int64_t Q, uint32_t cap,
                  uint32_t *interior, uint32_t *nin, uint32_t *nshell);
+// the same with every device buffer in a fixed file-scope scratch array
+void closed_ball_noalloc(Context &c, const uint32_t *ids, const uint8_t *sizes, int64_t Q, uint32_t cap,
+                         uint32_t *interior, uint32_t *nin, uint32_t *nshell);
+size_t closed_ball_scratch_bytes();
 
 // knn.hip
 void knn_all(Context &c, int k, uint32_t *d_out, uint8_t *d_flags);

@@ -95,6 +95,9 @@ def lib() -> ctypes.CDLL:
         L.mhgp_ctx_stats.argtypes = [p, ctypes.POINTER(Stats), i32]
         L.mhgp_ctx_count_flops.argtypes = [p, i32]
         L.mhgp_closed_ball.argtypes = [p, p, p, i64, ctypes.c_uint32, p, p, p]
+        L.mhgp_closed_ball_noalloc.argtypes = [p, p, p, i64, ctypes.c_uint32, p, p, p]
+        L.mhgp_closed_ball_noalloc_scratch_bytes.argtypes = []
+        L.mhgp_closed_ball_noalloc_scratch_bytes.restype = i64
         L.mhgp_server_create.argtypes = [i32, i32, i64, ctypes.POINTER(p)]
         L.mhgp_server_submit.argtypes = [p, p, i64, i32, ctypes.POINTER(i64)]
         L.mhgp_server_wait.argtypes = [p, i64, ctypes.POINTER(RequestResult)]
@@ -117,7 +120,8 @@ EXPORTED = [
     "mhgp_cloud_upload", "mhgp_cloud_upload_device", "mhgp_point_count",
     "mhgp_canonical_points", "mhgp_lbvh_snapshot", "mhgp_lbvh_certify", "mhgp_knn", "mhgp_knn_partition", "mhgp_emst", "mhgp_supports",
     "mhgp_supports_fetch", "mhgp_supports_radii", "mhgp_supports_diagnostics", "mhgp_tower", "mhgp_tower_fetch", "mhgp_tower_vertical", "mhgp_tower_cut", "mhgp_ctx_stats", "mhgp_ctx_count_flops",
-    "mhgp_closed_ball", "mhgp_server_create", "mhgp_server_submit", "mhgp_server_wait", "mhgp_server_fetch",
+    "mhgp_closed_ball", "mhgp_closed_ball_noalloc", "mhgp_closed_ball_noalloc_scratch_bytes",
+    "mhgp_server_create", "mhgp_server_submit", "mhgp_server_wait", "mhgp_server_fetch",
     "mhgp_server_release", "mhgp_server_destroy", "mhgp_batch_run", "mhgp_batch_info_get", "mhgp_batch_result",
     "mhgp_batch_fetch", "mhgp_batch_destroy",
 ]
@@ -254,10 +258,11 @@ class Context:
         _check(lib().mhgp_emst(self._h, _ptr(out)))
         return out
 
-    def closed_ball(self, supports, cap: int = 16):
+    def closed_ball(self, supports, cap: int = 16, noalloc: bool = False):
         """Interior ids (complete when the count <= cap), interior counts and
         shell counts of the circumspheres of the given supports (lists of 1..4
-        canonical PointIds)."""
+        canonical PointIds).  noalloc: the entry point that allocates no
+        device memory (same results)."""
         q = len(supports)
         ids = np.full((max(q, 1), 4), 0xFFFFFFFF, dtype=np.uint32)
         sizes = np.zeros(max(q, 1), dtype=np.uint8)
@@ -267,7 +272,8 @@ class Context:
         inter = np.zeros((max(q, 1), max(cap, 1)), dtype=np.uint32)
         nin = np.zeros(max(q, 1), dtype=np.uint32)
         nsh = np.zeros(max(q, 1), dtype=np.uint32)
-        _check(lib().mhgp_closed_ball(self._h, _ptr(ids), _ptr(sizes), q, cap, _ptr(inter), _ptr(nin), _ptr(nsh)))
+        fn = lib().mhgp_closed_ball_noalloc if noalloc else lib().mhgp_closed_ball
+        _check(fn(self._h, _ptr(ids), _ptr(sizes), q, cap, _ptr(inter), _ptr(nin), _ptr(nsh)))
         return inter[:q, :cap], nin[:q], nsh[:q]
 
     def supports(self, k_max: int, fetch: bool = True):

@@ -142,6 +142,23 @@ int mhgp_closed_ball(mhgp_ctx *ctx, const uint32_t *support_ids, const uint8_t *
                      int64_t queries, uint32_t cap, uint32_t *interior_ids, uint32_t *interior_count,
                      uint32_t *shell_count);
 
+/* mhgp_closed_ball without device allocation.  No counterpart in the
+ * reference: it exists for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and errors as mhgp_closed_ball; the
+ * query staging, per-query counts and flags and the undecided-point list live
+ * in one fixed device scratch array of mhgp_closed_ball_noalloc_scratch_bytes()
+ * bytes per GPU, held until the process ends.  A call needs about
+ * 93 + 4 * max(cap, 1) bytes per query (at least 8 KiB for the undecided
+ * list); beyond the scratch it returns MHGP_E_CAPACITY.  Every call on one GPU
+ * shares that GPU's scratch: calls that overlap on one GPU (several contexts
+ * on several host threads) race and must be serialised by the caller.  The
+ * rare exact settlement of undecided points runs on the host, in host memory. */
+int mhgp_closed_ball_noalloc(mhgp_ctx *ctx, const uint32_t *support_ids, const uint8_t *support_sizes,
+                             int64_t queries, uint32_t cap, uint32_t *interior_ids, uint32_t *interior_count,
+                             uint32_t *shell_count);
+int64_t mhgp_closed_ball_noalloc_scratch_bytes(void);
+
 /* Critical-sphere catalogue for orders 1..k_max (s_max = min(min(k_max,n)+1, n)):
  * every support of 2..4 points whose circumcentre lies strictly inside its
  * convex hull, whose sphere holds no further point and whose closed rank is
