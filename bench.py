@@ -157,6 +157,9 @@ def copy_calibration(mib: int) -> None:
     print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4, "launches": 3}))
 
 
+SERVE_REGION_LIMIT = 640  # enumeration budget of the served and batched legs (plain path alike)
+
+
 def plain_request(ctx, pts, k_max):
     """One request on the plain path, as the server runs it: catalogue, then
     the tower unless the catalogue is conditional; returns (code, status)."""
@@ -199,6 +202,7 @@ def served_stream(k_values=(4, 10), count=32, lo=2_000, hi=200_000, rate_mult=4.
     ks = rng.choice(np.asarray(k_values), size=count)
     clouds = [c5_cloud(int(m), 1000 + i) for i, m in enumerate(sizes)]
     ctx = mhgp.Context(0)
+    ctx.set_region_limit(SERVE_REGION_LIMIT)
     plain_request(ctx, clouds[0], int(ks[0]))  # warm
     t0 = time.perf_counter()
     plain_lat, plain_out = [], []
@@ -206,14 +210,14 @@ def served_stream(k_values=(4, 10), count=32, lo=2_000, hi=200_000, rate_mult=4.
         a = time.perf_counter()
         plain_out.append(plain_request(ctx, pts, int(k)))
         plain_lat.append(1e3 * (time.perf_counter() - a))
-        if (i + 1) % 8 == 0:
-            print(f"served, plain path: {i + 1}/{count} done", file=sys.stderr, flush=True)
+        print(f"served, plain path: cloud {i + 1}/{count} ({len(pts)} points, K {k}) {plain_lat[-1]:.0f} ms",
+              file=sys.stderr, flush=True)
     plain_s = time.perf_counter() - t0
     ctx.close()
     plain_rate = count / plain_s
     rate = rate_mult * plain_rate
     gaps = rng.exponential(1.0 / rate, size=count)
-    srv = mhgp.Server(0, slots=slots)
+    srv = mhgp.Server(0, slots=slots, region_limit=SERVE_REGION_LIMIT)
     warm = srv.submit(clouds[0], int(ks[0]))
     srv.wait(warm)
     srv.release(warm)
@@ -232,11 +236,14 @@ def served_stream(k_values=(4, 10), count=32, lo=2_000, hi=200_000, rate_mult=4.
         if (r.code, r.status) != plain_out[i]:
             raise mhgp.MhgpError(r.code, "a served request ended unlike the same request on the plain path")
         lat.append(r.latency_ms)
+        if (i + 1) % 4 == 0:
+            print(f"served: {i + 1}/{count} done", file=sys.stderr, flush=True)
         srv.release(t)
     served_s = time.perf_counter() - t0
     srv.close()
     return {
         "clouds": count, "sizes": [lo, hi], "points": int(sizes.sum()), "slots": slots,
+        "region_limit": SERVE_REGION_LIMIT,
         "offered_rate_hz": round(rate, 2),
         "served_clouds_per_s": round(count / served_s, 3),
         "plain_clouds_per_s": round(plain_rate, 3),
@@ -258,6 +265,7 @@ def batched(count=256, slots=4, plain_subset=32):
     sizes = workloads.batch_sizes(count)
     clouds = [workloads.batch_cloud(int(m), seed=2000 + i) for i, m in enumerate(sizes)]
     ctx = mhgp.Context(0)
+    ctx.set_region_limit(SERVE_REGION_LIMIT)
     plain = []
     t0 = time.perf_counter()
     for i, pts in enumerate(clouds[:plain_subset]):
@@ -266,7 +274,7 @@ def batched(count=256, slots=4, plain_subset=32):
             print(f"batched, plain path: {i + 1}/{plain_subset} done", file=sys.stderr, flush=True)
     plain_s = time.perf_counter() - t0
     ctx.close()
-    srv = mhgp.Server(0, slots=slots)
+    srv = mhgp.Server(0, slots=slots, region_limit=SERVE_REGION_LIMIT)
     t0 = time.perf_counter()
     tickets = [srv.submit(p, 10) for p in clouds]
     outcomes = {}
@@ -276,13 +284,13 @@ def batched(count=256, slots=4, plain_subset=32):
         outcomes[key] = outcomes.get(key, 0) + 1
         if i < plain_subset and (r.code, r.status) != plain[i]:
             raise mhgp.MhgpError(r.code, "a served cloud ended unlike the same cloud on the plain path")
-        if (i + 1) % 32 == 0:
+        if (i + 1) % 8 == 0:
             print(f"batched: {i + 1}/{count} done", file=sys.stderr, flush=True)
         srv.release(t)
     served_s = time.perf_counter() - t0
     srv.close()
     return {
-        "clouds": count, "points": int(sizes.sum()), "slots": slots,
+        "clouds": count, "points": int(sizes.sum()), "slots": slots, "region_limit": SERVE_REGION_LIMIT,
         "served_clouds_per_s": round(count / served_s, 3),
         "plain_clouds_per_s_first_%d" % plain_subset: round(plain_subset / plain_s, 3),
         "outcomes": outcomes, "served_total_s": round(served_s, 2),

@@ -270,6 +270,13 @@ extern "C" int mhgp_ctx_stats(mhgp_ctx *ctx, mhgp_stats *stats, int reset) {
   return MHGP_OK;
 }
 
+extern "C" int mhgp_ctx_set_region_limit(mhgp_ctx *ctx, int64_t limit) {
+  if (!ctx) return fail(nullptr, MHGP_E_ARGUMENT, "null context", false);
+  if (limit < 640 || limit > 2048) return fail(&ctx->c, MHGP_E_ARGUMENT, "region limit must lie in 640..2048", false);
+  ctx->c.region_limit = (uint32_t)limit;
+  return MHGP_OK;
+}
+
 extern "C" int mhgp_ctx_count_flops(mhgp_ctx *ctx, int on) {
   if (!ctx) return fail(nullptr, MHGP_E_ARGUMENT, "null context", false);
   ctx->c.count_flops = on != 0;

@@ -30,6 +30,9 @@ struct Context {
   hipStream_t stream = nullptr;
   bool poisoned = false;
   bool count_flops = false;  // FP64 operation count of the anchor kernel (stats.anchor_flops)
+  // largest candidate region an anchor's enumeration takes; larger ones stay
+  // open and the catalogue is conditional (the enumeration budget)
+  uint32_t region_limit = 2048;
   std::string last_error;
 
   int64_t n = 0;

@@ -30,6 +30,7 @@ struct Request {
   int64_t ticket;
   std::vector<double> xyz;
   int k_max;
+  uint32_t region_limit;  // the server's enumeration budget when it was submitted
   int64_t predicted;
   Clock::time_point arrival, start, end;
   mhgp_request_result result{};
@@ -49,6 +50,7 @@ int64_t predict_bytes(int64_t n, int k_max) {
 struct Server {
   int device = 0;
   int64_t cap = 0;
+  uint32_t region_limit = 2048;  // enumeration budget of the requests submitted from now on
   std::vector<std::thread> workers;
   std::mutex mu;
   std::condition_variable cv_work, cv_done, cv_start;
@@ -162,6 +164,7 @@ void Server::worker_main(int slot) {
     Context &c = *ctx;
     try {
       MHGP_HIP(hipSetDevice(device));
+      c.region_limit = r->region_limit;
       upload_host(c, r->xyz.data(), (int64_t)(r->xyz.size() / 3));
       std::vector<uint32_t> edges(2 * std::max<int64_t>(c.n - 1, 0));
       emst_host(c, edges.data());
@@ -266,6 +269,13 @@ int mhgp_server_create(int device, int slots, int64_t memory_cap_bytes, mhgp_ser
   return MHGP_OK;
 }
 
+int mhgp_server_set_region_limit(mhgp_server *srv, int64_t limit) {
+  if (!srv || limit < 640 || limit > 2048) return MHGP_E_ARGUMENT;
+  std::lock_guard<std::mutex> g(srv->s.mu);
+  srv->s.region_limit = (uint32_t)limit;
+  return MHGP_OK;
+}
+
 int mhgp_server_submit(mhgp_server *srv, const double *xyz, int64_t n, int k_max, int64_t *ticket) {
   if (!srv || !xyz || n <= 0 || k_max < 1 || k_max > 10 || !ticket) return MHGP_E_ARGUMENT;
   auto r = std::make_shared<mhgp::Request>();
@@ -275,6 +285,7 @@ int mhgp_server_submit(mhgp_server *srv, const double *xyz, int64_t n, int k_max
   r->arrival = mhgp::Clock::now();
   {
     std::lock_guard<std::mutex> g(srv->s.mu);
+    r->region_limit = srv->s.region_limit;
     if (srv->s.live == 0) return MHGP_E_DEVICE;
     r->ticket = srv->s.next_ticket++;
     srv->s.queue.push_back(r);

@@ -33,6 +33,7 @@
 // host fallback queue with the anchor's region.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <condition_variable>
 #include <map>
 #include <memory>
@@ -69,11 +70,12 @@ __device__ inline void wave_sync() {
 enum : uint32_t { ST_PENDING = 0, ST_DONE = 1, ST_WIDEN = 2, ST_BIG = 3, ST_HUGE = 4, ST_PROBED = 5 };
 
 // Regions of the global-memory path are enumerated face by face in
-// O(region^3) work; beyond this size one anchor would keep a workgroup busy
-// for many seconds.  Such an anchor is not enumerated: the catalogue is
-// returned complete for every other anchor with status conditional (a
-// budget stop), and the anchor and its region size are reported.
-constexpr uint32_t kBigRegionMax = 2048;
+// O(region^3) work; beyond the context's region limit (at most this) one
+// anchor would keep a workgroup busy for many seconds.  Such an anchor is not
+// enumerated: the catalogue is returned complete for every other anchor with
+// status conditional (a budget stop), and the anchor and its region size are
+// reported.
+constexpr uint32_t kBigRegionMax = kRegionLimitMax;
 
 struct Params {
   const double *X, *Y, *Z;       // canonical coordinates
@@ -101,6 +103,7 @@ struct Params {
   uint64_t slab_doubles;
   uint32_t gcap, rcap;
   int count_flops;               // 1: count the FP64 operations of the predicates (counters[5])
+  uint32_t region_limit;         // largest region the global-memory path enumerates
 };
 
 struct Work {
@@ -686,6 +689,114 @@ __device__ inline bool face_feasible(const Work &w, uint32_t si, uint32_t sj, do
   return !(D > 1e-6 * (aa * bb + fabs(ab) * fabs(ab)) && r2 > rmax * rmax * (1.0 + 1e-6));
 }
 
+// Pairs, faces and tetrahedra of anchor u over its compacted region (M
+// points at the front of the point arrays, masks by region slot, cell radii
+// in w.rho); *sK == 0 and *sregion == kInvalid on entry, visible to every
+// lane.  Marks the anchor done.
+template <bool BIG>
+__device__ void anchor_faces(const Params &P, const Work &w, uint32_t u, uint32_t M, uint32_t *sK_,
+                             uint32_t *sregion_) {
+  uint32_t &sK = *sK_;
+  uint32_t &sregion = *sregion_;
+  const int t = threadIdx.x;
+  // candidate vertices (ids above the anchor); their order only decides
+  // which lane takes which face (a rank sort by direction cell, to give
+  // neighbouring lanes overlapping pencil capsules, measured slower)
+  for (uint32_t s = t; s < M; s += blockDim.x)
+    if (w.gid[s] > u) w.kidx[atomicAdd(&sK, 1u)] = s;
+  __syncthreads();
+  const uint32_t Kc = sK;
+  // 3. pairs (u, v), v > u
+  for (uint32_t k = t; k < Kc; k += blockDim.x) process_pair(P, w, u, M, w.kidx[k], &sregion);
+  // 4. faces (u, v, w), u < v < w, whose members share a cell.  Each wave
+  // takes 64 candidate faces at a time: first passes one per lane, then the
+  // queued sides and tetrahedra one per lane (LDS path; the global-memory
+  // path works in place).
+  __shared__ uint32_t q_side[kBlock / 64][kSideQ];
+  __shared__ float q_side_r[kBlock / 64][kSideQ];
+  __shared__ uint32_t q_face[kBlock / 64][128];
+  __shared__ uint32_t q_tet[kBlock / 64][kTetQ];
+  __shared__ uint32_t q_n[kBlock / 64][2];
+  const int wave = t >> 6, lane = t & 63, nwaves = (int)(blockDim.x >> 6);
+  WaveQueues wq{q_side[wave], q_side_r[wave], q_tet[wave], &q_n[wave][0], &q_n[wave][1]};
+  WaveQueues *q = BIG ? nullptr : &wq;
+  if (lane == 0) { q_n[wave][0] = 0; q_n[wave][1] = 0; }
+  wave_sync();
+  const uint64_t F = (uint64_t)Kc * (Kc - 1) / 2;
+  // candidate face fidx -> region slots of (v, w): the pair (i < j) of the
+  // strictly lower triangle
+  auto face_of = [&](uint64_t fidx, uint32_t &si, uint32_t &sj) {
+    uint64_t j = (uint64_t)((1.0 + sqrt(1.0 + 8.0 * (double)fidx)) * 0.5);
+    while (j * (j - 1) / 2 > fidx) j--;
+    while ((j + 1) * j / 2 <= fidx) j++;
+    const uint64_t i = fidx - j * (j - 1) / 2;
+    si = w.kidx[i];
+    sj = w.kidx[j];
+  };
+  if (BIG) {
+    for (uint64_t base = (uint64_t)wave * 64; base < F; base += (uint64_t)nwaves * 64) {
+      const uint64_t fidx = base + lane;
+      if (fidx >= F) continue;
+      uint32_t si, sj;
+      face_of(fidx, si, sj);
+      double rmax;
+      if (face_feasible(w, si, sj, rmax)) process_face<false>(P, w, u, M, si, sj, rmax, &sregion, q);
+    }
+  } else {
+    // Each wave first collects up to 64 feasible faces in its LDS list (the
+    // faces that fail the shared-cell or radius test cost only the test),
+    // then runs their first passes one per lane, then the queued sides and
+    // tetrahedra.  The lists keep region slots only; a lane recomputes the
+    // face's rmax (same arithmetic, same value).
+    uint32_t nf = 0;  // faces in the wave's list (uniform)
+    uint64_t cbase = (uint64_t)wave * 64;
+    const uint64_t cstride = (uint64_t)nwaves * 64;
+    for (;;) {
+      while (nf < 64 && cbase < F) {
+        const uint64_t fidx = cbase + lane;
+        bool ok = false;
+        uint32_t si = 0, sj = 0;
+        if (fidx < F) {
+          face_of(fidx, si, sj);
+          double rmax;
+          ok = face_feasible(w, si, sj, rmax);
+        }
+        const uint64_t m = __ballot(ok);
+        if (ok) q_face[wave][nf + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = si | (sj << 10);
+        nf += (uint32_t)__popcll(m);
+        cbase += cstride;
+      }
+      if (nf == 0) break;
+      wave_sync();
+      const uint32_t take = min(nf, 64u);
+      nf -= take;
+      if ((uint32_t)lane < take) {
+        const uint32_t fv = q_face[wave][nf + lane];
+        const uint32_t si = fv & 1023u, sj = fv >> 10;
+        double rmax;
+        face_feasible(w, si, sj, rmax);
+        process_face<true>(P, w, u, M, si, sj, rmax, &sregion, q);
+      }
+      wave_sync();
+      const uint32_t ns = min(q_n[wave][0], kSideQ);
+      for (uint32_t it = lane; it < ns; it += 64)
+        side_item(P, w, u, M, q_side[wave][it], (double)q_side_r[wave][it], &sregion, q);
+      wave_sync();
+      const uint32_t nt = min(q_n[wave][1], kTetQ);
+      for (uint32_t it = lane; it < nt; it += 64) tet_item(P, w, u, M, q_tet[wave][it], &sregion);
+      wave_sync();
+      if (lane == 0) { q_n[wave][0] = 0; q_n[wave][1] = 0; }
+      wave_sync();
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    if (*w.cand) atomicAdd(&P.counters[3], (unsigned long long)*w.cand);
+    if (w.flops && *w.flops) atomicAdd(&P.counters[5], *w.flops);
+    P.status[u] = ST_DONE;
+  }
+}
+
 // BIG: arrays in global slabs, regions beyond LDS; the face phase works in
 // place instead of through the wave queues.
 template <bool BIG>
@@ -855,15 +966,11 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     if (t == 0) P.status[u] = ST_BIG;
     return;
   }
-  if (P.big && M > kBigRegionMax) {
+  if (P.big && M > P.region_limit) {
     if (t == 0) {
       P.need[u] = (double)M;  // the region's size, reported with the conditional status
       P.status[u] = ST_HUGE;
     }
-    return;
-  }
-  if (P.probe) {
-    if (t == 0) P.status[u] = ST_PROBED;
     return;
   }
   // the region's points move to the front of the point arrays, in region
@@ -890,102 +997,16 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     }
     __syncthreads();
   }
-  // candidate vertices (ids above the anchor); their order only decides
-  // which lane takes which face (a rank sort by direction cell, to give
-  // neighbouring lanes overlapping pencil capsules, measured slower)
-  for (uint32_t s = t; s < M; s += blockDim.x)
-    if (w.gid[s] > u) w.kidx[atomicAdd(&sK, 1u)] = s;
-  __syncthreads();
-  const uint32_t Kc = sK;
-  // 3. pairs (u, v), v > u
-  for (uint32_t k = t; k < Kc; k += blockDim.x) process_pair(P, w, u, M, w.kidx[k], &sregion);
-  // 4. faces (u, v, w), u < v < w, whose members share a cell.  Each wave
-  // takes 64 candidate faces at a time: first passes one per lane, then the
-  // queued sides and tetrahedra one per lane (LDS path; the global-memory
-  // path works in place).
-  __shared__ uint32_t q_side[kBlock / 64][kSideQ];
-  __shared__ float q_side_r[kBlock / 64][kSideQ];
-  __shared__ uint32_t q_face[kBlock / 64][128];
-  __shared__ uint32_t q_tet[kBlock / 64][kTetQ];
-  __shared__ uint32_t q_n[kBlock / 64][2];
-  const int wave = t >> 6, lane = t & 63, nwaves = (int)(blockDim.x >> 6);
-  WaveQueues wq{q_side[wave], q_side_r[wave], q_tet[wave], &q_n[wave][0], &q_n[wave][1]};
-  WaveQueues *q = BIG ? nullptr : &wq;
-  if (lane == 0) { q_n[wave][0] = 0; q_n[wave][1] = 0; }
-  wave_sync();
-  const uint64_t F = (uint64_t)Kc * (Kc - 1) / 2;
-  // candidate face fidx -> region slots of (v, w): the pair (i < j) of the
-  // strictly lower triangle
-  auto face_of = [&](uint64_t fidx, uint32_t &si, uint32_t &sj) {
-    uint64_t j = (uint64_t)((1.0 + sqrt(1.0 + 8.0 * (double)fidx)) * 0.5);
-    while (j * (j - 1) / 2 > fidx) j--;
-    while ((j + 1) * j / 2 <= fidx) j++;
-    const uint64_t i = fidx - j * (j - 1) / 2;
-    si = w.kidx[i];
-    sj = w.kidx[j];
-  };
-  if (BIG) {
-    for (uint64_t base = (uint64_t)wave * 64; base < F; base += (uint64_t)nwaves * 64) {
-      const uint64_t fidx = base + lane;
-      if (fidx >= F) continue;
-      uint32_t si, sj;
-      face_of(fidx, si, sj);
-      double rmax;
-      if (face_feasible(w, si, sj, rmax)) process_face<false>(P, w, u, M, si, sj, rmax, &sregion, q);
+  if (P.probe) {
+    // the region is known and compacted in the slab: its size is reported,
+    // and a region that fits LDS is enumerated there (k_anchor_region)
+    if (t == 0) {
+      P.need[u] = (double)M;
+      P.status[u] = ST_PROBED;
     }
-  } else {
-    // Each wave first collects up to 64 feasible faces in its LDS list (the
-    // faces that fail the shared-cell or radius test cost only the test),
-    // then runs their first passes one per lane, then the queued sides and
-    // tetrahedra.  The lists keep region slots only; a lane recomputes the
-    // face's rmax (same arithmetic, same value).
-    uint32_t nf = 0;  // faces in the wave's list (uniform)
-    uint64_t cbase = (uint64_t)wave * 64;
-    const uint64_t cstride = (uint64_t)nwaves * 64;
-    for (;;) {
-      while (nf < 64 && cbase < F) {
-        const uint64_t fidx = cbase + lane;
-        bool ok = false;
-        uint32_t si = 0, sj = 0;
-        if (fidx < F) {
-          face_of(fidx, si, sj);
-          double rmax;
-          ok = face_feasible(w, si, sj, rmax);
-        }
-        const uint64_t m = __ballot(ok);
-        if (ok) q_face[wave][nf + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = si | (sj << 10);
-        nf += (uint32_t)__popcll(m);
-        cbase += cstride;
-      }
-      if (nf == 0) break;
-      wave_sync();
-      const uint32_t take = min(nf, 64u);
-      nf -= take;
-      if ((uint32_t)lane < take) {
-        const uint32_t fv = q_face[wave][nf + lane];
-        const uint32_t si = fv & 1023u, sj = fv >> 10;
-        double rmax;
-        face_feasible(w, si, sj, rmax);
-        process_face<true>(P, w, u, M, si, sj, rmax, &sregion, q);
-      }
-      wave_sync();
-      const uint32_t ns = min(q_n[wave][0], kSideQ);
-      for (uint32_t it = lane; it < ns; it += 64)
-        side_item(P, w, u, M, q_side[wave][it], (double)q_side_r[wave][it], &sregion, q);
-      wave_sync();
-      const uint32_t nt = min(q_n[wave][1], kTetQ);
-      for (uint32_t it = lane; it < nt; it += 64) tet_item(P, w, u, M, q_tet[wave][it], &sregion);
-      wave_sync();
-      if (lane == 0) { q_n[wave][0] = 0; q_n[wave][1] = 0; }
-      wave_sync();
-    }
+    return;
   }
-  __syncthreads();
-  if (t == 0) {
-    if (*w.cand) atomicAdd(&P.counters[3], (unsigned long long)*w.cand);
-    if (w.flops && *w.flops) atomicAdd(&P.counters[5], *w.flops);
-    P.status[u] = ST_DONE;
-  }
+  anchor_faces<BIG>(P, w, u, M, &sK, &sregion);
 }
 
 // The LDS path in two sizes: most anchors fit the small one, which leaves
@@ -1085,6 +1106,44 @@ __global__ void __launch_bounds__(kBlock) k_anchor_big(Params P) {
   anchor_work<true>(P, w, u);
 }
 
+// Regions measured by the global-memory path's probe that fit the wide LDS
+// caps: the anchor's compacted region (coordinates, PointIds, masks) and cell
+// radii are loaded from its slab into LDS, and the face phase runs there with
+// the wave queues.  Same parameters and slab layout as k_anchor_big's probe
+// launch; anchors not probed, or with larger regions, are left to it.
+__global__ void __launch_bounds__(kBlock, 2) k_anchor_region(Params P) {
+  constexpr int RCAP = kRegionCapWide;
+  const uint32_t a = blockIdx.x;
+  if (a >= P.anchor_count) return;
+  const uint32_t u = P.anchors[a];
+  if (P.status[u] != ST_PROBED || !(P.need[u] <= (double)RCAP)) return;
+  const uint32_t M = (uint32_t)P.need[u];
+  const uint64_t G = P.gcap;
+  const double *slab = P.scratch + (uint64_t)a * P.slab_doubles;
+  const uint64_t *smask = (const uint64_t *)(slab + 3 * G + kCells);
+  const uint32_t *sgid = (const uint32_t *)(slab + 3 * G + kCells + 6 * G);
+  __shared__ double gx[RCAP], gy[RCAP], gz[RCAP];
+  __shared__ uint32_t gid[RCAP];
+  __shared__ double rho[kCells];
+  __shared__ uint64_t rmask[6 * RCAP];
+  __shared__ uint32_t kidx[RCAP];
+  __shared__ uint32_t cand, sK, sregion;
+  __shared__ unsigned long long flops;
+  const int t = threadIdx.x;
+  for (uint32_t i = t; i < M; i += blockDim.x) {
+    gx[i] = slab[i];
+    gy[i] = slab[G + i];
+    gz[i] = slab[2 * G + i];
+    gid[i] = sgid[i];
+  }
+  for (uint32_t i = t; i < 6 * M; i += blockDim.x) rmask[i] = smask[i];
+  for (int c = t; c < kCells; c += blockDim.x) rho[c] = slab[3 * G + c];
+  if (t == 0) { cand = 0; sK = 0; sregion = kInvalid; flops = 0; }
+  __syncthreads();
+  Work w{gx, gy, gz, gid, rho, kidx, rmask, kidx, nullptr, &cand, P.count_flops ? &flops : nullptr};
+  anchor_faces<false>(P, w, u, M, &sK, &sregion);
+}
+
 // Ball count / emit around every anchor of the list.
 // Pruning runs in FP32 on the two-level LBVH copy (boxes rounded outward, the
 // radius inflated by the rounding of the query and of the FP32 arithmetic; a
@@ -1147,6 +1206,97 @@ __global__ void __launch_bounds__(B) k_gather(int64_t n, const WideNode *__restr
     }
   }
   if (!EMIT) gcount[u] = cnt;
+}
+
+// The same gather with one wave per anchor, for the widened passes, whose
+// radii hold hundreds to thousands of points: the wave pops up to 64 pending
+// nodes at once (one per lane), tests their four children, and appends hits
+// and pending children with ballot prefix counts, so one anchor's traversal
+// is spread over 64 lanes instead of walked by one.  The traversal order is
+// fixed, so the count and emit passes visit the same points.
+constexpr int kWaveGatherStack = 4096;
+constexpr int kWaveGatherBlock = 128;  // two waves, two anchors per workgroup
+template <bool EMIT>
+__global__ void __launch_bounds__(kWaveGatherBlock) k_gather_wave(int64_t n, const WideNode *__restrict__ wide,
+                                                                  const double *__restrict__ LX,
+                                                                  const double *__restrict__ LY,
+                                                                  const double *__restrict__ LZ,
+                                                                  const uint32_t *__restrict__ leaf_id,
+                                                                  const uint32_t *__restrict__ leaf_pos,
+                                                                  const uint32_t *__restrict__ anchors, uint32_t count,
+                                                                  const double *__restrict__ radius, uint32_t *gcount,
+                                                                  const uint64_t *__restrict__ goff, uint32_t *glist,
+                                                                  uint32_t *overflow) {
+  __shared__ uint32_t stk[kWaveGatherBlock / 64][kWaveGatherStack];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t a = blockIdx.x * (kWaveGatherBlock / 64) + wave;
+  if (a >= count) return;  // uniform over the wave
+  uint32_t *S = stk[wave];
+  const uint64_t below = (1ull << lane) - 1ull;
+  const uint32_t u = anchors[a];
+  const uint32_t p = leaf_pos[u];
+  const double q[3] = {LX[p], LY[p], LZ[p]};
+  const double R = radius[u];
+  const double R2 = R * R * (1.0 + 1e-9);
+  const float qf[3] = {(float)q[0], (float)q[1], (float)q[2]};
+  const double qmax = fmax(fmax(fabs(q[0]), fabs(q[1])), fabs(q[2]));
+  const double rb = sqrt(R2) * (1.0 + 1e-6) + qmax * 1e-6 + 1e-30;
+  float Tf = (float)(rb * rb * (1.0 + 1e-5));
+  if (!(Tf < 3.0e38f)) Tf = INFINITY;
+  uint32_t cnt = 0, sp = 0;
+  const uint64_t base = EMIT ? goff[u] : 0;
+  if (n > 1) {
+    if (lane == 0) S[0] = 0;
+    sp = 1;
+  }
+  bool lost = false;
+  wave_sync();
+  while (sp > 0) {
+    const uint32_t take = min(sp, 64u);
+    sp -= take;
+    const uint32_t node = (uint32_t)lane < take ? S[sp + lane] : kInvalid;
+    wave_sync();  // every pop is read before a push overwrites it
+    const WideNode *N = node != kInvalid ? &wide[node] : nullptr;
+    for (int c = 0; c < 4; ++c) {
+      bool hit = false, push = false;
+      uint32_t x = kInvalid;
+      if (N) {
+        x = N->child[c];
+        if (x != kInvalid) {
+          float d2 = 0.f;
+          for (int ax = 0; ax < 3; ++ax) {
+            const float d = fmaxf(fmaxf(N->lo[c][ax] - qf[ax], qf[ax] - N->hi[c][ax]), 0.f);
+            d2 += d * d;
+          }
+          if (d2 <= Tf) {
+            if (x & kLeafBit) {
+              const uint32_t lp = x & ~kLeafBit;
+              if (lp != p) {
+                const double dx = LX[lp] - q[0], dy = LY[lp] - q[1], dz = LZ[lp] - q[2];
+                hit = dx * dx + dy * dy + dz * dz <= R2;
+                if (hit) x = lp;
+              }
+            } else {
+              push = true;
+            }
+          }
+        }
+      }
+      const uint64_t mh = __ballot(hit);
+      if (EMIT && hit) glist[base + cnt + (uint32_t)__popcll(mh & below)] = leaf_id[x];
+      cnt += (uint32_t)__popcll(mh);
+      const uint64_t mp = __ballot(push);
+      const uint32_t slot = sp + (uint32_t)__popcll(mp & below);
+      if (push) {
+        if (slot < (uint32_t)kWaveGatherStack) S[slot] = x;
+        else lost = true;
+      }
+      sp = min(sp + (uint32_t)__popcll(mp), (uint32_t)kWaveGatherStack);
+    }
+    wave_sync();
+  }
+  if (lost) atomicOr(overflow, 1u);
+  if (!EMIT && lane == 0) gcount[u] = cnt;
 }
 
 // Initial gather radius of every point: factor x the distance to its kq-th
@@ -1351,6 +1501,14 @@ void launch_gather(int B, hipStream_t s, Context &c, CatalogBuffers &Bf, uint32_
   if (B == 64) launch_gather<EMIT, 64>(s, c, Bf, cnt, overflow);
   else if (B == 256) launch_gather<EMIT, 256>(s, c, Bf, cnt, overflow);
   else launch_gather<EMIT, 128>(s, c, Bf, cnt, overflow);
+}
+
+template <bool EMIT>
+void launch_gather_wave(hipStream_t s, Context &c, CatalogBuffers &Bf, uint32_t cnt, uint32_t *overflow) {
+  const uint32_t per = kWaveGatherBlock / 64;
+  hipLaunchKernelGGL((k_gather_wave<EMIT>), dim3((cnt + per - 1) / per), dim3(kWaveGatherBlock), 0, s, c.n, c.wide.p,
+                     c.LX.p, c.LY.p, c.LZ.p, c.leaf_id.p, c.leaf_pos.p, Bf.anchors.p, cnt, Bf.radius.p, Bf.gcount.p,
+                     Bf.goff.p, Bf.glist.p, overflow);
 }
 
 // Times `run(tile)` for every candidate tile; returns the fastest.
@@ -1594,14 +1752,18 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     if (iter > 12) throw CapacityError("anchor gather radii did not converge");
     const uint32_t cnt = (uint32_t)pending.size();
     MHGP_HIP(hipMemcpyAsync(B.anchors.p, pending.data(), cnt * 4, hipMemcpyHostToDevice, s));
-    launch_gather<false>(tile.gather, s, c, B, cnt, overflow.p);
+    // the first pass takes one thread per anchor (small radii); the widened
+    // passes one wave per anchor
+    if (iter == 0) launch_gather<false>(tile.gather, s, c, B, cnt, overflow.p);
+    else launch_gather_wave<false>(s, c, B, cnt, overflow.p);
     hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, B.anchors.p, cnt, B.gcount.p,
                        B.goff.p, total.p);
     uint64_t tot = 0;
     MHGP_HIP(hipMemcpyAsync(&tot, total.p, 8, hipMemcpyDeviceToHost, s));
     MHGP_HIP(hipStreamSynchronize(s));
     B.glist.ensure(tot + 1);
-    launch_gather<true>(tile.gather, s, c, B, cnt, overflow.p);
+    if (iter == 0) launch_gather<true>(tile.gather, s, c, B, cnt, overflow.p);
+    else launch_gather_wave<true>(s, c, B, cnt, overflow.p);
     Params P{};
     P.X = c.X.p; P.Y = c.Y.p; P.Z = c.Z.p;
     P.gcount = B.gcount.p; P.goff = B.goff.p; P.glist = B.glist.p; P.radius = B.radius.p;
@@ -1616,6 +1778,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     P.anchors = B.anchors.p; P.anchor_count = cnt;
     P.big = 0; P.gcap = kGatherCap; P.rcap = kRegionCap;
     P.count_flops = c.count_flops ? 1 : 0;
+    P.region_limit = std::min<uint32_t>(c.region_limit, kBigRegionMax);
     tm.mark("catalogue gather");
     uint64_t ctr_before[8];
     copy_wait(ctr_before, B.counters.p, sizeof(ctr_before), hipMemcpyDeviceToHost, s);
@@ -1678,6 +1841,8 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
       MHGP_HIP(hipMemcpyAsync(st.data(), B.status.p, n * 4, hipMemcpyDeviceToHost, s));
       MHGP_HIP(hipMemcpyAsync(need.data(), B.need.p, n * 8, hipMemcpyDeviceToHost, s));
       MHGP_HIP(hipStreamSynchronize(s));
+      if (tm.on) std::fprintf(stderr, "[mhgp] wide anchors: %u\n", wc);
+      tm.mark("catalogue wide anchors");
     }
     // big anchors: widen inside the big path too
     if (!big.empty()) {
@@ -1688,22 +1853,48 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
         uint32_t gmax = 0;
         for (uint32_t u : big) gmax = std::max(gmax, gc[u]);
         gmax = std::max<uint32_t>(gmax, 64);
-        const uint64_t slab = big_fc_offset(gmax) + 2ull * gmax;
-        const uint32_t chunk = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(bc, (1ull << 30) / (slab * 8)));
-        B.scratch.ensure(slab * chunk);
         // the CSR of the big anchors is still valid from this iteration's gather
         std::vector<uint32_t> again;
-        auto launch_big = [&](const std::vector<uint32_t> &list, int probe) {
-          const uint32_t lc = (uint32_t)list.size();
-          for (uint32_t b0 = 0; b0 < lc; b0 += chunk) {
-            uint32_t bn = std::min(chunk, lc - b0);
+        // Launches in chunks of anchors of similar gather size (ascending), each
+        // chunk's slabs sized by its own largest gather within one scratch
+        // budget: a few anchors that gather most of the cloud no longer shrink
+        // every launch to a handful of workgroups.
+        constexpr uint64_t kScratchBudget = 1ull << 30;  // bytes
+        auto slab_of = [](uint32_t g) { return big_fc_offset(g) + 2ull * g; };
+        auto launch_big = [&](const std::vector<uint32_t> &list_in, int probe) {
+          std::vector<uint32_t> list(list_in);
+          std::sort(list.begin(), list.end(),
+                    [&](uint32_t x, uint32_t y) { return gc[x] < gc[y] || (gc[x] == gc[y] && x < y); });
+          const size_t lc = list.size();
+          std::vector<std::pair<size_t, uint32_t>> chunks;  // (end, gather cap)
+          uint64_t scratch = 0;
+          for (size_t b0 = 0; b0 < lc;) {
+            size_t b1 = b0;
+            uint32_t g = 64;
+            while (b1 < lc) {
+              const uint32_t gn = std::max<uint32_t>(64, gc[list[b1]]);
+              if (b1 > b0 && slab_of(gn) * 8 * (b1 - b0 + 1) > kScratchBudget) break;
+              g = gn;
+              ++b1;
+            }
+            chunks.push_back({b1, g});
+            scratch = std::max<uint64_t>(scratch, slab_of(g) * (b1 - b0));
+            b0 = b1;
+          }
+          B.scratch.ensure(scratch);
+          size_t b0 = 0;
+          for (const auto &ch : chunks) {
+            const uint32_t bn = (uint32_t)(ch.first - b0), g = ch.second;
             MHGP_HIP(hipMemcpyAsync(B.anchors.p, list.data() + b0, bn * 4, hipMemcpyHostToDevice, s));
+            b0 = ch.first;
             Params Q = P;
             // the gather CSR may have been reallocated by a regather below
             Q.gcount = B.gcount.p; Q.goff = B.goff.p; Q.glist = B.glist.p; Q.radius = B.radius.p;
             Q.anchors = B.anchors.p; Q.anchor_count = bn; Q.big = 1; Q.probe = probe;
-            Q.scratch = B.scratch.p; Q.slab_doubles = slab; Q.gcap = gmax; Q.rcap = gmax;
+            Q.scratch = B.scratch.p; Q.slab_doubles = slab_of(g); Q.gcap = g; Q.rcap = g;
             hipLaunchKernelGGL(k_anchor_big, dim3(bn), dim3(kBlock), 0, s, Q);
+            // probed regions that fit LDS: enumerated there, from this chunk's slabs
+            if (probe) hipLaunchKernelGGL(k_anchor_region, dim3(bn), dim3(kBlock), 0, s, Q);
             MHGP_LAUNCH_CHECK();
           }
           MHGP_HIP(hipMemcpyAsync(st.data(), B.status.p, n * 4, hipMemcpyDeviceToHost, s));
@@ -1713,6 +1904,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
         // first the region sizes only (no enumeration), so that a region the
         // path cannot take is refused before any long launch
         launch_big(big, 1);
+        tm.mark("catalogue big probe (+ LDS regions)");
         std::vector<uint32_t> full;
         for (uint32_t u : big) {
           if (st[u] == ST_WIDEN) {
@@ -1725,11 +1917,17 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
             largest_open = std::max<int64_t>(largest_open, (int64_t)need[u]);
           } else if (st[u] == ST_PROBED) {
             full.push_back(u);
+          } else if (st[u] == ST_DONE) {
+            // region enumerated in LDS after the probe
           } else {
             throw std::runtime_error("a big anchor did not complete its probe");
           }
         }
+        if (tm.on)
+          std::fprintf(stderr, "[mhgp] big pass %d: %u anchors, gather cap %u: %zu widen, %zu on slabs, %lld open so far\n",
+                       biter, bc, gmax, again.size(), full.size(), (long long)open_anchors);
         if (!full.empty()) launch_big(full, 0);
+        tm.mark("catalogue big on slabs");
         for (uint32_t u : full)
           if (st[u] != ST_DONE) throw std::runtime_error("a big anchor did not complete");
         if (again.empty()) break;
@@ -1737,15 +1935,16 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
         const uint32_t ac = (uint32_t)again.size();
         copy_wait(B.radius.p, rad.data(), n * 8, hipMemcpyHostToDevice, s);
         MHGP_HIP(hipMemcpyAsync(B.anchors.p, again.data(), ac * 4, hipMemcpyHostToDevice, s));
-        launch_gather<false>(tile.gather, s, c, B, ac, overflow.p);
+        launch_gather_wave<false>(s, c, B, ac, overflow.p);
         hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, B.anchors.p, ac, B.gcount.p,
                            B.goff.p, total.p);
         MHGP_HIP(hipMemcpyAsync(&tot, total.p, 8, hipMemcpyDeviceToHost, s));
         MHGP_HIP(hipStreamSynchronize(s));
         B.glist.ensure(tot + 1);
-        launch_gather<true>(tile.gather, s, c, B, ac, overflow.p);
+        launch_gather_wave<true>(s, c, B, ac, overflow.p);
         MHGP_HIP(hipMemcpyAsync(gc.data(), B.gcount.p, n * 4, hipMemcpyDeviceToHost, s));
         MHGP_HIP(hipStreamSynchronize(s));
+        tm.mark("catalogue big regather");
         big = again;
       }
       big.clear();

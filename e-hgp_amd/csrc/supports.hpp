@@ -10,6 +10,7 @@ namespace mhgp {
 constexpr int kCells = 384;      // 6 cube faces x 8 x 8 direction cells
 constexpr int kGatherCap = 600;  // gathered neighbours held in LDS
 constexpr int kRegionCap = 384;  // region members held in LDS
+constexpr uint32_t kRegionLimitMax = 2048;  // largest candidate region any path enumerates
 constexpr int kGatherCapWide = 1024;  // the same for the wide LDS path
 constexpr int kRegionCapWide = 640;
 constexpr int kMaxInterior = 10; // s_max - 2 for s_max <= 11 (K_max <= 10)

@@ -636,6 +636,7 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
     }
     std::vector<uint32_t> elem;
     tower_vm_gather(c, pos, elem);
+    tm.mark("tower vertical gather");
     std::vector<std::thread> th;
     std::vector<std::exception_ptr> verr(K + 1);
     for (int o = 2; o <= K; ++o)

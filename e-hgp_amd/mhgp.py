@@ -94,6 +94,8 @@ def lib() -> ctypes.CDLL:
         L.mhgp_tower_cut.argtypes = [p, i32, i64, i32, p, p, p]
         L.mhgp_ctx_stats.argtypes = [p, ctypes.POINTER(Stats), i32]
         L.mhgp_ctx_count_flops.argtypes = [p, i32]
+        L.mhgp_ctx_set_region_limit.argtypes = [p, i64]
+        L.mhgp_server_set_region_limit.argtypes = [p, i64]
         L.mhgp_closed_ball.argtypes = [p, p, p, i64, ctypes.c_uint32, p, p, p]
         L.mhgp_closed_ball_noalloc.argtypes = [p, p, p, i64, ctypes.c_uint32, p, p, p]
         L.mhgp_closed_ball_noalloc_scratch_bytes.argtypes = []
@@ -119,9 +121,9 @@ EXPORTED = [
     "mhgp_last_error", "mhgp_ctx_create", "mhgp_ctx_destroy", "mhgp_ctx_device_bytes",
     "mhgp_cloud_upload", "mhgp_cloud_upload_device", "mhgp_point_count",
     "mhgp_canonical_points", "mhgp_lbvh_snapshot", "mhgp_lbvh_certify", "mhgp_knn", "mhgp_knn_partition", "mhgp_emst", "mhgp_supports",
-    "mhgp_supports_fetch", "mhgp_supports_radii", "mhgp_supports_diagnostics", "mhgp_tower", "mhgp_tower_fetch", "mhgp_tower_vertical", "mhgp_tower_cut", "mhgp_ctx_stats", "mhgp_ctx_count_flops",
+    "mhgp_supports_fetch", "mhgp_supports_radii", "mhgp_supports_diagnostics", "mhgp_tower", "mhgp_tower_fetch", "mhgp_tower_vertical", "mhgp_tower_cut", "mhgp_ctx_stats", "mhgp_ctx_count_flops", "mhgp_ctx_set_region_limit",
     "mhgp_closed_ball", "mhgp_closed_ball_noalloc", "mhgp_closed_ball_noalloc_scratch_bytes",
-    "mhgp_server_create", "mhgp_server_submit", "mhgp_server_wait", "mhgp_server_fetch",
+    "mhgp_server_create", "mhgp_server_set_region_limit", "mhgp_server_submit", "mhgp_server_wait", "mhgp_server_fetch",
     "mhgp_server_release", "mhgp_server_destroy", "mhgp_batch_run", "mhgp_batch_info_get", "mhgp_batch_result",
     "mhgp_batch_fetch", "mhgp_batch_destroy",
 ]
@@ -198,6 +200,10 @@ class Context:
     def count_flops(self, on: bool) -> None:
         """FP64 operation count of the anchor kernel (Stats.anchor_flops)."""
         _check(lib().mhgp_ctx_count_flops(self._h, 1 if on else 0))
+
+    def set_region_limit(self, limit: int) -> None:
+        """Enumeration budget: the largest candidate region enumerated (640..2048)."""
+        _check(lib().mhgp_ctx_set_region_limit(self._h, int(limit)))
 
     def upload(self, points) -> None:
         a = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
@@ -367,10 +373,12 @@ class Server:
     """Continuous-batching server on one GPU: worker slots with their own
     streams, memory admission under a cap (default 80% of HBM)."""
 
-    def __init__(self, device: int = 0, slots: int = 4, memory_cap_bytes: int = 0):
+    def __init__(self, device: int = 0, slots: int = 4, memory_cap_bytes: int = 0, region_limit: int = 0):
         h = ctypes.c_void_p()
         _check(lib().mhgp_server_create(device, slots, memory_cap_bytes, ctypes.byref(h)))
         self._h = h
+        if region_limit:
+            _check(lib().mhgp_server_set_region_limit(self._h, int(region_limit)))
 
     def close(self) -> None:
         if self._h:

@@ -282,6 +282,14 @@ int mhgp_ctx_stats(mhgp_ctx *ctx, mhgp_stats *stats, int reset);
  * default: the kernel then skips the per-call LDS atomics of the count). */
 int mhgp_ctx_count_flops(mhgp_ctx *ctx, int on);
 
+/* Enumeration budget of the catalogue: the largest candidate region (points)
+ * the global-memory anchor path enumerates, 640 <= limit <= 2048 (default
+ * 2048).  Anchors whose region exceeds it stay open and mhgp_supports reports
+ * status 1 (conditional) with their count and largest region, never an exact
+ * status.  Role of the reference's work budgets that end a stream with
+ * budget_exhausted (pair_support_stream.hpp:48-62). */
+int mhgp_ctx_set_region_limit(mhgp_ctx *ctx, int64_t limit);
+
 /* Continuous-batching server: a pool of worker slots (each one context and
  * one stream) that runs upload -> EMST -> catalogue -> tower for submitted
  * clouds.  A request is admitted when a slot is free and its predicted
@@ -302,6 +310,9 @@ typedef struct {
 } mhgp_request_result;
 
 int mhgp_server_create(int device, int slots, int64_t memory_cap_bytes, mhgp_server **out);
+/* Enumeration budget (as mhgp_ctx_set_region_limit) of the requests
+ * submitted after the call. */
+int mhgp_server_set_region_limit(mhgp_server *srv, int64_t limit);
 /* Copies xyz (n x 3 binary64); k_max in 1..10. */
 int mhgp_server_submit(mhgp_server *srv, const double *xyz, int64_t n, int k_max,
                        int64_t *ticket);

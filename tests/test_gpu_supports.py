@@ -141,6 +141,39 @@ def test_big_regions_give_a_conditional_catalogue():
     assert np.array_equal(cat.ids, ref.ids) and np.array_equal(cat.level_class, ref.level_class)
 
 
+def test_region_limit_is_an_enumeration_budget():
+    """mhgp_ctx_set_region_limit: with the limit at 640 the anisotropic
+    mixture's anchors whose candidate regions exceed it stay open (status
+    conditional, largest open region within (640, 2048]), and every record
+    of that catalogue is a record of the unlimited one.  Limits outside
+    640..2048 are refused."""
+    rng = workloads.SplitMix64(1012).numpy_rng()
+    rng.integers(3)
+    pts = workloads.gaussian_mixture(6467, blobs=int(rng.integers(5, 21)), noise_fraction=0.15, box=10.0,
+                                     snap_bits=None, seed=1012)
+    print("region limit: anisotropic mixture n=6467 k=10", flush=True)
+
+    def records(cat):
+        return {(tuple(int(x) for x in cat.ids[i, : cat.size[i]]), cat.interior[i], int(cat.rank[i]))
+                for i in range(len(cat.size))}
+
+    with mhgp.Context(0) as ctx:
+        ctx.upload(pts)
+        full = ctx.supports(10)
+        assert full.info.status == 0
+        for bad in (639, 2049):
+            with pytest.raises(mhgp.MhgpError) as err:
+                ctx.set_region_limit(bad)
+            assert err.value.code == mhgp.E_ARGUMENT
+        ctx.set_region_limit(640)
+        ctx.upload(pts)
+        part = ctx.supports(10)
+    assert part.info.status == 1 and part.info.anchors_open > 0
+    assert 640 < part.info.largest_open_region <= 2048
+    mine, ref = records(part), records(full)
+    assert mine < ref
+
+
 def test_context_capacity_refuses_and_recovers():
     """A context created with a device capacity refuses a catalogue whose
     record queues would exceed it (MHGP_E_CAPACITY, no poisoning); after an
