@@ -518,6 +518,54 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
     c.tower_lanes.push_back(std::move(w));
   }
   std::mutex mu;  // workers, the host descender
+  // Vertical maps of the order pair (o -> o - 1) need both orders contracted:
+  // whichever of the two contractions finishes last runs the pair, while the
+  // lanes go on with the orders below.  The element gathers share the first
+  // lane's scratch (sized here for every order, so that no buffer is freed
+  // while the lanes run) and take turns.
+  std::vector<char> contracted(K + 2, 0), pair_done(K + 2, 0);
+  std::mutex vm_mu, vm_gather_mu;
+  {
+    uint64_t most = 16;
+    for (int o = 2; o <= K; ++o) most = std::max<uint64_t>(most, 2ull * (W0.rank_hi[o] - W0.rank_lo[o]) + 16);
+    if (K >= 2) {
+      W0.vm_idx.ensure(most);
+      W0.vm_out.ensure(most);
+    }
+  }
+  auto vertical_pair = [&](int o) {
+    const Forest &S = forests[o - 1];
+    const uint32_t base = W0.rank_lo[o], span = W0.rank_hi[o] - base;
+    std::vector<uint32_t> pos(S.order.size());
+    for (size_t i = 0; i < S.order.size(); ++i) {
+      const int32_t d = S.rep_elem[i];
+      if (d < 0) throw std::runtime_error("vertical map: a node without a representative element");
+      if (!S.raw_of_dense.empty() && (size_t)d >= S.raw_of_dense.size())
+        throw std::runtime_error("vertical map: a representative outside the order's elements");
+      const uint32_t raw = S.raw_of_dense.empty() ? (uint32_t)d : S.raw_of_dense[d];
+      if (raw >= span) throw std::runtime_error("vertical map: a minimum outside the order's rank range");
+      pos[i] = base + raw;
+    }
+    std::vector<uint32_t> elem;
+    {
+      std::lock_guard<std::mutex> g(vm_gather_mu);
+      tower_vm_gather(c, pos, elem);
+    }
+    vertical_order(forests[o - 1], forests[o - 2], elem, forests[o - 1].vtarget);
+  };
+  auto after_contract = [&](int k) {
+    int ready[2], nready = 0;
+    {
+      std::lock_guard<std::mutex> g(vm_mu);
+      contracted[k] = 1;
+      for (int o : {k + 1, k})
+        if (o >= 2 && o <= K && contracted[o] && contracted[o - 1] && !pair_done[o]) {
+          pair_done[o] = 1;
+          ready[nready++] = o;
+        }
+    }
+    for (int q = 0; q < nready; ++q) vertical_pair(ready[q]);
+  };
   // The device lanes take the orders from a shared counter, highest first
   // (their contractions are the longest), so one lane's host
   // synchronisations overlap the other's kernels and a lane that finishes
@@ -573,6 +621,7 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
       workers.emplace_back([&, k] {
         try {
           contract_order(k, n, Ds[k - 1], forests[k - 1]);
+          after_contract(k);
         } catch (...) {
           errors[k - 1] = std::current_exception();
         }
@@ -615,43 +664,9 @@ void tower(Context &c, int k_max, mhgp_tower_info *info) {
   for (auto &e : errors)
     if (e) std::rethrow_exception(e);
   tm.mark("tower forests (host)");
-  // vertical maps: one gather of every node's element, then per order a walk
-  // up the order below
-  {
-    std::vector<uint32_t> pos;
-    std::vector<size_t> first(K + 1, 0);
-    for (int o = 2; o <= K; ++o) {
-      const Forest &S = forests[o - 1];
-      first[o] = pos.size();
-      const uint32_t base = c.tower_work->rank_lo[o], span = c.tower_work->rank_hi[o] - base;
-      for (size_t i = 0; i < S.order.size(); ++i) {
-        const int32_t d = S.rep_elem[i];
-        if (d < 0) throw std::runtime_error("vertical map: a node without a representative element");
-        if (!S.raw_of_dense.empty() && (size_t)d >= S.raw_of_dense.size())
-          throw std::runtime_error("vertical map: a representative outside the order's elements");
-        const uint32_t raw = S.raw_of_dense.empty() ? (uint32_t)d : S.raw_of_dense[d];
-        if (raw >= span) throw std::runtime_error("vertical map: a minimum outside the order's rank range");
-        pos.push_back(base + raw);
-      }
-    }
-    std::vector<uint32_t> elem;
-    tower_vm_gather(c, pos, elem);
-    tm.mark("tower vertical gather");
-    std::vector<std::thread> th;
-    std::vector<std::exception_ptr> verr(K + 1);
-    for (int o = 2; o <= K; ++o)
-      th.emplace_back([&, o] {
-        try {
-          const std::vector<uint32_t> e(elem.begin() + first[o], elem.begin() + first[o] + forests[o - 1].order.size());
-          vertical_order(forests[o - 1], forests[o - 2], e, forests[o - 1].vtarget);
-        } catch (...) {
-          verr[o] = std::current_exception();
-        }
-      });
-    for (auto &t : th) t.join();
-    for (auto &e : verr)
-      if (e) std::rethrow_exception(e);
-  }
+  // every pair's vertical map ran beside the lanes
+  for (int o = 2; o <= K; ++o)
+    if (!pair_done[o]) throw std::logic_error("vertical map of an order pair was not built");
   tm.mark("tower vertical maps (host)");
   // the orders stay separate; fetches concatenate them once
   for (const Forest &F : forests) {

@@ -906,7 +906,8 @@ void tower_vm_gather(Context &c, const std::vector<uint32_t> &pos, std::vector<u
   const int64_t q = (int64_t)pos.size();
   out.assign(q, kInvalid);
   if (!q) return;
-  hipStream_t s = c.stream;
+  if (!R.vm_stream) MHGP_HIP(hipStreamCreateWithFlags(&R.vm_stream, hipStreamNonBlocking));
+  hipStream_t s = R.vm_stream;
   R.vm_idx.ensure(q);
   R.vm_out.ensure(q);
   MHGP_HIP(hipMemcpyAsync(R.vm_idx.p, pos.data(), q * 4, hipMemcpyHostToDevice, s));

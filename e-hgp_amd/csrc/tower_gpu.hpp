@@ -46,7 +46,11 @@ struct TowerWork {
   TowerWork &operator=(const TowerWork &) = delete;
   ~TowerWork() {
     if (own_stream && stream) (void)hipStreamDestroy(stream);
+    if (vm_stream) (void)hipStreamDestroy(vm_stream);
   }
+  // the vertical maps' element gathers run on a stream of their own (first
+  // lane only), beside the lanes' work
+  hipStream_t vm_stream = nullptr;
   // records grouped by closed rank: rank r occupies rank_vals[rank_lo[r], rank_hi[r])
   DevBuf<uint64_t> rank_keys;
   DevBuf<uint32_t> rank_vals, rank_bounds;
@@ -90,7 +94,8 @@ void tower_rank_index(Context &c);
 // tower_rank_index) on the lane W; R holds the rank index.
 void tower_order_device(Context &c, TowerWork &W, const TowerWork &R, int k, TowerOrderData &out);
 // Vertical maps: contraction elements at the given rank-index positions
-// (first lane's stream), and host-settled elements written back.
+// (the first lane's gather stream and scratch; one caller at a time), and
+// host-settled elements written back.
 void tower_vm_gather(Context &c, const std::vector<uint32_t> &pos, std::vector<uint32_t> &out);
 void tower_vm_store(Context &c, hipStream_t s, uint32_t first_pos, const std::vector<uint32_t> &elems);
 
