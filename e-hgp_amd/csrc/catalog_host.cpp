@@ -7,6 +7,7 @@
 #include "pipeline.hpp"
 #include "supports.hpp"
 #include "timing.hpp"
+#include "tower_gpu.hpp"
 #include "tower_state.hpp"
 
 namespace mhgp {
@@ -128,6 +129,8 @@ void supports(Context &c, int k_max, mhgp_catalog_info *info) {
     }
   }
   tm.mark("catalogue host fallback");
+  // the tower's neighbour lists run beside the level order
+  if (B.count > 0) tower_prefetch_neighbours(c);
   order_catalogue(c, B);
   info->event_count = B.count - diagnostics;
   info->diagnostic_count = diagnostics;

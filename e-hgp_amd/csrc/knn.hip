@@ -331,12 +331,13 @@ void knn_partition(Context &c, const int64_t *num, const int64_t *den, int64_t Q
   }
 }
 
-void knn_all(Context &c, int k, uint32_t *d_out, uint8_t *d_flags) {
+void knn_all(Context &c, int k, uint32_t *d_out, uint8_t *d_flags, hipStream_t stream) {
   if (k < 1 || k > 32) throw std::invalid_argument("k must lie in 1..32");
   if (k >= c.n) throw std::invalid_argument("k must be smaller than the point count");
-  MHGP_HIP(hipMemsetAsync(d_flags, 0, c.n, c.stream));
-  hipLaunchKernelGGL(k_knn, dim3(blocks_for(c.n, 128)), dim3(128), 0, c.stream, c.n, k, c.nodes.p,
-                     c.LX.p, c.LY.p, c.LZ.p, c.leaf_id.p, d_out, d_flags);
+  hipStream_t s = stream ? stream : c.stream;
+  MHGP_HIP(hipMemsetAsync(d_flags, 0, c.n, s));
+  hipLaunchKernelGGL(k_knn, dim3(blocks_for(c.n, 128)), dim3(128), 0, s, c.n, k, c.nodes.p, c.LX.p, c.LY.p, c.LZ.p,
+                     c.leaf_id.p, d_out, d_flags);
   MHGP_LAUNCH_CHECK();
 }
 

@@ -62,6 +62,11 @@ static void after_upload(Context &c, int64_t n) {
   c.host.reset();
   if (c.catalog) c.catalog->valid = false;
   if (c.tower) c.tower->valid = false;
+  if (c.tower_work) {
+    // lists of the previous cloud: finished (they read its LBVH) and dropped
+    if (c.tower_work->vm_stream) MHGP_HIP(hipStreamSynchronize(c.tower_work->vm_stream));
+    c.tower_work->nn_pending = false;
+  }
   try {
     canonicalize_and_build(c);
   } catch (const std::invalid_argument &e) {

@@ -45,7 +45,8 @@ void closed_ball_noalloc(Context &c, const uint32_t *ids, const uint8_t *sizes, 
 size_t closed_ball_scratch_bytes();
 
 // knn.hip
-void knn_all(Context &c, int k, uint32_t *d_out, uint8_t *d_flags);
+// on `stream` (null: the context stream)
+void knn_all(Context &c, int k, uint32_t *d_out, uint8_t *d_flags, hipStream_t stream = nullptr);
 // k nearest neighbours to host memory, near ties settled exactly on the host
 void knn_host(Context &c, int k, uint32_t *ids_out);
 // exact top-k partitions of rational query points

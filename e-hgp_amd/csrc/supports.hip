@@ -1106,17 +1106,21 @@ __global__ void __launch_bounds__(kBlock) k_anchor_big(Params P) {
   anchor_work<true>(P, w, u);
 }
 
-// Regions measured by the global-memory path's probe that fit the wide LDS
-// caps: the anchor's compacted region (coordinates, PointIds, masks) and cell
-// radii are loaded from its slab into LDS, and the face phase runs there with
-// the wave queues.  Same parameters and slab layout as k_anchor_big's probe
-// launch; anchors not probed, or with larger regions, are left to it.
-__global__ void __launch_bounds__(kBlock, 2) k_anchor_region(Params P) {
-  constexpr int RCAP = kRegionCapWide;
+// Regions measured by the global-memory path's probe that fit LDS: the
+// anchor's compacted region (coordinates, PointIds, masks) and cell radii are
+// loaded from its slab into LDS, and the face phase runs there with the wave
+// queues.  Two sizes: regions of up to 640 points (two workgroups per CU) and
+// of 641 to 1 023 points (one workgroup per CU, 93 KB of LDS; region slots
+// stay below 1 024, as the packed queue items need).  Same parameters and
+// slab layout as k_anchor_big's probe launch; anchors not probed, or with
+// larger regions, are left to it.
+constexpr int kRegionCapHuge = 1023;
+template <int RCAP, int RMIN>
+__global__ void __launch_bounds__(kBlock, RCAP <= kRegionCapWide ? 2 : 1) k_anchor_region(Params P) {
   const uint32_t a = blockIdx.x;
   if (a >= P.anchor_count) return;
   const uint32_t u = P.anchors[a];
-  if (P.status[u] != ST_PROBED || !(P.need[u] <= (double)RCAP)) return;
+  if (P.status[u] != ST_PROBED || !(P.need[u] <= (double)RCAP) || !(P.need[u] > (double)RMIN)) return;
   const uint32_t M = (uint32_t)P.need[u];
   const uint64_t G = P.gcap;
   const double *slab = P.scratch + (uint64_t)a * P.slab_doubles;
@@ -1894,7 +1898,10 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
             Q.scratch = B.scratch.p; Q.slab_doubles = slab_of(g); Q.gcap = g; Q.rcap = g;
             hipLaunchKernelGGL(k_anchor_big, dim3(bn), dim3(kBlock), 0, s, Q);
             // probed regions that fit LDS: enumerated there, from this chunk's slabs
-            if (probe) hipLaunchKernelGGL(k_anchor_region, dim3(bn), dim3(kBlock), 0, s, Q);
+            if (probe) {
+              hipLaunchKernelGGL((k_anchor_region<kRegionCapWide, 0>), dim3(bn), dim3(kBlock), 0, s, Q);
+              hipLaunchKernelGGL((k_anchor_region<kRegionCapHuge, kRegionCapWide>), dim3(bn), dim3(kBlock), 0, s, Q);
+            }
             MHGP_LAUNCH_CHECK();
           }
           MHGP_HIP(hipMemcpyAsync(st.data(), B.status.p, n * 4, hipMemcpyDeviceToHost, s));

@@ -921,6 +921,29 @@ void tower_vm_store(Context &c, hipStream_t s, uint32_t first_pos, const std::ve
   copy_wait(c.tower_work->vm_elem.p + first_pos, elems.data(), elems.size() * 4, hipMemcpyHostToDevice, s);
 }
 
+void tower_prefetch_neighbours(Context &c) {
+  if (c.n <= kNearK) return;
+  if (!c.tower_work) c.tower_work = std::make_unique<TowerWork>();
+  TowerWork &W = *c.tower_work;
+  if (!W.vm_stream) MHGP_HIP(hipStreamCreateWithFlags(&W.vm_stream, hipStreamNonBlocking));
+  if (!W.nn_start) {
+    MHGP_HIP(hipEventCreateWithFlags(&W.nn_start, hipEventDisableTiming));
+    MHGP_HIP(hipEventCreateWithFlags(&W.nn_done, hipEventDisableTiming));
+  }
+  W.nn.ensure((size_t)c.n * kNearK);
+  W.nn_d2.ensure(c.n);
+  W.nn_flags.ensure(c.n);
+  // the side stream starts after the context stream's work so far (the LBVH)
+  MHGP_HIP(hipEventRecord(W.nn_start, c.stream));
+  MHGP_HIP(hipStreamWaitEvent(W.vm_stream, W.nn_start, 0));
+  knn_all(c, kNearK, W.nn.p, W.nn_flags.p, W.vm_stream);
+  hipLaunchKernelGGL(k_nn_last_d2, dim3(blocks_for(c.n, 256)), dim3(256), 0, W.vm_stream, c.n, W.nn.p, W.nn_flags.p,
+                     c.X.p, c.Y.p, c.Z.p, W.nn_d2.p);
+  MHGP_LAUNCH_CHECK();
+  MHGP_HIP(hipEventRecord(W.nn_done, W.vm_stream));
+  W.nn_pending = true;
+}
+
 void tower_rank_index(Context &c) {
   CatalogBuffers &B = *c.catalog;
   hipStream_t s = c.stream;
@@ -933,16 +956,13 @@ void tower_rank_index(Context &c) {
   W.rank_keys.ensure(n);
   W.rank_vals.ensure(n);
   W.vm_elem.ensure(n);
-  // neighbour lists of the points, for the descents' ball queries
+  // neighbour lists of the points, for the descents' ball queries: started
+  // beside the level order, or computed here
   W.nn_ready = false;
   if (c.n > kNearK) {
-    W.nn.ensure((size_t)c.n * kNearK);
-    W.nn_d2.ensure(c.n);
-    W.nn_flags.ensure(c.n);
-    knn_all(c, kNearK, W.nn.p, W.nn_flags.p);
-    hipLaunchKernelGGL(k_nn_last_d2, dim3(blocks_for(c.n, 256)), dim3(256), 0, s, c.n, W.nn.p, W.nn_flags.p, c.X.p,
-                       c.Y.p, c.Z.p, W.nn_d2.p);
-    MHGP_LAUNCH_CHECK();
+    if (!W.nn_pending) tower_prefetch_neighbours(c);
+    MHGP_HIP(hipStreamWaitEvent(s, W.nn_done, 0));
+    W.nn_pending = false;
     W.nn_ready = true;
   }
   W.rank_bounds.ensure(512);

@@ -45,12 +45,17 @@ struct TowerWork {
   TowerWork(const TowerWork &) = delete;
   TowerWork &operator=(const TowerWork &) = delete;
   ~TowerWork() {
+    if (vm_stream) (void)hipStreamSynchronize(vm_stream);
+    if (nn_start) (void)hipEventDestroy(nn_start);
+    if (nn_done) (void)hipEventDestroy(nn_done);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
     if (vm_stream) (void)hipStreamDestroy(vm_stream);
   }
-  // the vertical maps' element gathers run on a stream of their own (first
-  // lane only), beside the lanes' work
+  // the side stream of the first lane: the neighbour lists, computed beside
+  // the catalogue's level order, and the vertical maps' element gathers
   hipStream_t vm_stream = nullptr;
+  hipEvent_t nn_start = nullptr, nn_done = nullptr;
+  bool nn_pending = false;  // lists launched on the side stream for the current cloud
   // records grouped by closed rank: rank r occupies rank_vals[rank_lo[r], rank_hi[r])
   DevBuf<uint64_t> rank_keys;
   DevBuf<uint32_t> rank_vals, rank_bounds;
@@ -88,6 +93,9 @@ struct TowerWork {
   }
 };
 
+// Starts the neighbour lists of the descents on the side stream (after the
+// LBVH, before the catalogue's level order), so that they overlap it.
+void tower_prefetch_neighbours(Context &c);
 // Groups the catalogue's records by closed rank (once per tower call).
 void tower_rank_index(Context &c);
 // Resolves the arms of order k and reduces its saddles (after
