@@ -163,6 +163,42 @@ def test_tower_lattice90():
     _check(workloads.jittered_lattice(90, side=5, seed=44), 5, "lattice90")
 
 
+def test_plain_reference_configuration_uniform256_k3():
+    """The plain CPU reference configuration (256 points uniform in [0, 1)^3,
+    K_max 3): EMST edges, catalogue records and the hgp_reduced forests equal
+    the exact oracle's.  The oracle's catalogue is its per-anchor exhaustive
+    enumeration with radius 4, which encloses every well-centred sphere of
+    points of the unit cube; the forests are its direct construction."""
+    pts = workloads.uniform_cloud(256, seed=0)
+    canon = oracle.canonicalize(pts)
+    print("plain reference configuration: uniform n=256 k=3, oracle", flush=True)
+    ref_edges = np.asarray(oracle.emst(canon), dtype=np.int64)
+    recs = []
+    for a in range(len(canon)):
+        recs += oracle.anchor_records(canon, 3, a, 4.0)
+    ref = otower.direct_forests(canon, 3, events=recs)
+    print("plain reference configuration: GPU", flush=True)
+    with mhgp.Context(0) as ctx:
+        ctx.upload(pts)
+        edges = ctx.emst()
+        cat = ctx.supports(3)
+        t = ctx.tower(3)
+    assert np.array_equal(np.asarray(edges, dtype=np.int64), ref_edges)
+    exact = {(r.support, r.interior): (r.rank, r.level) for r in recs}
+    mine = {}
+    for i in range(len(cat.size)):
+        key = (tuple(int(x) for x in cat.ids[i, : cat.size[i]]), cat.interior[i])
+        mine[key] = (int(cat.rank[i]), int(cat.level_class[i]))
+    assert set(mine) == set(exact)
+    levels = {-1: 0}
+    for key, (rank, cls) in mine.items():
+        assert rank == exact[key][0], key
+        assert levels.setdefault(cls, exact[key][1]) == exact[key][1], key  # a class is one exact level
+    sigs = _signatures(t, levels)
+    for k in (1, 2, 3):
+        assert sigs.get(k, []) == otower.signatures(ref[k]), k
+
+
 def test_vertical_growth_target_regression():
     """tests/fixtures/regressions/vertical_q1_growth_target.json: the order-3
     component at level 4 maps onto the order-2 root born at level 1 covering
