@@ -306,7 +306,7 @@ void emst_host(Context &c, uint32_t *edges_out) {
   B.comp_ce.ensure(n);
   B.edges.ensure(2 * n);
   B.counters.ensure(4);
-  DevBuf<uint32_t> hook;
+  DevBuf<uint32_t> &hook = B.hook;
   hook.ensure(n);
   hipLaunchKernelGGL(k_init_labels, dim3(G), dim3(T), 0, s, n, B.label.p);
   MHGP_HIP(hipMemsetAsync(B.counters.p, 0, 4 * sizeof(uint32_t), s));

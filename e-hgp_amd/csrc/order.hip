@@ -217,7 +217,7 @@ void order_catalogue(Context &c, CatalogBuffers &B) {
   const uint64_t cap = std::max<int64_t>(1 << 16, n / 4);
   B.ambiguous.ensure(3 * cap);
   uint32_t *starts = B.ambiguous.p, *host_start = starts + cap, *host_len = host_start + cap;
-  DevBuf<uint64_t> cnt;
+  DevBuf<uint64_t> &cnt = B.run_count;
   cnt.ensure(2);
   MHGP_HIP(hipMemsetAsync(cnt.p, 0, 16, s));
   hipLaunchKernelGGL(k_run_starts, dim3(blocks_for(n, kRunTile)), dim3(256), 0, s, B.cls.p, n, starts,
@@ -249,8 +249,8 @@ void order_catalogue(Context &c, CatalogBuffers &B) {
     std::vector<uint32_t> pos(T);
     for (uint64_t r = 0; r < nh; ++r)
       for (uint32_t k = 0; k < hl[r]; ++k) pos[off[r] + k] = hs[r] + k;
-    DevBuf<uint32_t> dpos, dgidx, dnew, dlink;
-    DevBuf<EventRec> drec;
+    DevBuf<uint32_t> &dpos = B.run_pos, &dgidx = B.run_gidx, &dnew = B.run_new, &dlink = B.run_link;
+    DevBuf<EventRec> &drec = B.run_rec;
     dpos.ensure(T); dgidx.ensure(T); dnew.ensure(T); dlink.ensure(T); drec.ensure(T);
     MHGP_HIP(hipMemcpyAsync(dpos.p, pos.data(), T * 4, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_gather_runs, dim3(blocks_for(T, 256)), dim3(256), 0, s, B.events.p, B.order.p, dpos.p,

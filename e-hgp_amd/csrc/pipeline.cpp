@@ -32,6 +32,7 @@ int64_t device_bytes(const Context &c) {
               c.LX.bytes() * 3 + c.path.bytes() + c.nodes.bytes() + c.parent_int.bytes() +
               c.parent_leaf.bytes() + c.wit.bytes() + c.counter.bytes() + c.wide.bytes();
   if (c.catalog) b += catalog_device_bytes(*c.catalog);
+  if (c.emst) b += c.emst->bytes();
   if (c.tower_work) b += c.tower_work->bytes();
   for (const auto &w : c.tower_lanes) b += w->bytes();
   return b;

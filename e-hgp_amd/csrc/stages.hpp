@@ -28,6 +28,13 @@ struct EmstBuffers {
   DevBuf<uint32_t> counters;
   DevBuf<uint64_t> kappa_key;      // FP64 squared length bits of each edge
   DevBuf<uint32_t> kappa_idx;      // edge index, sorted with the keys
+  DevBuf<uint32_t> hook;           // per component: the component it hooks to this round
+  int64_t bytes() const {
+    return (int64_t)(label.bytes() + next_label.bytes() + node_label.bytes() + best_v.bytes() + best_d.bytes() +
+                     best_flag.bytes() + comp_d.bytes() + comp_e.bytes() + comp_amb.bytes() + comp_cw.bytes() +
+                     comp_ce.bytes() + edges.bytes() + counters.bytes() + kappa_key.bytes() + kappa_idx.bytes() +
+                     hook.bytes());
+  }
 };
 
 struct CatalogBuffers;

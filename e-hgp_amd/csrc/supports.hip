@@ -53,7 +53,9 @@ namespace mhgp {
 int64_t catalog_device_bytes(const CatalogBuffers &b) {
   return b.radius.bytes() + b.gcount.bytes() + b.goff.bytes() + b.glist.bytes() + b.need.bytes() +
          b.status.bytes() + b.events.bytes() + b.fallbacks.bytes() + b.fb_region.bytes() +
-         b.counters.bytes() + b.cells.bytes() + b.scratch.bytes() + b.anchors.bytes();
+         b.counters.bytes() + b.cells.bytes() + b.scratch.bytes() + b.anchors.bytes() + b.overflow.bytes() +
+         b.total.bytes() + b.run_count.bytes() + b.run_pos.bytes() + b.run_gidx.bytes() + b.run_new.bytes() +
+         b.run_link.bytes() + b.run_rec.bytes();
 }
 
 namespace {
@@ -1703,8 +1705,8 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
   // initial gather radius: twice the distance to the min(32, n-1)-th neighbour
   const int kq = (int)std::min<int64_t>(32, n - 1);
   StageTimer tm(s);
-  DevBuf<uint32_t> overflow;
-  DevBuf<uint64_t> total;
+  DevBuf<uint32_t> &overflow = B.overflow;
+  DevBuf<uint64_t> &total = B.total;
   overflow.ensure(1);
   total.ensure(1);
   MHGP_HIP(hipMemsetAsync(overflow.p, 0, 4, s));

@@ -62,6 +62,13 @@ struct CatalogBuffers {
   DevBuf<uint32_t> order;
   DevBuf<uint32_t> cls;
   DevBuf<uint32_t> ambiguous;     // open-run starts, and the runs left to the host (start, length)
+  // small per-call device scratch, kept so that no call frees device memory
+  // (a free waits for the whole device, side streams included)
+  DevBuf<uint32_t> overflow;      // gather traversal overflow flag
+  DevBuf<uint64_t> total;         // gathered entries of a pass
+  DevBuf<uint64_t> run_count;     // open runs, runs left to the host
+  DevBuf<uint32_t> run_pos, run_gidx, run_new, run_link;
+  DevBuf<EventRec> run_rec;       // records of the runs ordered on the host
   int64_t open_runs = 0;          // runs the FP64 intervals left open
   int64_t host_runs = 0;          // of those, runs ordered exactly on the host
   mhgp_catalog_info info{};
