@@ -371,6 +371,26 @@ struct Capsule {
   }
 };
 
+// The face passes scan the region in chunks of 64 slots: first the capsule
+// test of every slot of the chunk (a bit mask), then the point work for the
+// set bits only, in ascending slot order, exactly the order of a plain scan.
+// The lanes of a wave run the point work as often as the busiest lane of the
+// chunk needs it, instead of once for every slot that any lane needs.
+// tests counts the slots tested (the face's own vertices excluded).
+__device__ inline uint64_t capsule_chunk(const Work &w, const Capsule &cap, uint32_t s0, uint32_t M, uint32_t vi,
+                                         uint32_t wi, uint32_t &tests) {
+  const uint32_t cn = min(64u, M - s0);
+  uint64_t in = 0;
+#pragma unroll 4
+  for (uint32_t j = 0; j < cn; ++j) {
+    const uint32_t gi = s0 + j;
+    const bool keep = gi != vi && gi != wi && !cap.outside(gpt(w, gi));
+    in |= (uint64_t)keep << j;
+  }
+  tests += cn - (vi - s0 < cn ? 1u : 0u) - (wi - s0 < cn ? 1u : 0u);
+  return in;
+}
+
 // Per-wave work queues of the face phase (LDS).  The first pass of every
 // face runs one face per lane; the per-side passes and the tetrahedra it
 // leads to are queued and then taken one per lane, so the lanes of a wave
@@ -399,30 +419,32 @@ __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uin
   beh.init();
   bool undecided = false;
   uint32_t tests = 0, evals = 0;
+  // a point outside every feasible sphere of the pencil can neither be an
+  // apex nor be swept before one (dropping it only loosens the threshold)
 #pragma unroll 1
-  for (uint32_t s = 0; s < M; ++s) {
-    const uint32_t gi = s;
-    if (gi == vi || gi == wi) continue;
-    const V3 p = gpt(w, gi);
-    // outside every feasible sphere of the pencil: it can neither be an apex
-    // nor be swept before one (dropping it only loosens the threshold)
-    tests++;
-    if (cap.outside(p)) continue;
-    evals++;
-    double Pw, PA, H, HA;
-    f.power(p, Pw, PA);
-    f.height(p, H, HA);
-    const int sp = cert_sign(Pw, PA), sh = cert_sign(H, HA);
-    undecided |= sp == 0 || sh == 0;
-    const bool ahead = side == 0 ? sh > 0 : sh < 0;
-    double tau, terr;
-    sweep_key(Pw, PA, H, HA, tau, terr);
-    if (sp < 0) {
-      // inside the triangle's sphere, behind this side: inside the pencil's
-      // spheres until the sweep key reaches -tau
-      if (!ahead) beh.insert(-tau - terr);
-    } else if (ahead) {
-      top.insert(tau, terr, s);
+  for (uint32_t s0 = 0; s0 < M; s0 += 64) {
+    uint64_t in = capsule_chunk(w, cap, s0, M, vi, wi, tests);
+#pragma unroll 1
+    while (in) {
+      const uint32_t s = s0 + (uint32_t)(__ffsll((long long)in) - 1);
+      in &= in - 1;
+      const V3 p = gpt(w, s);
+      evals++;
+      double Pw, PA, H, HA;
+      f.power(p, Pw, PA);
+      f.height(p, H, HA);
+      const int sp = cert_sign(Pw, PA), sh = cert_sign(H, HA);
+      undecided |= sp == 0 || sh == 0;
+      const bool ahead = side == 0 ? sh > 0 : sh < 0;
+      double tau, terr;
+      sweep_key(Pw, PA, H, HA, tau, terr);
+      if (sp < 0) {
+        // inside the triangle's sphere, behind this side: inside the
+        // pencil's spheres until the sweep key reaches -tau
+        if (!ahead) beh.insert(-tau - terr);
+      } else if (ahead) {
+        top.insert(tau, terr, s);
+      }
     }
   }
   add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * (kFlopFacePoint + kFlopSweepKey));
@@ -529,33 +551,37 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
   int total_in = 0;
   uint32_t tri_int[kMaxInterior];
   uint32_t tests = 0, evals = 0;
+  // points outside the capsule are outside the triangle's sphere and every
+  // feasible one
 #pragma unroll 1
-  for (uint32_t s = 0; s < M; ++s) {
-    const uint32_t gi = s;
-    if (gi == vi || gi == wi) continue;
-    const V3 p = gpt(w, gi);
-    tests++;
-    if (cap.outside(p)) continue;  // outside the triangle's sphere and every feasible one
-    evals++;
-    double Pw, PA, H, HA;
-    f.power(p, Pw, PA);
-    f.height(p, H, HA);
-    const int sp = cert_sign(Pw, PA), sh = cert_sign(H, HA);
-    // every sign the side passes use is certified here, before anything is emitted
-    if (sp == 0 || sh == 0) {
-      add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopFacePoint);
-      emit_fallback(P, w, u, 3, ids, 3, M, region_slot);
-      return;
-    }
-    if (sp > 0) continue;
-    put_static(tri_int, total_in, w.gid[gi]);
-    total_in++;
-    cin0 += sh > 0;
-    cin1 += sh < 0;
-    // both sides saturated and the triangle out of rank: nothing left here
-    if (cin0 >= K && cin1 >= K && total_in + 3 > smax) {
-      add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopFacePoint);
-      return;
+  for (uint32_t s0 = 0; s0 < M; s0 += 64) {
+    uint64_t in = capsule_chunk(w, cap, s0, M, vi, wi, tests);
+#pragma unroll 1
+    while (in) {
+      const uint32_t gi = s0 + (uint32_t)(__ffsll((long long)in) - 1);
+      in &= in - 1;
+      const V3 p = gpt(w, gi);
+      evals++;
+      double Pw, PA, H, HA;
+      f.power(p, Pw, PA);
+      f.height(p, H, HA);
+      const int sp = cert_sign(Pw, PA), sh = cert_sign(H, HA);
+      // every sign the side passes use is certified here, before anything is emitted
+      if (sp == 0 || sh == 0) {
+        add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopFacePoint);
+        emit_fallback(P, w, u, 3, ids, 3, M, region_slot);
+        return;
+      }
+      if (sp > 0) continue;
+      put_static(tri_int, total_in, w.gid[gi]);
+      total_in++;
+      cin0 += sh > 0;
+      cin1 += sh < 0;
+      // both sides saturated and the triangle out of rank: nothing left here
+      if (cin0 >= K && cin1 >= K && total_in + 3 > smax) {
+        add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopFacePoint);
+        return;
+      }
     }
   }
   add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopFacePoint);
