@@ -33,6 +33,7 @@
 // host fallback queue with the anchor's region.
 #include <algorithm>
 #include <cmath>
+#include <cfloat>
 #include <cstdio>
 #include <condition_variable>
 #include <map>
@@ -53,7 +54,7 @@ namespace mhgp {
 int64_t catalog_device_bytes(const CatalogBuffers &b) {
   return b.radius.bytes() + b.gcount.bytes() + b.goff.bytes() + b.glist.bytes() + b.need.bytes() +
          b.status.bytes() + b.events.bytes() + b.fallbacks.bytes() + b.fb_region.bytes() +
-         b.counters.bytes() + b.cells.bytes() + b.scratch.bytes() + b.anchors.bytes() + b.overflow.bytes() +
+         b.counters.bytes() + b.cells.bytes() + b.cellf.bytes() + b.scratch.bytes() + b.anchors.bytes() + b.overflow.bytes() +
          b.total.bytes() + b.run_count.bytes() + b.run_pos.bytes() + b.run_gidx.bytes() + b.run_new.bytes() +
          b.run_link.bytes() + b.run_rec.bytes();
 }
@@ -86,6 +87,7 @@ struct Params {
   const uint32_t *glist;
   const double *radius;
   const double *cells;           // 8 doubles per cell
+  const float *cellf;            // 8 floats per cell: the FP32 cell (see cell_f)
   double blo[3], bhi[3];         // cloud AABB
   int s_max;
   double *need;
@@ -111,7 +113,7 @@ struct Params {
 struct Work {
   double *gx, *gy, *gz;
   uint32_t *gid;
-  double *rho;
+  float *rho;        // certificate radius per cell (FP32, rounded up)
   uint32_t *ridx;    // gathered index of each region slot (until the points are compacted)
   uint64_t *rmask;  // 6 words per member
   uint32_t *kidx;
@@ -633,7 +635,8 @@ __device__ void tet_item(const Params &P, const Work &w, uint32_t u, uint32_t M,
 
 // Cell direction and angular radius in FP32, rounded so that the cell only
 // grows (cos delta down, sin delta up): every FP32 test below errs towards a
-// larger certificate radius and a larger region.  Reciprocals and square
+// larger certificate radius and a larger region (the table is rounded on
+// the host, see make_cells_f).  Reciprocals and square
 // roots use the hardware approximations (1 ulp); their error, like the
 // rounding of the products, stays far inside the slack of the tests (1e-5 on
 // the mask cosines, 1e-4 on the certificate radius; cancellation is bounded
@@ -643,9 +646,9 @@ constexpr float kTinyLen = 1e-30f;
 struct CellF {
   float dx, dy, dz, cosd, sind;
 };
-__device__ inline CellF cell_f(const double *cd) {
-  return {(float)cd[0], (float)cd[1], (float)cd[2], (float)cd[3] * (1.0f - 1e-6f),
-          (float)cd[4] * (1.0f + 1e-6f) + 1e-7f};
+__device__ inline CellF cell_f(const float *cf) {
+  const float4 a = *reinterpret_cast<const float4 *>(cf);
+  return {a.x, a.y, a.z, a.w, cf[4]};
 }
 
 // Cells whose union of closed tangent balls (radius rho'_C) contains gathered
@@ -671,12 +674,12 @@ __device__ bool member_mask(const Params &P, const Work &w, uint32_t i, float rm
 #pragma unroll 1
     for (int b = 0; b < 64; ++b) {
       const int c = 64 * wd + b;
-      const CellF cf = cell_f(P.cells + 8 * c);
+      const CellF cf = cell_f(P.cellf + 8 * c);
       const float ct = (q.x * cf.dx + q.y * cf.dy + q.z * cf.dz) * inv;
       float cm;  // cos(max(theta - delta, 0))
       if (ct >= cf.cosd) cm = 1.0f;
       else cm = ct * cf.cosd + __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - ct * ct)) * cf.sind;
-      const float rho = (float)w.rho[c];
+      const float rho = w.rho[c];
       if (q.w <= 2.0f * rho * (cm + 1e-5f) * (1.0f + 1e-5f)) bits |= 1ull << b;
     }
     mask[wd] = bits;
@@ -895,7 +898,7 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   const bool by_distance = G <= 4 * blockDim.x;
   for (int c = t; c < kCells; c += blockDim.x) {
     const double *cd = P.cells + 8 * c;
-    const CellF cf = cell_f(cd);
+    const CellF cf = cell_f(P.cellf + 8 * c);
     const double chord = cd[5];
     float best[kCoverCap];
 #pragma unroll
@@ -928,7 +931,11 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
       else if (hi < 0) e = fmin(e, (P.blo[a] - uu[a]) / hi);
     }
     e = fmax(e, 0.0) * (1.0 + 1e-9) + 1e-300;
-    w.rho[c] = fmin(rc, e);
+    // stored in FP32, rounded up (a larger radius only enlarges the region)
+    const double rd = fmin(rc, e);
+    float rf = (float)rd;
+    if (!((double)rf >= rd)) rf = rd <= (double)FLT_MIN ? FLT_MIN : nextafterf(rf, INFINITY);
+    w.rho[c] = rf;
   }
   __syncthreads();
   __shared__ float s_rmax2;
@@ -1044,7 +1051,7 @@ template <int GCAP, int RCAP>
 __device__ void anchor_lds(const Params &P, uint32_t a) {
   __shared__ double gx[GCAP], gy[GCAP], gz[GCAP];
   __shared__ uint32_t gid[GCAP];
-  __shared__ double rho[kCells];
+  __shared__ float rho[kCells];
   __shared__ uint32_t ridx[RCAP];
   // the FP32 offsets (16 B per gathered point) live in the mask area until
   // the masks are written
@@ -1119,7 +1126,7 @@ __global__ void __launch_bounds__(kBlock) k_anchor_big(Params P) {
   w.gx = slab;
   w.gy = slab + G;
   w.gz = slab + 2 * G;
-  w.rho = slab + 3 * G;
+  w.rho = reinterpret_cast<float *>(slab + 3 * G);
   w.rmask = (uint64_t *)(slab + 3 * G + kCells);
   w.gid = (uint32_t *)(slab + 3 * G + kCells + 6 * G);
   w.ridx = w.gid + G;
@@ -1156,7 +1163,7 @@ __global__ void __launch_bounds__(kBlock, RCAP <= kRegionCapWide ? 2 : 1) k_anch
   const uint32_t *sgid = (const uint32_t *)(slab + 3 * G + kCells + 6 * G);
   __shared__ double gx[RCAP], gy[RCAP], gz[RCAP];
   __shared__ uint32_t gid[RCAP];
-  __shared__ double rho[kCells];
+  __shared__ float rho[kCells];
   __shared__ uint64_t rmask[6 * RCAP];
   __shared__ uint32_t kidx[RCAP];
   __shared__ uint32_t cand, sK, sregion;
@@ -1169,7 +1176,7 @@ __global__ void __launch_bounds__(kBlock, RCAP <= kRegionCapWide ? 2 : 1) k_anch
     gid[i] = sgid[i];
   }
   for (uint32_t i = t; i < 6 * M; i += blockDim.x) rmask[i] = smask[i];
-  for (int c = t; c < kCells; c += blockDim.x) rho[c] = slab[3 * G + c];
+  for (int c = t; c < kCells; c += blockDim.x) rho[c] = reinterpret_cast<const float *>(slab + 3 * G)[c];
   if (t == 0) { cand = 0; sK = 0; sregion = kInvalid; flops = 0; }
   __syncthreads();
   Work w{gx, gy, gz, gid, rho, kidx, rmask, kidx, nullptr, &cand, P.count_flops ? &flops : nullptr};
@@ -1495,6 +1502,23 @@ static std::vector<double> make_cells() {
   return out;
 }
 
+// The FP32 cells of the certificate and the membership masks (8 floats per
+// cell: direction xyz, cos delta rounded down, sin delta rounded up, padding),
+// so that every FP32 test errs towards a larger cell.
+static std::vector<float> make_cells_f(const std::vector<double> &cells) {
+  std::vector<float> out(8 * kCells, 0.0f);
+  for (int c = 0; c < kCells; ++c) {
+    const double *cd = cells.data() + 8 * c;
+    float *o = out.data() + 8 * c;
+    o[0] = (float)cd[0];
+    o[1] = (float)cd[1];
+    o[2] = (float)cd[2];
+    o[3] = (float)cd[3] * (1.0f - 1e-6f);
+    o[4] = (float)cd[4] * (1.0f + 1e-6f) + 1e-7f;
+  }
+  return out;
+}
+
 void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
                      std::vector<FallbackRec> &fbs, std::vector<uint32_t> &fb_region,
                      mhgp_catalog_info *info);
@@ -1727,6 +1751,9 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     auto cells = make_cells();
     B.cells.ensure(8 * kCells);
     copy_wait(B.cells.p, cells.data(), cells.size() * 8, hipMemcpyHostToDevice, s);
+    const std::vector<float> cf = make_cells_f(cells);
+    B.cellf.ensure(8 * kCells);
+    copy_wait(B.cellf.p, cf.data(), cf.size() * 4, hipMemcpyHostToDevice, s);
   }
   // initial gather radius: twice the distance to the min(32, n-1)-th neighbour
   const int kq = (int)std::min<int64_t>(32, n - 1);
@@ -1800,6 +1827,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     P.X = c.X.p; P.Y = c.Y.p; P.Z = c.Z.p;
     P.gcount = B.gcount.p; P.goff = B.goff.p; P.glist = B.glist.p; P.radius = B.radius.p;
     P.cells = B.cells.p;
+    P.cellf = B.cellf.p;
     for (int a = 0; a < 3; ++a) { P.blo[a] = bounds[a]; P.bhi[a] = bounds[3 + a]; }
     P.s_max = smax;
     P.need = B.need.p; P.status = B.status.p;

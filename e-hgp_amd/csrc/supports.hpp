@@ -53,6 +53,7 @@ struct CatalogBuffers {
   DevBuf<uint32_t> fb_region;     // region member ids of anchors with fallbacks
   DevBuf<uint64_t> counters;      // events, fallbacks, region words, candidates
   DevBuf<double> cells;           // per cell: dir xyz, cos d, sin d, chord
+  DevBuf<float> cellf;            // per cell: the FP32 cell of the certificate and the masks
   DevBuf<double> scratch;         // global slabs for anchors too big for LDS
   DevBuf<uint32_t> anchors;       // anchor list of the current launch
   // canonical order on the device: records (device emitted + host settled),
