@@ -453,7 +453,9 @@ __device__ void face_side_apexes(const Params &P, const Work &w, uint32_t u, uin
   // the face pass certified every sign with the same arithmetic
   if (undecided) return;
   const double thr = top.threshold(need);
-  if (top.evicted_lo <= thr) {
+  // evicted_lo stays +inf while no point has left or missed the list (every
+  // candidate is then listed, also when fewer than `need` are: thr = +inf)
+  if (top.evicted_lo != INFINITY && top.evicted_lo <= thr) {
     // a point outside the list may still qualify (near-equal keys): full scan
     tests = evals = 0;
 #pragma unroll 1
