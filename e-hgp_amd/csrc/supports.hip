@@ -333,6 +333,64 @@ struct BehindList {
   }
 };
 
+// The same two lists in FP32, filled for both sides during a face's first
+// pass: key bounds rounded outwards (lo down, hi up), so that every
+// threshold test below stays conservative (wider intervals only admit more
+// apex candidates, which are classified exactly).
+struct SweepListF {
+  static constexpr int kCap = 8;  // >= s_max - 3 for s_max <= 11
+  float hi[kCap];                 // upper key bounds, ascending
+  float lo[kCap];                 // lower key bounds
+  uint32_t slot[kCap];
+  float evicted_lo;  // smallest lower key bound of a point not in the list
+  __device__ void init() {
+#pragma unroll
+    for (int j = 0; j < kCap; ++j) { hi[j] = INFINITY; lo[j] = INFINITY; slot[j] = kInvalid; }
+    evicted_lo = INFINITY;
+  }
+  __device__ void insert(float h, float l, uint32_t s) {
+    if (!(h < hi[kCap - 1])) {
+      evicted_lo = fminf(evicted_lo, l);
+      return;
+    }
+    evicted_lo = fminf(evicted_lo, lo[kCap - 1]);
+#pragma unroll
+    for (int j = kCap - 1; j > 0; --j) {
+      const bool shift = h < hi[j - 1];
+      const bool place = !shift && h < hi[j];
+      hi[j] = shift ? hi[j - 1] : (place ? h : hi[j]);
+      lo[j] = shift ? lo[j - 1] : (place ? l : lo[j]);
+      slot[j] = shift ? slot[j - 1] : (place ? s : slot[j]);
+    }
+    if (h < hi[0]) { hi[0] = h; lo[0] = l; slot[0] = s; }
+  }
+  __device__ float threshold(int need) const {
+    float r = INFINITY;
+#pragma unroll
+    for (int j = 0; j < kCap; ++j) r = (j == need - 1) ? hi[j] : r;
+    return r;
+  }
+};
+struct BehindListF {
+  static constexpr int kCap = 4;
+  float v[kCap];
+  __device__ void init() {
+#pragma unroll
+    for (int j = 0; j < kCap; ++j) v[j] = -INFINITY;
+  }
+  __device__ void insert(float x) {
+#pragma unroll
+    for (int j = kCap - 1; j > 0; --j) v[j] = x > v[j - 1] ? v[j - 1] : (x > v[j] ? x : v[j]);
+    v[0] = fmaxf(v[0], x);
+  }
+  __device__ int count_above(float x) const {
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < kCap; ++j) c += v[j] > x;
+    return c;
+  }
+};
+
 // Sweep key of a point against the face (tau = P / |H|) and its error bound.
 // The reciprocal of |H| is the hardware estimate refined by two Newton
 // steps; the residual e = 1 - |H| r of the result bounds its relative error
@@ -354,6 +412,15 @@ __device__ inline void sweep_key(double Pw, double PA, double H, double HA, doub
     tau = 0.0;
     terr = INFINITY;
   }
+}
+
+// FP32 bounds of the key interval [tau - terr, tau + terr], rounded outwards.
+__device__ inline void key_bounds_f(double tau, double terr, float &hf, float &lf) {
+  const double hd = tau + terr, ld = tau - terr;
+  hf = (float)hd;
+  if (!((double)hf >= hd)) hf = nextafterf(hf, INFINITY);
+  lf = (float)ld;
+  if (!((double)lf <= ld)) lf = nextafterf(lf, -INFINITY);
 }
 
 // Apexes of one side of a face: the outside points ahead of the side whose
@@ -524,10 +591,55 @@ __device__ inline Capsule face_capsule(const TriSphere &f, double rmax) {
   return cap;
 }
 
+// Apexes of one side from the FP32 lists its face's first pass filled (the
+// tail of face_side_apexes).  When a point left or missed the list with a
+// lower bound within the threshold, the side takes its own pass instead
+// (queued, with the full scan that case needs).
+__device__ inline void fused_side(const Params &P, const Work &w, uint32_t u, uint32_t M, uint32_t vi, uint32_t wi,
+                                  uint32_t wid, int side, int cin_side, int K, const SweepListF &top,
+                                  const BehindListF &beh, double rmax, uint32_t *region_slot, WaveQueues *q) {
+  const int need = K - cin_side;
+  const float thr = top.threshold(need);
+  if (top.evicted_lo != INFINITY && top.evicted_lo <= thr) {
+    // at most 2 items for each of the wave's 64 faces: the queue cannot fill
+    const uint32_t at = atomicAdd(q->nside, 1u);
+    if (at < kSideQ) {
+      q->side[at] = vi | (wi << 10) | ((uint32_t)side << 20) | ((uint32_t)cin_side << 21);
+      q->side_r[at] = (float)rmax;  // exact: rmax is a float value
+    } else {
+      atomicAdd(&P.counters[4], 1ull);  // reported by the host as an internal error
+    }
+    return;
+  }
+  uint32_t cand = 0;
+#pragma unroll
+  for (int j = 0; j < SweepListF::kCap; ++j) {
+    // inside points behind: counted while their key exceeds z's
+    const bool ok = top.slot[j] != kInvalid && top.lo[j] <= thr && cin_side + beh.count_above(top.hi[j]) < K;
+    cand |= ok ? (1u << j) : 0u;
+  }
+  while (cand) {
+    const int j = __ffs(cand) - 1;
+    cand &= cand - 1;
+    uint32_t slot = 0;  // selected by static index, the list stays in registers
+#pragma unroll
+    for (int qq = 0; qq < SweepListF::kCap; ++qq) slot = qq == j ? top.slot[qq] : slot;
+    const uint32_t zi = slot;
+    if (w.gid[zi] <= wid) continue;
+    const uint32_t at = atomicAdd(q->ntet, 1u);
+    if (at < kTetQ) {
+      q->tet[at] = vi | (wi << 10) | (zi << 20);
+      continue;
+    }
+    process_tet(P, w, u, M, vi, wi, zi, region_slot);
+  }
+}
+
 // Face (u, v, w): its triangle event, then the apexes of each side.  The
 // first pass certifies every side and insideness, builds the triangle's
-// interior and counts the inside points ahead of each side; each side then
-// takes one pass of its own.
+// interior and counts the inside points ahead of each side.  On the LDS path
+// (QUEUED) it also fills both sides' sweep lists, from which the apexes
+// follow directly; otherwise each side takes one pass of its own.
 template <bool QUEUED>
 __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_t M, uint32_t vi,
                              uint32_t wi, double rmax, uint32_t *region_slot, WaveQueues *q) {
@@ -551,10 +663,21 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
   // cell shared by v and w, so its radius is at most rmax; its centre is
   // c3 + t n with r3^2 + t^2 <= rmax^2.  Only well-conditioned faces use it.
   const Capsule cap = face_capsule(f, rmax);
+  // the triangle's level, before the pass (the vertices need not stay live)
+  const double tri_level = f.level(), tri_level_err = f.level_err();
   int cin0 = 0, cin1 = 0;  // inside the triangle's sphere, ahead of side 0 / 1
   int total_in = 0;
   uint32_t tri_int[kMaxInterior];
   uint32_t tests = 0, evals = 0;
+  SweepListF top0, top1;  // outside points ahead of side 0 / 1
+  BehindListF beh0, beh1;  // inside points behind side 0 / 1
+  if (QUEUED) {
+    top0.init();
+    top1.init();
+    beh0.init();
+    beh1.init();
+  }
+  constexpr uint32_t kFlopPoint = kFlopFacePoint + (QUEUED ? kFlopSweepKey : 0u);
   // points outside the capsule are outside the triangle's sphere and every
   // feasible one
 #pragma unroll 1
@@ -572,9 +695,25 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
       const int sp = cert_sign(Pw, PA), sh = cert_sign(H, HA);
       // every sign the side passes use is certified here, before anything is emitted
       if (sp == 0 || sh == 0) {
-        add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopFacePoint);
+        add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopPoint);
         emit_fallback(P, w, u, 3, ids, 3, M, region_slot);
         return;
+      }
+      if (QUEUED) {
+        double tau, terr;
+        sweep_key(Pw, PA, H, HA, tau, terr);
+        float hf, lf;
+        key_bounds_f(tau, terr, hf, lf);
+        if (sp > 0) {
+          // outside: ahead of side 0 when sh > 0, of side 1 otherwise
+          if (sh > 0) top0.insert(hf, lf, gi);
+          else top1.insert(hf, lf, gi);
+        } else {
+          // inside: behind side 1 when sh > 0, behind side 0 otherwise; -hf
+          // is a lower bound of -tau - terr
+          if (sh > 0) beh1.insert(-hf);
+          else beh0.insert(-hf);
+        }
       }
       if (sp > 0) continue;
       put_static(tri_int, total_in, w.gid[gi]);
@@ -583,37 +722,31 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
       cin1 += sh < 0;
       // both sides saturated and the triangle out of rank: nothing left here
       if (cin0 >= K && cin1 >= K && total_in + 3 > smax) {
-        add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopFacePoint);
+        add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopPoint);
         return;
       }
     }
   }
-  add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopFacePoint);
+  add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopPoint);
   // triangle event
   if (wc > 0 && total_in + 3 <= smax) {
     atomicAdd(w.cand, 1u);
     sort_static(tri_int, total_in);
-    emit_event(P, u, ids, 3, total_in, tri_int, f.level(), f.level_err());
+    emit_event(P, u, ids, 3, total_in, tri_int, tri_level, tri_level_err);
   } else if (wc == 0 && total_in + 3 <= smax) {
     emit_fallback(P, w, u, 3, ids, 3, M, region_slot);
     return;
   }
   if (K <= 0) return;
+  if (QUEUED) {
+    if (cin0 < K) fused_side(P, w, u, M, vi, wi, wid, 0, cin0, K, top0, beh0, rmax, region_slot, q);
+    if (cin1 < K) fused_side(P, w, u, M, vi, wi, wid, 1, cin1, K, top1, beh1, rmax, region_slot, q);
+    return;
+  }
   for (int side = 0; side < 2; ++side) {
     const int cin_side = side == 0 ? cin0 : cin1;
     if (cin_side >= K) continue;
-    if (QUEUED) {
-      // at most 2 items for each of the wave's 64 faces: the queue cannot fill
-      const uint32_t at = atomicAdd(q->nside, 1u);
-      if (at < kSideQ) {
-        q->side[at] = vi | (wi << 10) | ((uint32_t)side << 20) | ((uint32_t)cin_side << 21);
-        q->side_r[at] = (float)rmax;  // exact: rmax is a float value
-      } else {
-        atomicAdd(&P.counters[4], 1ull);  // reported by the host as an internal error
-      }
-    } else {
-      face_side_apexes(P, w, u, M, f, cap, vi, wi, wid, side, cin_side, K, region_slot, nullptr);
-    }
+    face_side_apexes(P, w, u, M, f, cap, vi, wi, wid, side, cin_side, K, region_slot, nullptr);
   }
 }
 
