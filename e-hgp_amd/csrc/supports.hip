@@ -831,7 +831,7 @@ constexpr int kCoverCap = 10;  // s_max - 1 <= 10
 // rmax, the largest radius over the shared cells, is rounded up to a float
 // value, so that the queued side passes can keep it in a float and rebuild
 // exactly the first pass's capsule (a larger bound only prunes less).
-__device__ inline bool face_feasible(const Work &w, uint32_t si, uint32_t sj, double &rmax) {
+__device__ inline bool face_feasible(const Work &w, uint32_t si, uint32_t sj, double &rmax, float *key = nullptr) {
   bool share = false;
   for (int k = 0; k < 6; ++k) share |= (w.rmask[6 * si + k] & w.rmask[6 * sj + k]) != 0;
   rmax = 0;
@@ -852,7 +852,14 @@ __device__ inline bool face_feasible(const Work &w, uint32_t si, uint32_t sj, do
   float rf = (float)rmax;
   if ((double)rf < rmax) rf = nextafterf(rf, INFINITY);
   rmax = (double)rf;
-  return !(D > 1e-6 * (aa * bb + fabs(ab) * fabs(ab)) && r2 > rmax * rmax * (1.0 + 1e-6));
+  const bool well = D > 1e-6 * (aa * bb + fabs(ab) * fabs(ab));
+  if (key) {
+    // the squared extent of the face's pencil capsule (an estimate, it only
+    // orders the faces into waves of similar work)
+    const double r3 = sqrt(r2), t = sqrt(fmax(rmax * rmax - r2, 0.0)) + fmax(rmax, r3);
+    *key = well ? (float)(t * t) : INFINITY;
+  }
+  return !(well && r2 > rmax * rmax * (1.0 + 1e-6));
 }
 
 // Pairs, faces and tetrahedra of anchor u over its compacted region (M
@@ -881,6 +888,7 @@ __device__ void anchor_faces(const Params &P, const Work &w, uint32_t u, uint32_
   __shared__ uint32_t q_side[kBlock / 64][kSideQ];
   __shared__ float q_side_r[kBlock / 64][kSideQ];
   __shared__ uint32_t q_face[kBlock / 64][128];
+  __shared__ float q_key[kBlock / 64][128];
   __shared__ uint32_t q_tet[kBlock / 64][kTetQ];
   __shared__ uint32_t q_n[kBlock / 64][2];
   const int wave = t >> 6, lane = t & 63, nwaves = (int)(blockDim.x >> 6);
@@ -909,31 +917,63 @@ __device__ void anchor_faces(const Params &P, const Work &w, uint32_t u, uint32_
       if (face_feasible(w, si, sj, rmax)) process_face<false>(P, w, u, M, si, sj, rmax, &sregion, q);
     }
   } else {
-    // Each wave first collects up to 64 feasible faces in its LDS list (the
+    // Each wave first collects up to 128 feasible faces in its LDS list (the
     // faces that fail the shared-cell or radius test cost only the test),
-    // then runs their first passes one per lane, then the queued sides and
-    // tetrahedra.  The lists keep region slots only; a lane recomputes the
+    // orders them by the extent of their pencil capsules, then runs the
+    // first passes of the 64 largest one per lane (lanes of similar work),
+    // then the queued sides and tetrahedra; the smaller ones wait for the
+    // next round.  The lists keep region slots only; a lane recomputes the
     // face's rmax (same arithmetic, same value).
     uint32_t nf = 0;  // faces in the wave's list (uniform)
     uint64_t cbase = (uint64_t)wave * 64;
     const uint64_t cstride = (uint64_t)nwaves * 64;
     for (;;) {
-      while (nf < 64 && cbase < F) {
+      while (nf <= 64 && cbase < F) {
         const uint64_t fidx = cbase + lane;
         bool ok = false;
         uint32_t si = 0, sj = 0;
+        float key = 0.0f;
         if (fidx < F) {
           face_of(fidx, si, sj);
           double rmax;
-          ok = face_feasible(w, si, sj, rmax);
+          ok = face_feasible(w, si, sj, rmax, &key);
         }
         const uint64_t m = __ballot(ok);
-        if (ok) q_face[wave][nf + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = si | (sj << 10);
+        if (ok) {
+          const uint32_t at = nf + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+          q_face[wave][at] = si | (sj << 10);
+          q_key[wave][at] = key;
+        }
         nf += (uint32_t)__popcll(m);
         cbase += cstride;
       }
       if (nf == 0) break;
       wave_sync();
+      if (nf > 64) {
+        // rank sort of the list by key (ties by position), ascending
+        uint32_t fv[2], rk[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const uint32_t i = (uint32_t)lane + 64u * h;
+          fv[h] = 0;
+          rk[h] = 0;
+          if (i < nf) {
+            const float ki = q_key[wave][i];
+            fv[h] = q_face[wave][i];
+            uint32_t r = 0;
+            for (uint32_t j = 0; j < nf; ++j) {
+              const float kj = q_key[wave][j];
+              r += kj < ki || (kj == ki && j < i);
+            }
+            rk[h] = r;
+          }
+        }
+        wave_sync();
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          if ((uint32_t)lane + 64u * h < nf) q_face[wave][rk[h]] = fv[h];
+        wave_sync();
+      }
       const uint32_t take = min(nf, 64u);
       nf -= take;
       if ((uint32_t)lane < take) {
