@@ -120,6 +120,7 @@ struct Work {
   float4 *fc;       // FP32 offset and distance per gathered point (certificate and masks)
   uint32_t *cand;   // candidate supports classified (LDS, flushed once per anchor)
   unsigned long long *flops;  // FP64 operations of the certified predicates (LDS, flushed once per anchor; null: not counted)
+  float *fx, *fy, *fz;        // FP32 copies of the region's coordinates (LDS path only)
 };
 
 // FP64 operations (add, subtract, multiply; a fused multiply-add counts two)
@@ -423,6 +424,62 @@ __device__ inline void key_bounds_f(double tau, double terr, float &hf, float &l
   if (!((double)lf <= ld)) lf = nextafterf(lf, -INFINITY);
 }
 
+// A face's power and height polynomials in FP32 (the FP64 coefficients
+// rounded to nearest).  With the coordinates, the coefficients and the at
+// most 8 roundings of a chain in FP32, |P32 - P| < 1e-6 Pabs and
+// |H32 - H| < 1e-6 Habs (about 5e-7 each); a sign is certified when it
+// exceeds 1e-5 of its magnitude bound and the bound exceeds 1e-30 (FP32
+// underflow then adds at most ~1e-37, far below the threshold).  Undecided
+// points take the FP64 path.
+struct TriSphereF {
+  float D, DA, g[3], gA[3], n[3], nA[3];
+  __device__ void init(const TriSphere &f) {
+    D = (float)f.D;
+    DA = (float)f.DA;
+    g[0] = (float)f.g.x; g[1] = (float)f.g.y; g[2] = (float)f.g.z;
+    gA[0] = (float)f.gA.x; gA[1] = (float)f.gA.y; gA[2] = (float)f.gA.z;
+    n[0] = (float)f.n.x; n[1] = (float)f.n.y; n[2] = (float)f.n.z;
+    nA[0] = (float)f.nA.x; nA[1] = (float)f.nA.y; nA[2] = (float)f.nA.z;
+  }
+  __device__ void eval(float x, float y, float z, float &P, float &PA, float &H, float &HA) const {
+    const float pp = x * x + y * y + z * z;
+    const float ax = fabsf(x), ay = fabsf(y), az = fabsf(z);
+    P = D * pp - (x * g[0] + y * g[1] + z * g[2]);
+    PA = DA * pp + (ax * gA[0] + ay * gA[1] + az * gA[2]);
+    H = x * n[0] + y * n[1] + z * n[2];
+    HA = ax * nA[0] + ay * nA[1] + az * nA[2];
+  }
+};
+__device__ inline int cert32(float v, float a) {
+  const float e = 1e-5f * a;
+  return !(a > 1e-30f) ? 0 : (v > e ? 1 : (v < -e ? -1 : 0));
+}
+// Key interval of a point whose FP32 signs are certified: the true P lies
+// within 1e-6 PA of P32 and |H| within 1e-6 HA of |H32| (< |H32| / 10), so
+// tau = P / |H| lies between the quotients of the extreme values; the
+// roundings of the sums, the reciprocals (1 ulp) and the products stay
+// inside the final 1e-6 widening.
+__device__ inline void key_bounds_32(float Pv, float PA, float H, float HA, float &hf, float &lf) {
+  const float ah = fabsf(H), eP = 1e-6f * PA, eH = 1e-6f * HA;
+  const float rlo = __builtin_amdgcn_rcpf(ah + eH), rhi = __builtin_amdgcn_rcpf(ah - eH);
+  float lo, hi;
+  if (Pv > 0.0f) {
+    lo = (Pv - eP) * rlo;
+    hi = (Pv + eP) * rhi;
+  } else {
+    lo = (Pv - eP) * rhi;
+    hi = (Pv + eP) * rlo;
+  }
+  lo = lo - 1e-6f * fabsf(lo) - 1e-37f;
+  hi = hi + 1e-6f * fabsf(hi) + 1e-37f;
+  if (!(fabsf(lo) < INFINITY && fabsf(hi) < INFINITY)) {
+    lo = -INFINITY;
+    hi = INFINITY;
+  }
+  hf = hi;
+  lf = lo;
+}
+
 // Apexes of one side of a face: the outside points ahead of the side whose
 // sweep key is among the s_max-3 smallest (fewer when inside points are
 // already ahead), each checked against the inside points behind the side.
@@ -457,6 +514,24 @@ __device__ inline uint64_t capsule_chunk(const Work &w, const Capsule &cap, uint
     in |= (uint64_t)keep << j;
   }
   tests += cn - (vi - s0 < cn ? 1u : 0u) - (wi - s0 < cn ? 1u : 0u);
+  return in;
+}
+
+// The same test on the FP32 coordinates: centre rounded to FP32 and the
+// squared bound widened by 1e-5 (rounded up), which covers the FP32
+// rounding of the coordinates, the centre and the distance (about 3e-6 of
+// the bound for a point near it).
+__device__ inline uint64_t capsule_chunk_f(const Work &w, float cx, float cy, float cz, float b2, uint32_t s0,
+                                           uint32_t M, uint32_t vi, uint32_t wi) {
+  const uint32_t cn = min(64u, M - s0);
+  uint64_t in = 0;
+#pragma unroll 4
+  for (uint32_t j = 0; j < cn; ++j) {
+    const uint32_t gi = s0 + j;
+    const float dx = w.fx[gi] - cx, dy = w.fy[gi] - cy, dz = w.fz[gi] - cz;
+    const bool keep = gi != vi && gi != wi && !(dx * dx + dy * dy + dz * dz > b2);
+    in |= (uint64_t)keep << j;
+  }
   return in;
 }
 
@@ -640,7 +715,7 @@ __device__ inline void fused_side(const Params &P, const Work &w, uint32_t u, ui
 // interior and counts the inside points ahead of each side.  On the LDS path
 // (QUEUED) it also fills both sides' sweep lists, from which the apexes
 // follow directly; otherwise each side takes one pass of its own.
-template <bool QUEUED>
+template <bool QUEUED, bool F32>
 __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_t M, uint32_t vi,
                              uint32_t wi, double rmax, uint32_t *region_slot, WaveQueues *q) {
   // the sphere is built from its points in PointId order, so its FP64 level
@@ -678,32 +753,68 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
     beh1.init();
   }
   constexpr uint32_t kFlopPoint = kFlopFacePoint + (QUEUED ? kFlopSweepKey : 0u);
+  // the FP32 path: capsule and face polynomials in FP32, FP64 only for the
+  // points whose FP32 signs are not certified (no FP64 capsule tests counted)
+  TriSphereF f32;
+  float ccx = 0.0f, ccy = 0.0f, ccz = 0.0f, cb2 = INFINITY;
+  if (F32) {
+    f32.init(f);
+    ccx = (float)cap.c3.x;
+    ccy = (float)cap.c3.y;
+    ccz = (float)cap.c3.z;
+    const double b2 = cap.bound2 * (1.0 + 1e-5);
+    cb2 = (float)b2;
+    if (!((double)cb2 >= b2)) cb2 = nextafterf(cb2, INFINITY);
+  }
   // points outside the capsule are outside the triangle's sphere and every
   // feasible one
 #pragma unroll 1
   for (uint32_t s0 = 0; s0 < M; s0 += 64) {
-    uint64_t in = capsule_chunk(w, cap, s0, M, vi, wi, tests);
+    uint64_t in = F32 ? capsule_chunk_f(w, ccx, ccy, ccz, cb2, s0, M, vi, wi)
+                      : capsule_chunk(w, cap, s0, M, vi, wi, tests);
 #pragma unroll 1
     while (in) {
       const uint32_t gi = s0 + (uint32_t)(__ffsll((long long)in) - 1);
       in &= in - 1;
-      const V3 p = gpt(w, gi);
-      evals++;
-      double Pw, PA, H, HA;
-      f.power(p, Pw, PA);
-      f.height(p, H, HA);
-      const int sp = cert_sign(Pw, PA), sh = cert_sign(H, HA);
-      // every sign the side passes use is certified here, before anything is emitted
+      int sp = 0, sh = 0;
+      float hf = 0.0f, lf = 0.0f;
+      if (F32) {
+        float P32, PA32, H32, HA32;
+        f32.eval(w.fx[gi], w.fy[gi], w.fz[gi], P32, PA32, H32, HA32);
+        sp = cert32(P32, PA32);
+        sh = cert32(H32, HA32);
+        if (QUEUED && sp != 0 && sh != 0) key_bounds_32(P32, PA32, H32, HA32, hf, lf);
+      }
       if (sp == 0 || sh == 0) {
-        add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopPoint);
-        emit_fallback(P, w, u, 3, ids, 3, M, region_slot);
-        return;
+        const V3 p = gpt(w, gi);
+        evals++;
+        double Pw, PA, H, HA;
+        if (F32) {
+          // rare: the FP64 sphere is rebuilt (same arithmetic, same values)
+          // rather than kept live through the pass
+          TriSphere f64;
+          f64.init(gpt(w, vi), gpt(w, wi));
+          f64.power(p, Pw, PA);
+          f64.height(p, H, HA);
+        } else {
+          f.power(p, Pw, PA);
+          f.height(p, H, HA);
+        }
+        sp = cert_sign(Pw, PA);
+        sh = cert_sign(H, HA);
+        // every sign the side passes use is certified here, before anything is emitted
+        if (sp == 0 || sh == 0) {
+          add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopPoint);
+          emit_fallback(P, w, u, 3, ids, 3, M, region_slot);
+          return;
+        }
+        if (QUEUED) {
+          double tau, terr;
+          sweep_key(Pw, PA, H, HA, tau, terr);
+          key_bounds_f(tau, terr, hf, lf);
+        }
       }
       if (QUEUED) {
-        double tau, terr;
-        sweep_key(Pw, PA, H, HA, tau, terr);
-        float hf, lf;
-        key_bounds_f(tau, terr, hf, lf);
         if (sp > 0) {
           // outside: ahead of side 0 when sh > 0, of side 1 otherwise
           if (sh > 0) top0.insert(hf, lf, gi);
@@ -866,7 +977,7 @@ __device__ inline bool face_feasible(const Work &w, uint32_t si, uint32_t sj, do
 // points at the front of the point arrays, masks by region slot, cell radii
 // in w.rho); *sK == 0 and *sregion == kInvalid on entry, visible to every
 // lane.  Marks the anchor done.
-template <bool BIG>
+template <bool BIG, bool F32>
 __device__ void anchor_faces(const Params &P, const Work &w, uint32_t u, uint32_t M, uint32_t *sK_,
                              uint32_t *sregion_) {
   uint32_t &sK = *sK_;
@@ -874,9 +985,16 @@ __device__ void anchor_faces(const Params &P, const Work &w, uint32_t u, uint32_
   const int t = threadIdx.x;
   // candidate vertices (ids above the anchor); their order only decides
   // which lane takes which face (a rank sort by direction cell, to give
-  // neighbouring lanes overlapping pencil capsules, measured slower)
-  for (uint32_t s = t; s < M; s += blockDim.x)
+  // neighbouring lanes overlapping pencil capsules, measured slower); the
+  // FP32 copies of the region's coordinates for the face passes
+  for (uint32_t s = t; s < M; s += blockDim.x) {
     if (w.gid[s] > u) w.kidx[atomicAdd(&sK, 1u)] = s;
+    if (F32) {
+      w.fx[s] = (float)w.gx[s];
+      w.fy[s] = (float)w.gy[s];
+      w.fz[s] = (float)w.gz[s];
+    }
+  }
   __syncthreads();
   const uint32_t Kc = sK;
   // 3. pairs (u, v), v > u
@@ -914,7 +1032,7 @@ __device__ void anchor_faces(const Params &P, const Work &w, uint32_t u, uint32_
       uint32_t si, sj;
       face_of(fidx, si, sj);
       double rmax;
-      if (face_feasible(w, si, sj, rmax)) process_face<false>(P, w, u, M, si, sj, rmax, &sregion, q);
+      if (face_feasible(w, si, sj, rmax)) process_face<false, false>(P, w, u, M, si, sj, rmax, &sregion, q);
     }
   } else {
     // Each wave first collects up to 128 feasible faces in its LDS list (the
@@ -981,7 +1099,7 @@ __device__ void anchor_faces(const Params &P, const Work &w, uint32_t u, uint32_
         const uint32_t si = fv & 1023u, sj = fv >> 10;
         double rmax;
         face_feasible(w, si, sj, rmax);
-        process_face<true>(P, w, u, M, si, sj, rmax, &sregion, q);
+        process_face<true, F32>(P, w, u, M, si, sj, rmax, &sregion, q);
       }
       wave_sync();
       const uint32_t ns = min(q_n[wave][0], kSideQ);
@@ -1005,7 +1123,7 @@ __device__ void anchor_faces(const Params &P, const Work &w, uint32_t u, uint32_
 
 // BIG: arrays in global slabs, regions beyond LDS; the face phase works in
 // place instead of through the wave queues.
-template <bool BIG>
+template <bool BIG, bool F32>
 __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   __shared__ uint32_t sM, sK, sflag, sregion;
   const int t = threadIdx.x;
@@ -1216,7 +1334,7 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     }
     return;
   }
-  anchor_faces<BIG>(P, w, u, M, &sK, &sregion);
+  anchor_faces<BIG, F32>(P, w, u, M, &sK, &sregion);
 }
 
 // The LDS path in two sizes: most anchors fit the small one, which leaves
@@ -1243,9 +1361,13 @@ __device__ void anchor_lds(const Params &P, uint32_t a) {
   __shared__ unsigned long long flops;
   if (threadIdx.x == 0) { cand = 0; flops = 0; }
   __syncthreads();
+  // FP32 region coordinates in the tails of the gather arrays, free once the
+  // region is compacted to the front
+  static_assert(2 * (GCAP - RCAP) >= RCAP, "FP32 region copies fit the gather tails");
   Work w{gx, gy, gz, gid, rho, ridx, rmask, kidx, reinterpret_cast<float4 *>(rmask), &cand,
-         P.count_flops ? &flops : nullptr};
-  anchor_work<false>(P, w, u);
+         P.count_flops ? &flops : nullptr, reinterpret_cast<float *>(gx + RCAP),
+         reinterpret_cast<float *>(gy + RCAP), reinterpret_cast<float *>(gz + RCAP)};
+  anchor_work<false, true>(P, w, u);
 }
 
 __global__ void __launch_bounds__(kBlock, 3) k_anchor_lds(Params P) {
@@ -1313,7 +1435,7 @@ __global__ void __launch_bounds__(kBlock) k_anchor_big(Params P) {
   __syncthreads();
   w.cand = &cand;
   w.flops = P.count_flops ? &flops : nullptr;
-  anchor_work<true>(P, w, u);
+  anchor_work<true, false>(P, w, u);
 }
 
 // Regions measured by the global-memory path's probe that fit LDS: the
@@ -1355,7 +1477,7 @@ __global__ void __launch_bounds__(kBlock, RCAP <= kRegionCapWide ? 2 : 1) k_anch
   if (t == 0) { cand = 0; sK = 0; sregion = kInvalid; flops = 0; }
   __syncthreads();
   Work w{gx, gy, gz, gid, rho, kidx, rmask, kidx, nullptr, &cand, P.count_flops ? &flops : nullptr};
-  anchor_faces<false>(P, w, u, M, &sK, &sregion);
+  anchor_faces<false, false>(P, w, u, M, &sK, &sregion);
 }
 
 // Ball count / emit around every anchor of the list.
