@@ -338,15 +338,33 @@ struct BehindList {
 // pass: key bounds rounded outwards (lo down, hi up), so that every
 // threshold test below stays conservative (wider intervals only admit more
 // apex candidates, which are classified exactly).
+//
+// An entry keeps its upper bound and one word with the region slot (low 10
+// bits) and the interval's width, rounded up to a 22-bit float (the FP32
+// width's bit pattern rounded up past its 9 lowest bits); its lower bound is
+// hi - width, rounded down.  Listed entries have finite upper bounds (an
+// infinite one never enters the list).  The upper bounds stay sorted with
+// one median-of-three per entry.
+__device__ inline uint32_t pack_entry(float h, float l, uint32_t slot) {
+  // the rounded difference widened past its rounding error
+  float gap = (h - l) * (1.0f + 2.4e-7f) + 1e-37f;
+  if (!(gap >= 0.0f)) gap = INFINITY;
+  return (((__float_as_uint(gap) + 511u) >> 9) << 10) | slot;
+}
+__device__ inline float entry_lo(float h, uint32_t e) {
+  const float d = h - __uint_as_float((e >> 10) << 9);
+  return d - fabsf(d) * 2.4e-7f - 1e-37f;
+}
+__device__ inline uint32_t entry_slot(uint32_t e) { return e & 1023u; }
+
 struct SweepListF {
   static constexpr int kCap = 8;  // >= s_max - 3 for s_max <= 11
   float hi[kCap];                 // upper key bounds, ascending
-  float lo[kCap];                 // lower key bounds
-  uint32_t slot[kCap];
+  uint32_t ent[kCap];             // slot and width (pack_entry); kInvalid: empty
   float evicted_lo;  // smallest lower key bound of a point not in the list
   __device__ void init() {
 #pragma unroll
-    for (int j = 0; j < kCap; ++j) { hi[j] = INFINITY; lo[j] = INFINITY; slot[j] = kInvalid; }
+    for (int j = 0; j < kCap; ++j) { hi[j] = INFINITY; ent[j] = kInvalid; }
     evicted_lo = INFINITY;
   }
   __device__ void insert(float h, float l, uint32_t s) {
@@ -354,16 +372,15 @@ struct SweepListF {
       evicted_lo = fminf(evicted_lo, l);
       return;
     }
-    evicted_lo = fminf(evicted_lo, lo[kCap - 1]);
+    if (ent[kCap - 1] != kInvalid) evicted_lo = fminf(evicted_lo, entry_lo(hi[kCap - 1], ent[kCap - 1]));
+    const uint32_t e = pack_entry(h, l, s);
 #pragma unroll
     for (int j = kCap - 1; j > 0; --j) {
-      const bool shift = h < hi[j - 1];
-      const bool place = !shift && h < hi[j];
-      hi[j] = shift ? hi[j - 1] : (place ? h : hi[j]);
-      lo[j] = shift ? lo[j - 1] : (place ? l : lo[j]);
-      slot[j] = shift ? slot[j - 1] : (place ? s : slot[j]);
+      ent[j] = h < hi[j - 1] ? ent[j - 1] : (h < hi[j] ? e : ent[j]);
+      hi[j] = __builtin_amdgcn_fmed3f(hi[j - 1], h, hi[j]);
     }
-    if (h < hi[0]) { hi[0] = h; lo[0] = l; slot[0] = s; }
+    ent[0] = h < hi[0] ? e : ent[0];
+    hi[0] = fminf(hi[0], h);
   }
   __device__ float threshold(int need) const {
     float r = INFINITY;
@@ -380,8 +397,9 @@ struct BehindListF {
     for (int j = 0; j < kCap; ++j) v[j] = -INFINITY;
   }
   __device__ void insert(float x) {
+    // descending: the median of the entry above, x and the entry
 #pragma unroll
-    for (int j = kCap - 1; j > 0; --j) v[j] = x > v[j - 1] ? v[j - 1] : (x > v[j] ? x : v[j]);
+    for (int j = kCap - 1; j > 0; --j) v[j] = __builtin_amdgcn_fmed3f(v[j - 1], x, v[j]);
     v[0] = fmaxf(v[0], x);
   }
   __device__ int count_above(float x) const {
@@ -690,16 +708,17 @@ __device__ inline void fused_side(const Params &P, const Work &w, uint32_t u, ui
 #pragma unroll
   for (int j = 0; j < SweepListF::kCap; ++j) {
     // inside points behind: counted while their key exceeds z's
-    const bool ok = top.slot[j] != kInvalid && top.lo[j] <= thr && cin_side + beh.count_above(top.hi[j]) < K;
+    const bool ok = top.ent[j] != kInvalid && entry_lo(top.hi[j], top.ent[j]) <= thr &&
+                    cin_side + beh.count_above(top.hi[j]) < K;
     cand |= ok ? (1u << j) : 0u;
   }
   while (cand) {
     const int j = __ffs(cand) - 1;
     cand &= cand - 1;
-    uint32_t slot = 0;  // selected by static index, the list stays in registers
+    uint32_t ent = 0;  // selected by static index, the list stays in registers
 #pragma unroll
-    for (int qq = 0; qq < SweepListF::kCap; ++qq) slot = qq == j ? top.slot[qq] : slot;
-    const uint32_t zi = slot;
+    for (int qq = 0; qq < SweepListF::kCap; ++qq) ent = qq == j ? top.ent[qq] : ent;
+    const uint32_t zi = entry_slot(ent);
     if (w.gid[zi] <= wid) continue;
     const uint32_t at = atomicAdd(q->ntet, 1u);
     if (at < kTetQ) {
