@@ -205,6 +205,45 @@ __device__ void emit_fallback(const Params &P, const Work &w, uint32_t u, uint32
 
 // Certified classification of the region against a candidate sphere.
 // Returns 1 event, 0 not critical, -1 undecided (host).
+// FP32 certification of a sign: v against its magnitude bound a.  With
+// the coordinates, the coefficients and the at most ~9 roundings of a chain
+// in FP32 (the FP64 coefficients rounded to nearest), |v32 - v| < 1e-6 a
+// for the polynomials below; a sign is certified when |v32| > 1e-5 a and
+// a > 1e-30 (FP32 underflow then adds at most ~1e-37, far below the
+// threshold).  0 leaves the point to the FP64 test.
+__device__ inline int cert32(float v, float a) {
+  const float e = 1e-5f * a;
+  return !(a > 1e-30f) ? 0 : (v > e ? 1 : (v < -e ? -1 : 0));
+}
+
+// Pair and tetrahedron side tests in FP32 on the region's FP32 coordinates.
+struct PairSphereF {
+  float a[3];
+  __device__ int side(float x, float y, float z) const {
+    const float v = x * (x - a[0]) + y * (y - a[1]) + z * (z - a[2]);
+    const float ax = fabsf(x), ay = fabsf(y), az = fabsf(z);
+    const float va = ax * (ax + fabsf(a[0])) + ay * (ay + fabsf(a[1])) + az * (az + fabsf(a[2]));
+    return cert32(v, va);
+  }
+};
+struct TetSphereF {
+  float V, VA, N[3], NA[3];
+  int sg;  // sign of the FP64 volume (certified nonzero)
+  __device__ int side(float x, float y, float z) const {
+    const float pp = x * x + y * y + z * z;
+    const float Q = V * pp - (x * N[0] + y * N[1] + z * N[2]);
+    const float QA = VA * pp + (fabsf(x) * NA[0] + fabsf(y) * NA[1] + fabsf(z) * NA[2]);
+    return sg * cert32(Q, QA);
+  }
+};
+__device__ inline PairSphereF to_f32(const PairSphere &s) {
+  return {{(float)s.a.x, (float)s.a.y, (float)s.a.z}};
+}
+__device__ inline TetSphereF to_f32(const TetSphere &t) {
+  return {(float)t.V, (float)t.VA, {(float)t.N.x, (float)t.N.y, (float)t.N.z},
+          {(float)t.NA.x, (float)t.NA.y, (float)t.NA.z}, t.V > 0 ? 1 : -1};
+}
+
 template <class S>
 __device__ int classify(const Params &P, const Work &w, uint32_t M, const S &sph,
                         const uint32_t *skip, int m, uint32_t (&interior)[kMaxInterior], int &nint) {
@@ -213,13 +252,19 @@ __device__ int classify(const Params &P, const Work &w, uint32_t M, const S &sph
   int undecided = 0;
   constexpr uint32_t per = sizeof(S) == sizeof(PairSphere) ? kFlopPairSide : kFlopTetSide;
   uint32_t evals = 0;
+  // FP32 first where the region has FP32 copies; FP64 for what it leaves open
+  const bool f32 = w.fx != nullptr;
+  const auto s32 = to_f32(sph);
   for (uint32_t s = 0; s < M; ++s) {
     const uint32_t gi = s;
     bool on_support = false;
     for (int k = 0; k < m - 1; ++k) on_support |= (gi == skip[k]);
     if (on_support) continue;
-    int side = sph.side(gpt(w, gi));
-    evals++;
+    int side = f32 ? s32.side(w.fx[gi], w.fy[gi], w.fz[gi]) : 0;
+    if (side == 0) {
+      side = sph.side(gpt(w, gi));
+      evals++;
+    }
     if (side < 0) {
       put_static(interior, nint, w.gid[gi]);
       nint++;
@@ -468,10 +513,6 @@ struct TriSphereF {
     HA = ax * nA[0] + ay * nA[1] + az * nA[2];
   }
 };
-__device__ inline int cert32(float v, float a) {
-  const float e = 1e-5f * a;
-  return !(a > 1e-30f) ? 0 : (v > e ? 1 : (v < -e ? -1 : 0));
-}
 // Key interval of a point whose FP32 signs are certified: the true P lies
 // within 1e-6 PA of P32 and |H| within 1e-6 HA of |H32| (< |H32| / 10), so
 // tau = P / |H| lies between the quotients of the extreme values; the
@@ -1438,7 +1479,7 @@ __global__ void __launch_bounds__(kBlock) k_anchor_big(Params P) {
   const uint32_t u = P.anchors[a];
   double *slab = P.scratch + (uint64_t)a * P.slab_doubles;
   const uint32_t G = P.gcap;
-  Work w;
+  Work w{};
   w.gx = slab;
   w.gy = slab + G;
   w.gz = slab + 2 * G;
