@@ -1166,12 +1166,17 @@ __device__ void anchor_faces(const Params &P, const Work &w, uint32_t u, uint32_
       for (uint32_t it = lane; it < ns; it += 64)
         side_item(P, w, u, M, q_side[wave][it], (double)q_side_r[wave][it], &sregion, q);
       wave_sync();
+      // whole rounds of 64 tetrahedra (the top of the queue); the rest waits
+      // for the next round's, so that the lanes stay busy
       const uint32_t nt = min(q_n[wave][1], kTetQ);
-      for (uint32_t it = lane; it < nt; it += 64) tet_item(P, w, u, M, q_tet[wave][it], &sregion);
+      const uint32_t full = nt & ~63u;
+      for (uint32_t it = nt - full + lane; it < nt; it += 64) tet_item(P, w, u, M, q_tet[wave][it], &sregion);
       wave_sync();
-      if (lane == 0) { q_n[wave][0] = 0; q_n[wave][1] = 0; }
+      if (lane == 0) { q_n[wave][0] = 0; q_n[wave][1] = nt - full; }
       wave_sync();
     }
+    const uint32_t nt = min(q_n[wave][1], kTetQ);
+    for (uint32_t it = lane; it < nt; it += 64) tet_item(P, w, u, M, q_tet[wave][it], &sregion);
   }
   __syncthreads();
   if (t == 0) {
