@@ -29,7 +29,7 @@ struct Context {
   int tower_lane_count = 2;
   hipStream_t stream = nullptr;
   bool poisoned = false;
-  bool count_flops = false;  // FP64 operation count of the anchor kernel (stats.anchor_flops)
+  bool count_flops = false;  // FP64 / FP32 operation counts of the anchor kernel (stats.anchor_flops, anchor_flops32)
   // largest candidate region an anchor's enumeration takes; larger ones stay
   // open and the catalogue is conditional (the enumeration budget)
   uint32_t region_limit = 2048;

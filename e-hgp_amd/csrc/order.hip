@@ -168,16 +168,34 @@ void exact_level_of(const ExactCloud &ex, const EventRec &e, BigInt &num, BigInt
 }
 
 // Exact canonical order of one run of host records [b, e) of the gathered
-// arrays: writes the permuted record indices and the class breaks.
+// arrays: writes the permuted record indices and the class breaks.  Two
+// records whose FP64 level intervals (level_lower .. level_upper) are
+// disjoint compare by them; the exact rational levels
+// are made only for the pairs whose intervals overlap (once per record).
 void exact_run(const ExactCloud &ex, const uint32_t *gidx, const EventRec *grec, int64_t b, int64_t e,
                uint32_t *new_idx, uint32_t *new_link) {
   const int64_t len = e - b;
   std::vector<BigInt> num(len), den(len);
-  for (int64_t k = 0; k < len; ++k) exact_level_of(ex, grec[b + k], num[k], den[k]);
+  std::vector<char> made(len, 0);
+  std::vector<double> lo(len), hi(len);
+  for (int64_t k = 0; k < len; ++k) {
+    lo[k] = level_lower(grec[b + k].level, grec[b + k].level_err);
+    hi[k] = level_upper(grec[b + k].level, grec[b + k].level_err);
+  }
+  auto level_cmp = [&](int64_t x, int64_t y) {
+    if (hi[x] < lo[y]) return -1;
+    if (hi[y] < lo[x]) return 1;
+    for (int64_t z : {x, y})
+      if (!made[z]) {
+        exact_level_of(ex, grec[b + z], num[z], den[z]);
+        made[z] = 1;
+      }
+    return compare_levels(num[x], den[x], num[y], den[y]);
+  };
   std::vector<int64_t> perm(len);
   std::iota(perm.begin(), perm.end(), 0);
   std::sort(perm.begin(), perm.end(), [&](int64_t x, int64_t y) {
-    int cmp = compare_levels(num[x], den[x], num[y], den[y]);
+    int cmp = level_cmp(x, y);
     if (cmp) return cmp < 0;
     const EventRec &rx = grec[b + x], &ry = grec[b + y];
     uint32_t kx = (rx.meta >> 8) & 0xff, ky = (ry.meta >> 8) & 0xff;
@@ -189,7 +207,7 @@ void exact_run(const ExactCloud &ex, const uint32_t *gidx, const EventRec *grec,
   for (int64_t k = 0; k < len; ++k) {
     new_idx[b + k] = gidx[b + perm[k]];
     // the link into the run was certain; inside it, a break at every exact increase
-    new_link[b + k] = k == 0 ? 1u : (compare_levels(num[perm[k - 1]], den[perm[k - 1]], num[perm[k]], den[perm[k]]) != 0);
+    new_link[b + k] = k == 0 ? 1u : (level_cmp(perm[k - 1], perm[k]) != 0);
   }
 }
 
@@ -260,7 +278,10 @@ void order_catalogue(Context &c, CatalogBuffers &B) {
     MHGP_HIP(hipMemcpyAsync(gidx.data(), dgidx.p, T * 4, hipMemcpyDeviceToHost, s));
     MHGP_HIP(hipMemcpyAsync(grec.data(), drec.p, T * sizeof(EventRec), hipMemcpyDeviceToHost, s));
     MHGP_HIP(hipStreamSynchronize(s));
+    if (tm.on) std::fprintf(stderr, "[mhgp] host runs: %lld records\n", (long long)T);
+    tm.mark("catalogue runs gathered");
     const ExactCloud &ex = exact_cloud(c);
+    tm.mark("catalogue exact cloud");
     const int threads = (int)std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(),
                                                                       (int64_t)nh}));
     std::vector<std::thread> pool;
@@ -277,6 +298,7 @@ void order_catalogue(Context &c, CatalogBuffers &B) {
     for (auto &th : pool) th.join();
     for (auto &e : errs)
       if (e) std::rethrow_exception(e);
+    tm.mark("catalogue runs ordered");
     MHGP_HIP(hipMemcpyAsync(dnew.p, new_idx.data(), T * 4, hipMemcpyHostToDevice, s));
     MHGP_HIP(hipMemcpyAsync(dlink.p, new_link.data(), T * 4, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_scatter_runs, dim3(blocks_for(T, 256)), dim3(256), 0, s, dpos.p, dnew.p, dlink.p, T,

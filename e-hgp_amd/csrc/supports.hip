@@ -106,7 +106,7 @@ struct Params {
   double *scratch;
   uint64_t slab_doubles;
   uint32_t gcap, rcap;
-  int count_flops;               // 1: count the FP64 operations of the predicates (counters[5])
+  int count_flops;               // 1: count the FP64 (counters[5]) and FP32 (counters[6]) operations
   uint32_t region_limit;         // largest region the global-memory path enumerates
 };
 
@@ -121,6 +121,7 @@ struct Work {
   uint32_t *cand;   // candidate supports classified (LDS, flushed once per anchor)
   unsigned long long *flops;  // FP64 operations of the certified predicates (LDS, flushed once per anchor; null: not counted)
   float *fx, *fy, *fz;        // FP32 copies of the region's coordinates (LDS path only)
+  unsigned long long *flops32;  // FP32 operations of the tests (as flops; null: not counted)
 };
 
 // FP64 operations (add, subtract, multiply; a fused multiply-add counts two)
@@ -138,6 +139,20 @@ constexpr uint32_t kFlopTetInit = 150;   // TetSphere::init
 // timed path pays for one uniform branch per call.
 __device__ inline void add_flops(const Work &w, uint64_t v) {
   if (w.flops && v) atomicAdd(w.flops, (unsigned long long)v);
+}
+// FP32 operations of the FP32 tests, counted the same way (the certificate
+// and mask tests of a cell, the capsule test, the face, pair and tetrahedron
+// polynomials with their bounds, a key interval); reciprocals and square
+// roots count one each.
+constexpr uint32_t kF32Cover = 16;      // one point against one cell (certificate)
+constexpr uint32_t kF32Mask = 16;       // one point against one cell (membership)
+constexpr uint32_t kF32Capsule = 8;     // capsule_chunk_f, per slot
+constexpr uint32_t kF32FacePoint = 31;  // TriSphereF::eval and two certified signs
+constexpr uint32_t kF32Key = 16;        // key_bounds_32
+constexpr uint32_t kF32PairSide = 17;   // PairSphereF::side
+constexpr uint32_t kF32TetSide = 20;    // TetSphereF::side
+__device__ inline void add_flops32(const Work &w, uint64_t v) {
+  if (w.flops32 && v) atomicAdd(w.flops32, (unsigned long long)v);
 }
 
 __device__ inline V3 gpt(const Work &w, uint32_t i) { return {w.gx[i], w.gy[i], w.gz[i]}; }
@@ -251,7 +266,8 @@ __device__ int classify(const Params &P, const Work &w, uint32_t M, const S &sph
   const int cap = P.s_max - m;
   int undecided = 0;
   constexpr uint32_t per = sizeof(S) == sizeof(PairSphere) ? kFlopPairSide : kFlopTetSide;
-  uint32_t evals = 0;
+  constexpr uint32_t per32 = sizeof(S) == sizeof(PairSphere) ? kF32PairSide : kF32TetSide;
+  uint32_t evals = 0, evals32 = 0;
   // FP32 first where the region has FP32 copies; FP64 for what it leaves open
   const bool f32 = w.fx != nullptr;
   const auto s32 = to_f32(sph);
@@ -261,6 +277,7 @@ __device__ int classify(const Params &P, const Work &w, uint32_t M, const S &sph
     for (int k = 0; k < m - 1; ++k) on_support |= (gi == skip[k]);
     if (on_support) continue;
     int side = f32 ? s32.side(w.fx[gi], w.fy[gi], w.fz[gi]) : 0;
+    evals32 += f32;
     if (side == 0) {
       side = sph.side(gpt(w, gi));
       evals++;
@@ -268,12 +285,17 @@ __device__ int classify(const Params &P, const Work &w, uint32_t M, const S &sph
     if (side < 0) {
       put_static(interior, nint, w.gid[gi]);
       nint++;
-      if (nint > cap) { add_flops(w, (uint64_t)evals * per); return 0; }
+      if (nint > cap) {
+        add_flops(w, (uint64_t)evals * per);
+        add_flops32(w, (uint64_t)evals32 * per32);
+        return 0;
+      }
     } else if (side == 0) {
       undecided++;
     }
   }
   add_flops(w, (uint64_t)evals * per);
+  add_flops32(w, (uint64_t)evals32 * per32);
   return undecided ? -1 : 1;
 }
 
@@ -581,7 +603,7 @@ __device__ inline uint64_t capsule_chunk(const Work &w, const Capsule &cap, uint
 // rounding of the coordinates, the centre and the distance (about 3e-6 of
 // the bound for a point near it).
 __device__ inline uint64_t capsule_chunk_f(const Work &w, float cx, float cy, float cz, float b2, uint32_t s0,
-                                           uint32_t M, uint32_t vi, uint32_t wi) {
+                                           uint32_t M, uint32_t vi, uint32_t wi, uint32_t &tests32) {
   const uint32_t cn = min(64u, M - s0);
   uint64_t in = 0;
 #pragma unroll 4
@@ -591,6 +613,7 @@ __device__ inline uint64_t capsule_chunk_f(const Work &w, float cx, float cy, fl
     const bool keep = gi != vi && gi != wi && !(dx * dx + dy * dy + dz * dz > b2);
     in |= (uint64_t)keep << j;
   }
+  tests32 += cn - (vi - s0 < cn ? 1u : 0u) - (wi - s0 < cn ? 1u : 0u);
   return in;
 }
 
@@ -813,6 +836,7 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
     beh1.init();
   }
   constexpr uint32_t kFlopPoint = kFlopFacePoint + (QUEUED ? kFlopSweepKey : 0u);
+  uint32_t tests32 = 0, evals32 = 0, keys32 = 0;  // FP32 work (F32 path)
   // the FP32 path: capsule and face polynomials in FP32, FP64 only for the
   // points whose FP32 signs are not certified (no FP64 capsule tests counted)
   TriSphereF f32;
@@ -830,7 +854,7 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
   // feasible one
 #pragma unroll 1
   for (uint32_t s0 = 0; s0 < M; s0 += 64) {
-    uint64_t in = F32 ? capsule_chunk_f(w, ccx, ccy, ccz, cb2, s0, M, vi, wi)
+    uint64_t in = F32 ? capsule_chunk_f(w, ccx, ccy, ccz, cb2, s0, M, vi, wi, tests32)
                       : capsule_chunk(w, cap, s0, M, vi, wi, tests);
 #pragma unroll 1
     while (in) {
@@ -843,7 +867,11 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
         f32.eval(w.fx[gi], w.fy[gi], w.fz[gi], P32, PA32, H32, HA32);
         sp = cert32(P32, PA32);
         sh = cert32(H32, HA32);
-        if (QUEUED && sp != 0 && sh != 0) key_bounds_32(P32, PA32, H32, HA32, hf, lf);
+        evals32++;
+        if (QUEUED && sp != 0 && sh != 0) {
+          key_bounds_32(P32, PA32, H32, HA32, hf, lf);
+          keys32++;
+        }
       }
       if (sp == 0 || sh == 0) {
         const V3 p = gpt(w, gi);
@@ -865,6 +893,7 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
         // every sign the side passes use is certified here, before anything is emitted
         if (sp == 0 || sh == 0) {
           add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopPoint);
+          add_flops32(w, (uint64_t)tests32 * kF32Capsule + (uint64_t)evals32 * kF32FacePoint + (uint64_t)keys32 * kF32Key);
           emit_fallback(P, w, u, 3, ids, 3, M, region_slot);
           return;
         }
@@ -894,11 +923,13 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
       // both sides saturated and the triangle out of rank: nothing left here
       if (cin0 >= K && cin1 >= K && total_in + 3 > smax) {
         add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopPoint);
+        add_flops32(w, (uint64_t)tests32 * kF32Capsule + (uint64_t)evals32 * kF32FacePoint + (uint64_t)keys32 * kF32Key);
         return;
       }
     }
   }
   add_flops(w, (uint64_t)tests * kFlopCapsule + (uint64_t)evals * kFlopPoint);
+  add_flops32(w, (uint64_t)tests32 * kF32Capsule + (uint64_t)evals32 * kF32FacePoint + (uint64_t)keys32 * kF32Key);
   // triangle event
   if (wc > 0 && total_in + 3 <= smax) {
     atomicAdd(w.cand, 1u);
@@ -971,6 +1002,7 @@ __device__ bool member_mask(const Params &P, const Work &w, uint32_t i, float rm
     return true;
   }
   if (q.w > rmax2) return false;
+  add_flops32(w, (uint64_t)kCells * kF32Mask);
   const float inv = __builtin_amdgcn_rcpf(q.w);
   bool any = false;
   // one mask word per 64 cells, each built in a register
@@ -1182,6 +1214,7 @@ __device__ void anchor_faces(const Params &P, const Work &w, uint32_t u, uint32_
   if (t == 0) {
     if (*w.cand) atomicAdd(&P.counters[3], (unsigned long long)*w.cand);
     if (w.flops && *w.flops) atomicAdd(&P.counters[5], *w.flops);
+    if (w.flops32 && *w.flops32) atomicAdd(&P.counters[6], *w.flops32);
     P.status[u] = ST_DONE;
   }
 }
@@ -1254,6 +1287,7 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   // factor below), so once l / 2 exceeds the list's last entry no later point
   // can enter it (and dropping points only enlarges the radius anyway)
   const bool by_distance = G <= 4 * blockDim.x;
+  uint32_t covers = 0;  // point-cell tests (FP32 count)
   for (int c = t; c < kCells; c += blockDim.x) {
     const double *cd = P.cells + 8 * c;
     const CellF cf = cell_f(P.cellf + 8 * c);
@@ -1265,6 +1299,7 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
       const float4 q = w.fc[i];
       if (!(q.w > kTinyLen)) continue;  // skipping a point only enlarges the radius
       if (by_distance && q.w * 0.49999f > best[kCoverCap - 1]) break;
+      covers++;
       const float ct = (q.x * cf.dx + q.y * cf.dy + q.z * cf.dz) * __builtin_amdgcn_rcpf(q.w);
       if (ct <= 0.0f) continue;
       const float st = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
@@ -1295,6 +1330,7 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     if (!((double)rf >= rd)) rf = rd <= (double)FLT_MIN ? FLT_MIN : nextafterf(rf, INFINITY);
     w.rho[c] = rf;
   }
+  add_flops32(w, (uint64_t)covers * kF32Cover);
   __syncthreads();
   __shared__ float s_rmax2;
   if (t == 0) {
@@ -1423,15 +1459,16 @@ __device__ void anchor_lds(const Params &P, uint32_t a) {
     return;
   }
   __shared__ uint32_t cand;
-  __shared__ unsigned long long flops;
-  if (threadIdx.x == 0) { cand = 0; flops = 0; }
+  __shared__ unsigned long long flops, flops32;
+  if (threadIdx.x == 0) { cand = 0; flops = 0; flops32 = 0; }
   __syncthreads();
   // FP32 region coordinates in the tails of the gather arrays, free once the
   // region is compacted to the front
   static_assert(2 * (GCAP - RCAP) >= RCAP, "FP32 region copies fit the gather tails");
   Work w{gx, gy, gz, gid, rho, ridx, rmask, kidx, reinterpret_cast<float4 *>(rmask), &cand,
          P.count_flops ? &flops : nullptr, reinterpret_cast<float *>(gx + RCAP),
-         reinterpret_cast<float *>(gy + RCAP), reinterpret_cast<float *>(gz + RCAP)};
+         reinterpret_cast<float *>(gy + RCAP), reinterpret_cast<float *>(gz + RCAP),
+         P.count_flops ? &flops32 : nullptr};
   anchor_work<false, true>(P, w, u);
 }
 
@@ -2225,6 +2262,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
       c.stats.anchor_bytes += (int64_t)tot * 28 + (int64_t)ev * (int64_t)sizeof(EventRec);
       // FP64 operations of the certified predicates the launch evaluated
       c.stats.anchor_flops += (int64_t)(ctr_after[5] - ctr_before[5]);
+      c.stats.anchor_flops32 += (int64_t)(ctr_after[6] - ctr_before[6]);
     }
     }
     tm.mark("catalogue anchors");

@@ -47,7 +47,7 @@ class Stats(ctypes.Structure):
                 ("anchor_bytes", ctypes.c_int64), ("arm_ms", ctypes.c_double),
                 ("arm_launches", ctypes.c_int64), ("arm_bytes", ctypes.c_int64),
                 ("gathered_points", ctypes.c_int64), ("events", ctypes.c_int64),
-                ("anchor_flops", ctypes.c_int64)]
+                ("anchor_flops", ctypes.c_int64), ("anchor_flops32", ctypes.c_int64)]
 
 
 class RequestResult(ctypes.Structure):
@@ -198,7 +198,7 @@ class Context:
         return st
 
     def count_flops(self, on: bool) -> None:
-        """FP64 operation count of the anchor kernel (Stats.anchor_flops)."""
+        """FP64 and FP32 operation counts of the anchor kernel (Stats.anchor_flops, anchor_flops32)."""
         _check(lib().mhgp_ctx_count_flops(self._h, 1 if on else 0))
 
     def set_region_limit(self, limit: int) -> None:

@@ -275,11 +275,15 @@ typedef struct {
   int64_t anchor_flops;      /* FP64 operations of the certified predicates the anchor kernel evaluated
                                 (per evaluation: the value, its magnitude bound, the certified sign);
                                 counted only while mhgp_ctx_count_flops is on */
+  int64_t anchor_flops32;    /* FP32 operations of its FP32 tests, counted the same way (certificate and
+                                membership tests, capsule tests, the FP32 predicates and key intervals);
+                                over the same launches, while mhgp_ctx_count_flops is on */
 } mhgp_stats;
 
 int mhgp_ctx_stats(mhgp_ctx *ctx, mhgp_stats *stats, int reset);
-/* Turns the FP64 operation count of the anchor kernel on (1) or off (0, the
- * default: the kernel then skips the per-call LDS atomics of the count). */
+/* Turns the FP64 and FP32 operation counts of the anchor kernel on (1) or off
+ * (0, the default: the kernel then skips the per-call LDS atomics of the
+ * counts). */
 int mhgp_ctx_count_flops(mhgp_ctx *ctx, int on);
 
 /* Enumeration budget of the catalogue: the largest candidate region (points)
