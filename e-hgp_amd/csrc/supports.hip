@@ -825,7 +825,9 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
   const double tri_level = f.level(), tri_level_err = f.level_err();
   int cin0 = 0, cin1 = 0;  // inside the triangle's sphere, ahead of side 0 / 1
   int total_in = 0;
-  uint32_t tri_int[kMaxInterior];
+  // region slots of the first 9 inside points, 10 bits each, 3 per word (a
+  // triangle event has at most s_max - 3 <= 8 interior points)
+  uint32_t tri_pk[3] = {0u, 0u, 0u};
   uint32_t tests = 0, evals = 0;
   SweepListF top0, top1;  // outside points ahead of side 0 / 1
   BehindListF beh0, beh1;  // inside points behind side 0 / 1
@@ -916,7 +918,11 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
         }
       }
       if (sp > 0) continue;
-      put_static(tri_int, total_in, w.gid[gi]);
+      if (total_in < 9) {
+        const uint32_t word = (uint32_t)total_in / 3u, shift = ((uint32_t)total_in % 3u) * 10u;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) tri_pk[k] |= (uint32_t)k == word ? gi << shift : 0u;
+      }
       total_in++;
       cin0 += sh > 0;
       cin1 += sh < 0;
@@ -933,6 +939,10 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
   // triangle event
   if (wc > 0 && total_in + 3 <= smax) {
     atomicAdd(w.cand, 1u);
+    uint32_t tri_int[kMaxInterior];
+#pragma unroll
+    for (int j = 0; j < kMaxInterior; ++j)
+      tri_int[j] = j < 9 && j < total_in ? w.gid[(tri_pk[j / 3] >> ((j % 3) * 10)) & 1023u] : kInvalid;
     sort_static(tri_int, total_in);
     emit_event(P, u, ids, 3, total_in, tri_int, tri_level, tri_level_err);
   } else if (wc == 0 && total_in + 3 <= smax) {
