@@ -35,6 +35,7 @@
 #include <cmath>
 #include <cfloat>
 #include <cstdio>
+#include <cstring>
 #include <condition_variable>
 #include <map>
 #include <memory>
@@ -825,9 +826,9 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
   const double tri_level = f.level(), tri_level_err = f.level_err();
   int cin0 = 0, cin1 = 0;  // inside the triangle's sphere, ahead of side 0 / 1
   int total_in = 0;
-  // region slots of the first 9 inside points, 10 bits each, 3 per word (a
+  // region slots of the first 8 inside points, 16 bits each, 2 per word (a
   // triangle event has at most s_max - 3 <= 8 interior points)
-  uint32_t tri_pk[3] = {0u, 0u, 0u};
+  uint32_t tri_pk[4] = {0u, 0u, 0u, 0u};
   uint32_t tests = 0, evals = 0;
   SweepListF top0, top1;  // outside points ahead of side 0 / 1
   BehindListF beh0, beh1;  // inside points behind side 0 / 1
@@ -918,10 +919,10 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
         }
       }
       if (sp > 0) continue;
-      if (total_in < 9) {
-        const uint32_t word = (uint32_t)total_in / 3u, shift = ((uint32_t)total_in % 3u) * 10u;
+      if (total_in < 8) {
+        const uint32_t word = (uint32_t)total_in >> 1, shift = ((uint32_t)total_in & 1u) * 16u;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) tri_pk[k] |= (uint32_t)k == word ? gi << shift : 0u;
+        for (int k = 0; k < 4; ++k) tri_pk[k] |= (uint32_t)k == word ? gi << shift : 0u;
       }
       total_in++;
       cin0 += sh > 0;
@@ -942,7 +943,7 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
     uint32_t tri_int[kMaxInterior];
 #pragma unroll
     for (int j = 0; j < kMaxInterior; ++j)
-      tri_int[j] = j < 9 && j < total_in ? w.gid[(tri_pk[j / 3] >> ((j % 3) * 10)) & 1023u] : kInvalid;
+      tri_int[j] = j < 8 && j < total_in ? w.gid[(tri_pk[j / 2] >> ((j % 2) * 16)) & 0xFFFFu] : kInvalid;
     sort_static(tri_int, total_in);
     emit_event(P, u, ids, 3, total_in, tri_int, tri_level, tri_level_err);
   } else if (wc == 0 && total_in + 3 <= smax) {
@@ -2164,6 +2165,15 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     B.cellf.ensure(8 * kCells);
     copy_wait(B.cellf.p, cf.data(), cf.size() * 4, hipMemcpyHostToDevice, s);
   }
+  // Test hook (MHGP_TEST_ANCHOR_PATH): "region" sends every anchor through
+  // the big path's probe and LDS region kernels, "slabs" through the
+  // global-memory kernel, so that small clouds check those paths against the
+  // exhaustive oracle.  Unset, the anchors take their normal tiers.
+  int force_path = 0;
+  if (const char *tp = std::getenv("MHGP_TEST_ANCHOR_PATH")) {
+    if (std::strcmp(tp, "region") == 0) force_path = 1;
+    else if (std::strcmp(tp, "slabs") == 0) force_path = 2;
+  }
   // initial gather radius: twice the distance to the min(32, n-1)-th neighbour
   const int kq = (int)std::min<int64_t>(32, n - 1);
   StageTimer tm(s);
@@ -2254,7 +2264,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     if (c.batch && attempt == 0 && iter == 0) {
       // first pass of a batched cloud: one persistent launch over the batch
       c.batch->arrive(c.batch_slot, c, P, cnt);
-    } else {
+    } else if (!force_path) {
     c.anchor_clock.start(s);
     hipLaunchKernelGGL(k_anchor_lds, dim3(cnt), dim3(kBlock), 0, s, P);
     c.anchor_clock.stop(s);
@@ -2284,6 +2294,8 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
     MHGP_HIP(hipMemcpyAsync(rad.data(), B.radius.p, n * 8, hipMemcpyDeviceToHost, s));
     MHGP_HIP(hipMemcpyAsync(gc.data(), B.gcount.p, n * 4, hipMemcpyDeviceToHost, s));
     MHGP_HIP(hipStreamSynchronize(s));
+    if (force_path)
+      for (uint32_t u : pending) st[u] = ST_BIG;
     std::vector<uint32_t> next, wide_list;
     for (int tier = 0; tier < 2; ++tier) {
       std::vector<uint32_t> wide;
@@ -2293,7 +2305,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
           next.push_back(u);
           widened++;
         } else if (st[u] == ST_BIG) {
-          if (tier == 0 && gc[u] <= (uint32_t)kGatherCapWide) wide.push_back(u);
+          if (tier == 0 && !force_path && gc[u] <= (uint32_t)kGatherCapWide) wide.push_back(u);
           else big.push_back(u);
         }
       }
@@ -2364,7 +2376,7 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
             Q.scratch = B.scratch.p; Q.slab_doubles = slab_of(g); Q.gcap = g; Q.rcap = g;
             hipLaunchKernelGGL(k_anchor_big, dim3(bn), dim3(kBlock), 0, s, Q);
             // probed regions that fit LDS: enumerated there, from this chunk's slabs
-            if (probe) {
+            if (probe && force_path != 2) {
               hipLaunchKernelGGL((k_anchor_region<kRegionCapWide, 0>), dim3(bn), dim3(kBlock), 0, s, Q);
               hipLaunchKernelGGL((k_anchor_region<kRegionCapHuge, kRegionCapWide>), dim3(bn), dim3(kBlock), 0, s, Q);
             }

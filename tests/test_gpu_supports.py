@@ -116,6 +116,19 @@ def test_catalog_snapped_mixture_small(seed, n, bits):
         assert cat.info.diagnostic_count > 0 or cat.info.fallback_count > 0
 
 
+@pytest.mark.parametrize("path", ["region", "slabs"])
+def test_catalog_big_paths_small(path, monkeypatch):
+    """The big-anchor paths (probe + LDS region kernels; global-memory slabs)
+    against the exhaustive oracle: the test hook MHGP_TEST_ANCHOR_PATH sends
+    every anchor of these small clouds through them."""
+    monkeypatch.setenv("MHGP_TEST_ANCHOR_PATH", path)
+    _compare(workloads.uniform_cloud(60, seed=31), 10, f"uniform60-{path}")
+    _compare(workloads.jittered_lattice(64, side=5, seed=32), 10, f"lattice64-{path}")
+    pts = workloads.gaussian_mixture(100, blobs=3, noise_fraction=0.15, box=2.0, snap_bits=3, seed=33)
+    _compare(pts, 10, f"snapped100-{path}")
+    _compare(workloads.uniform_cloud(40, seed=34), 4, f"uniform40-k4-{path}")
+
+
 def test_big_regions_give_a_conditional_catalogue():
     """Anchors whose candidate region exceeds the enumeration bound are left
     open: the catalogue reports status conditional with their count and
