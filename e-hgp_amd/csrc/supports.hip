@@ -407,9 +407,10 @@ struct BehindList {
 // threshold test below stays conservative (wider intervals only admit more
 // apex candidates, which are classified exactly).
 //
-// An entry keeps its upper bound and one word with the region slot (low 10
-// bits) and the interval's width, rounded up to a 22-bit float (the FP32
-// width's bit pattern rounded up past its 9 lowest bits); its lower bound is
+// An entry keeps its upper bound and one word with the region slot (low 11
+// bits, regions of up to 2048 points) and the interval's width, rounded up
+// to a 21-bit float (the FP32 width's bit pattern rounded up past its 10
+// lowest bits); its lower bound is
 // hi - width, rounded down.  Listed entries have finite upper bounds (an
 // infinite one never enters the list).  The upper bounds stay sorted with
 // one median-of-three per entry.
@@ -417,13 +418,13 @@ __device__ inline uint32_t pack_entry(float h, float l, uint32_t slot) {
   // the rounded difference widened past its rounding error
   float gap = (h - l) * (1.0f + 2.4e-7f) + 1e-37f;
   if (!(gap >= 0.0f)) gap = INFINITY;
-  return (((__float_as_uint(gap) + 511u) >> 9) << 10) | slot;
+  return (((__float_as_uint(gap) + 1023u) >> 10) << 11) | slot;
 }
 __device__ inline float entry_lo(float h, uint32_t e) {
-  const float d = h - __uint_as_float((e >> 10) << 9);
+  const float d = h - __uint_as_float((e >> 11) << 10);
   return d - fabsf(d) * 2.4e-7f - 1e-37f;
 }
-__device__ inline uint32_t entry_slot(uint32_t e) { return e & 1023u; }
+__device__ inline uint32_t entry_slot(uint32_t e) { return e & 2047u; }
 
 struct SweepListF {
   static constexpr int kCap = 8;  // >= s_max - 3 for s_max <= 11
@@ -759,6 +760,15 @@ __device__ inline void fused_side(const Params &P, const Work &w, uint32_t u, ui
   const int need = K - cin_side;
   const float thr = top.threshold(need);
   if (top.evicted_lo != INFINITY && top.evicted_lo <= thr) {
+    if (!q) {
+      // in place (global-memory path): the face's sphere rebuilt (same
+      // arithmetic, same values), then the side's own pass
+      TriSphere f;
+      f.init(gpt(w, vi), gpt(w, wi));
+      const Capsule cap = face_capsule(f, rmax);
+      face_side_apexes(P, w, u, M, f, cap, vi, wi, wid, side, cin_side, K, region_slot, nullptr);
+      return;
+    }
     // at most 2 items for each of the wave's 64 faces: the queue cannot fill
     const uint32_t at = atomicAdd(q->nside, 1u);
     if (at < kSideQ) {
@@ -785,20 +795,23 @@ __device__ inline void fused_side(const Params &P, const Work &w, uint32_t u, ui
     for (int qq = 0; qq < SweepListF::kCap; ++qq) ent = qq == j ? top.ent[qq] : ent;
     const uint32_t zi = entry_slot(ent);
     if (w.gid[zi] <= wid) continue;
-    const uint32_t at = atomicAdd(q->ntet, 1u);
-    if (at < kTetQ) {
-      q->tet[at] = vi | (wi << 10) | (zi << 20);
-      continue;
+    if (q) {
+      const uint32_t at = atomicAdd(q->ntet, 1u);
+      if (at < kTetQ) {
+        q->tet[at] = vi | (wi << 10) | (zi << 20);
+        continue;
+      }
     }
     process_tet(P, w, u, M, vi, wi, zi, region_slot);
   }
 }
 
-// Face (u, v, w): its triangle event, then the apexes of each side.  The
-// first pass certifies every side and insideness, builds the triangle's
-// interior and counts the inside points ahead of each side.  On the LDS path
-// (QUEUED) it also fills both sides' sweep lists, from which the apexes
-// follow directly; otherwise each side takes one pass of its own.
+// Face (u, v, w): its triangle event, then the apexes of each side.  One
+// pass certifies every side and insideness, builds the triangle's interior,
+// counts the inside points ahead of each side and fills both sides' sweep
+// lists, from which the apexes follow (fused_side).  QUEUED (LDS paths):
+// tetrahedra and rescans go to the wave's queues; otherwise in place.  F32:
+// the pass runs on the region's FP32 copies.
 template <bool QUEUED, bool F32>
 __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_t M, uint32_t vi,
                              uint32_t wi, double rmax, uint32_t *region_slot, WaveQueues *q) {
@@ -832,13 +845,11 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
   uint32_t tests = 0, evals = 0;
   SweepListF top0, top1;  // outside points ahead of side 0 / 1
   BehindListF beh0, beh1;  // inside points behind side 0 / 1
-  if (QUEUED) {
-    top0.init();
-    top1.init();
-    beh0.init();
-    beh1.init();
-  }
-  constexpr uint32_t kFlopPoint = kFlopFacePoint + (QUEUED ? kFlopSweepKey : 0u);
+  top0.init();
+  top1.init();
+  beh0.init();
+  beh1.init();
+  constexpr uint32_t kFlopPoint = kFlopFacePoint + kFlopSweepKey;
   uint32_t tests32 = 0, evals32 = 0, keys32 = 0;  // FP32 work (F32 path)
   // the FP32 path: capsule and face polynomials in FP32, FP64 only for the
   // points whose FP32 signs are not certified (no FP64 capsule tests counted)
@@ -871,7 +882,7 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
         sp = cert32(P32, PA32);
         sh = cert32(H32, HA32);
         evals32++;
-        if (QUEUED && sp != 0 && sh != 0) {
+        if (sp != 0 && sh != 0) {
           key_bounds_32(P32, PA32, H32, HA32, hf, lf);
           keys32++;
         }
@@ -900,23 +911,19 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
           emit_fallback(P, w, u, 3, ids, 3, M, region_slot);
           return;
         }
-        if (QUEUED) {
-          double tau, terr;
-          sweep_key(Pw, PA, H, HA, tau, terr);
-          key_bounds_f(tau, terr, hf, lf);
-        }
+        double tau, terr;
+        sweep_key(Pw, PA, H, HA, tau, terr);
+        key_bounds_f(tau, terr, hf, lf);
       }
-      if (QUEUED) {
-        if (sp > 0) {
-          // outside: ahead of side 0 when sh > 0, of side 1 otherwise
-          if (sh > 0) top0.insert(hf, lf, gi);
-          else top1.insert(hf, lf, gi);
-        } else {
-          // inside: behind side 1 when sh > 0, behind side 0 otherwise; -hf
-          // is a lower bound of -tau - terr
-          if (sh > 0) beh1.insert(-hf);
-          else beh0.insert(-hf);
-        }
+      if (sp > 0) {
+        // outside: ahead of side 0 when sh > 0, of side 1 otherwise
+        if (sh > 0) top0.insert(hf, lf, gi);
+        else top1.insert(hf, lf, gi);
+      } else {
+        // inside: behind side 1 when sh > 0, behind side 0 otherwise; -hf
+        // is a lower bound of -tau - terr
+        if (sh > 0) beh1.insert(-hf);
+        else beh0.insert(-hf);
       }
       if (sp > 0) continue;
       if (total_in < 8) {
@@ -951,16 +958,10 @@ __device__ void process_face(const Params &P, const Work &w, uint32_t u, uint32_
     return;
   }
   if (K <= 0) return;
-  if (QUEUED) {
-    if (cin0 < K) fused_side(P, w, u, M, vi, wi, wid, 0, cin0, K, top0, beh0, rmax, region_slot, q);
-    if (cin1 < K) fused_side(P, w, u, M, vi, wi, wid, 1, cin1, K, top1, beh1, rmax, region_slot, q);
-    return;
-  }
-  for (int side = 0; side < 2; ++side) {
-    const int cin_side = side == 0 ? cin0 : cin1;
-    if (cin_side >= K) continue;
-    face_side_apexes(P, w, u, M, f, cap, vi, wi, wid, side, cin_side, K, region_slot, nullptr);
-  }
+  // queued on the LDS paths, in place on the global-memory path
+  WaveQueues *qq = QUEUED ? q : nullptr;
+  if (cin0 < K) fused_side(P, w, u, M, vi, wi, wid, 0, cin0, K, top0, beh0, rmax, region_slot, qq);
+  if (cin1 < K) fused_side(P, w, u, M, vi, wi, wid, 1, cin1, K, top1, beh1, rmax, region_slot, qq);
 }
 
 // A queued side of a face: the face's sphere rebuilt (same arithmetic, same
@@ -1542,13 +1543,18 @@ __global__ void __launch_bounds__(kBlock) k_anchor_big(Params P) {
   w.ridx = w.gid + G;
   w.kidx = w.ridx + G;
   w.fc = reinterpret_cast<float4 *>(slab + big_fc_offset(G));
+  // the FP32 region copies take the FP32 offsets' area once the region is
+  // compacted (4G floats hold 3G)
+  w.fx = reinterpret_cast<float *>(w.fc);
+  w.fy = w.fx + G;
+  w.fz = w.fy + G;
   __shared__ uint32_t cand;
   __shared__ unsigned long long flops;
   if (threadIdx.x == 0) { cand = 0; flops = 0; }
   __syncthreads();
   w.cand = &cand;
   w.flops = P.count_flops ? &flops : nullptr;
-  anchor_work<true, false>(P, w, u);
+  anchor_work<true, true>(P, w, u);
 }
 
 // Regions measured by the global-memory path's probe that fit LDS: the
@@ -1576,6 +1582,7 @@ __global__ void __launch_bounds__(kBlock, RCAP <= kRegionCapWide ? 2 : 1) k_anch
   __shared__ float rho[kCells];
   __shared__ uint64_t rmask[6 * RCAP];
   __shared__ uint32_t kidx[RCAP];
+  __shared__ float fx[RCAP], fy[RCAP], fz[RCAP];
   __shared__ uint32_t cand, sK, sregion;
   __shared__ unsigned long long flops;
   const int t = threadIdx.x;
@@ -1589,8 +1596,8 @@ __global__ void __launch_bounds__(kBlock, RCAP <= kRegionCapWide ? 2 : 1) k_anch
   for (int c = t; c < kCells; c += blockDim.x) rho[c] = reinterpret_cast<const float *>(slab + 3 * G)[c];
   if (t == 0) { cand = 0; sK = 0; sregion = kInvalid; flops = 0; }
   __syncthreads();
-  Work w{gx, gy, gz, gid, rho, kidx, rmask, kidx, nullptr, &cand, P.count_flops ? &flops : nullptr};
-  anchor_faces<false, false>(P, w, u, M, &sK, &sregion);
+  Work w{gx, gy, gz, gid, rho, kidx, rmask, kidx, nullptr, &cand, P.count_flops ? &flops : nullptr, fx, fy, fz};
+  anchor_faces<false, true>(P, w, u, M, &sK, &sregion);
 }
 
 // Ball count / emit around every anchor of the list.
