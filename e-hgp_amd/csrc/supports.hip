@@ -1005,8 +1005,9 @@ __device__ inline CellF cell_f(const float *cf) {
 // FP32 with relative and absolute slack; a point may only be included too
 // often, which costs work but never a support.
 // rmax2 bounds 2 rho'_C over every cell from above: a point beyond it is in
-// no cell's balls.
-__device__ bool member_mask(const Params &P, const Work &w, uint32_t i, float rmax2, uint64_t mask[6]) {
+// no cell's balls; wlim[wd] bounds it over the 64 cells of mask word wd.
+__device__ bool member_mask(const Params &P, const Work &w, uint32_t i, float rmax2, const float *wlim,
+                            uint64_t mask[6]) {
   const float4 q = w.fc[i];
   for (int k = 0; k < 6; ++k) mask[k] = 0;
   if (!(q.w > kTinyLen)) {  // distance lost to underflow: member everywhere
@@ -1014,12 +1015,14 @@ __device__ bool member_mask(const Params &P, const Work &w, uint32_t i, float rm
     return true;
   }
   if (q.w > rmax2) return false;
-  add_flops32(w, (uint64_t)kCells * kF32Mask);
   const float inv = __builtin_amdgcn_rcpf(q.w);
   bool any = false;
-  // one mask word per 64 cells, each built in a register
+  // one mask word per 64 cells, each built in a register; a word whose
+  // largest radius the point exceeds stays empty
 #pragma unroll
   for (int wd = 0; wd < kCells / 64; ++wd) {
+    if (q.w > wlim[wd]) continue;
+    add_flops32(w, 64ull * kF32Mask);
     uint64_t bits = 0;
 #pragma unroll 1
     for (int b = 0; b < 64; ++b) {
@@ -1344,12 +1347,17 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
   }
   add_flops32(w, (uint64_t)covers * kF32Cover);
   __syncthreads();
-  __shared__ float s_rmax2;
+  __shared__ float s_rmax2, s_wlim[kCells / 64];
   if (t == 0) {
-    double m = 0;
-    for (int c = 0; c < kCells; ++c) m = fmax(m, w.rho[c]);
+    double m = 0, wm[kCells / 64] = {};
+    for (int c = 0; c < kCells; ++c) {
+      m = fmax(m, w.rho[c]);
+      wm[c >> 6] = fmax(wm[c >> 6], w.rho[c]);
+    }
     // 2 max rho in FP32, rounded up past the slack of the membership test
+    // (over all cells and over each mask word's 64)
     s_rmax2 = (float)(2.0 * m) * 1.0001f;
+    for (int wd = 0; wd < kCells / 64; ++wd) s_wlim[wd] = (float)(2.0 * wm[wd]) * 1.0001f;
     if (2.0 * m > Rg) {
       P.need[u] = 2.0 * m;
       P.status[u] = ST_WIDEN;
@@ -1374,7 +1382,7 @@ __device__ void anchor_work(const Params &P, const Work &w, uint32_t u) {
     for (uint32_t r = 0; r < 4; ++r) {
       mem[r] = false;
       if (r < chunk && t * chunk + r < g) {
-        mem[r] = member_mask(P, w, b0 + r, rmax2, mk[r]);
+        mem[r] = member_mask(P, w, b0 + r, rmax2, s_wlim, mk[r]);
         cnt += mem[r];
       }
     }
