@@ -2355,8 +2355,10 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
         // Launches in chunks of anchors of similar gather size (ascending), each
         // chunk's slabs sized by its own largest gather within one scratch
         // budget: a few anchors that gather most of the cloud no longer shrink
-        // every launch to a handful of workgroups.
-        constexpr uint64_t kScratchBudget = 1ull << 30;  // bytes
+        // every launch to a handful of workgroups.  The budget (8 GiB, 1 GiB
+        // in a context with a device capacity) keeps hundreds of workgroups
+        // per launch even for gathers of 100 000 points (~10 MB of slab each).
+        const uint64_t kScratchBudget = c.capacity_bytes > 0 ? (1ull << 30) : (8ull << 30);  // bytes
         auto slab_of = [](uint32_t g) { return big_fc_offset(g) + 2ull * g; };
         auto launch_big = [&](const std::vector<uint32_t> &list_in, int probe) {
           std::vector<uint32_t> list(list_in);
