@@ -312,7 +312,7 @@ def traffic_from_profile():
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=50_000)
     ap.add_argument("--kmax", type=int, default=10)
