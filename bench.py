@@ -309,6 +309,23 @@ def traffic_from_profile():
         return json.load(fh).get("bytes_per_launch")
 
 
+def mixture_catalogues() -> None:
+    """Catalogue time and record counts of three served-stream mixtures
+    (6 467, 111 721 and 195 898 points; K_max 10, enumeration budget 640):
+    the workload of the big-anchor paths' kernel statistics."""
+    ctx = mhgp.Context(0)
+    ctx.set_region_limit(SERVE_REGION_LIMIT)
+    for n, seed in [(6467, 1012), (111721, 1005), (195898, 1017)]:
+        pts = c5_cloud(n, seed)
+        a = time.perf_counter()
+        ctx.upload(pts)
+        ctx.emst()
+        info = ctx.supports(10, fetch=False)
+        print(f"mixture n={n}: catalogue {1e3 * (time.perf_counter() - a):.0f} ms status {info.status} "
+              f"records {info.event_count} open {info.anchors_open} largest {info.largest_open_region}", flush=True)
+    ctx.close()
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -319,11 +336,16 @@ def main() -> None:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-served", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-cloud batch leg")
+    ap.add_argument("--mixture-catalogues", action="store_true",
+                    help="only time the catalogues of three served-stream mixtures (big-anchor paths)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain kernel of known traffic (counter scaling)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.mixture_catalogues:
+        mixture_catalogues()
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
