@@ -2358,7 +2358,15 @@ void supports_device(Context &c, int k_max, std::vector<EventRec> &events,
         // every launch to a handful of workgroups.  The budget (8 GiB, 1 GiB
         // in a context with a device capacity) keeps hundreds of workgroups
         // per launch even for gathers of 100 000 points (~10 MB of slab each).
-        const uint64_t kScratchBudget = c.capacity_bytes > 0 ? (1ull << 30) : (8ull << 30);  // bytes
+        // The slabs are reused from one chunk to the next.
+        uint64_t kScratchBudget = c.capacity_bytes > 0 ? (1ull << 30) : (8ull << 30);  // bytes
+        if (c.capacity_bytes <= 0) {
+          // and at most a quarter of the device memory still free (other
+          // contexts on the device keep theirs), but never below 1 GiB
+          size_t free_b = 0, total_b = 0;
+          MHGP_HIP(hipMemGetInfo(&free_b, &total_b));
+          kScratchBudget = std::max<uint64_t>(1ull << 30, std::min<uint64_t>(kScratchBudget, free_b / 4));
+        }
         auto slab_of = [](uint32_t g) { return big_fc_offset(g) + 2ull * g; };
         auto launch_big = [&](const std::vector<uint32_t> &list_in, int probe) {
           std::vector<uint32_t> list(list_in);
